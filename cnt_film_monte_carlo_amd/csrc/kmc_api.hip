@@ -1,0 +1,840 @@
+// kmc_api.hip -- extern "C" entry points of libcntkmc.so (declared in include/cntkmc.h).
+//
+// One kmc_ctx owns one HIP stream, two timing events and every device buffer of one KMC
+// problem: sites, cell list, rate tables, CSR, excitons, MSD records and trace.  Inputs are
+// caller-owned host arrays copied in; outputs go to caller-provided host arrays.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cntkmc.h"
+#include "kmc_internal.h"
+#include "kmc_kernels.h"
+#include "philox.h"
+
+using namespace cntkmc;
+
+namespace {
+thread_local std::string g_last_error;
+
+template <typename T>
+struct dbuf {
+  T* p = nullptr;
+  size_t n = 0;
+};
+}  // namespace
+
+struct kmc_ctx {
+  kmc_config cfg{};
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+
+  // sites
+  int64_t n_sites = 0;
+  std::vector<double> h_pos[3], h_ori[3];
+  std::vector<uint8_t> h_chir, h_flags;
+  std::vector<int32_t> h_left, h_right, h_src;
+  double bb_lo[3] = {0, 0, 0}, bb_hi[3] = {0, 0, 0};
+  dbuf<double> d_pos[3], d_ori[3];
+  dbuf<uint8_t> d_chir, d_flags;
+  dbuf<int32_t> d_left, d_right, d_src;
+
+  // cells
+  bool cells_ok = false;
+  cell_desc cells{};
+  int64_t n_cells = 0;
+  dbuf<int32_t> d_cell_of, d_cell_sites;
+  dbuf<uint32_t> d_cell_count;
+  dbuf<int64_t> d_cell_start, d_scan_tmp;
+
+  // rate tables
+  bool table_ok = false;
+  int32_t tab_n[5] = {0, 0, 0, 0, 0};  // n_chir, n_th, n_z, n_a1, n_a2
+  dbuf<double> d_tab_rates, d_tab_grid;
+
+  // CSR
+  int csr_state = 0;  // 0 none, 1 waiting for pair rates, 2 ready
+  int64_t nnz = 0;
+  dbuf<uint32_t> d_deg;
+  dbuf<int64_t> d_row_ptr;
+  dbuf<site_hdr> d_hdr;
+  dbuf<double> d_cum, d_pair_rates;
+  dbuf<edge_rec> d_edge;
+  dbuf<double> d_geo[4];
+  dbuf<int32_t> d_err;
+
+  // excitons
+  int64_t n_exc = 0;
+  dbuf<int32_t> x_site;
+  dbuf<double> x_t, x_dx, x_dy, x_dz;
+  dbuf<uint64_t> x_hops, x_hash;
+  dbuf<uint32_t> x_drains, x_krec;
+  dbuf<uint8_t> x_heading;
+  dbuf<uint32_t> d_site_count;
+
+  // MSD records
+  int32_t n_bins = 0;
+  double t_sample_max = 0.0;
+  dbuf<double> r_dx, r_dy, r_dz, d_reduced;
+  dbuf<uint32_t> r_drain;
+
+  // trace
+  int64_t n_trace = 0, tr_max = 0;
+  dbuf<int32_t> tr_site;
+  dbuf<double> tr_time;
+  dbuf<int64_t> tr_n;
+};
+
+namespace {
+
+int fail(kmc_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  g_last_error = msg;
+  return code;
+}
+
+int hip_fail(kmc_ctx* c, hipError_t e, const char* where) {
+  std::string m = std::string(where) + ": " + hipGetErrorString(e);
+  return fail(c, KMC_ERR_HIP, m);
+}
+
+#define KMC_HIP(c, call)                                    \
+  do {                                                      \
+    hipError_t e_ = (call);                                 \
+    if (e_ != hipSuccess) return hip_fail((c), e_, #call);  \
+  } while (0)
+
+template <typename T>
+void dfree(dbuf<T>& b) {
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.n = 0;
+}
+
+template <typename T>
+int dalloc(kmc_ctx* c, dbuf<T>& b, size_t n) {
+  if (b.p && b.n == n) return KMC_OK;
+  dfree(b);
+  if (n == 0) return KMC_OK;
+  hipError_t e = hipMalloc((void**)&b.p, n * sizeof(T));
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    return fail(c, KMC_ERR_ALLOC, "hipMalloc of " + std::to_string(n * sizeof(T)) +
+                                      " bytes failed: " + hipGetErrorString(e));
+  }
+  b.n = n;
+  return KMC_OK;
+}
+
+#define KMC_TRY(expr)            \
+  do {                           \
+    int s_ = (expr);             \
+    if (s_ != KMC_OK) return s_; \
+  } while (0)
+
+site_arrays sites_of(kmc_ctx* c) {
+  site_arrays s;
+  s.n = c->n_sites;
+  s.px = c->d_pos[0].p;
+  s.py = c->d_pos[1].p;
+  s.pz = c->d_pos[2].p;
+  s.ox = c->d_ori[0].p;
+  s.oy = c->d_ori[1].p;
+  s.oz = c->d_ori[2].p;
+  s.chir = c->d_chir.p;
+  s.flags = c->d_flags.p;
+  return s;
+}
+
+csr_arrays csr_of(kmc_ctx* c) {
+  csr_arrays a;
+  a.hdr = c->d_hdr.p;
+  a.cum = c->d_cum.p;
+  a.edge = c->d_edge.p;
+  return a;
+}
+
+exciton_arrays excitons_of(kmc_ctx* c) {
+  exciton_arrays x;
+  x.n = c->n_exc;
+  x.site = c->x_site.p;
+  x.t_next = c->x_t.p;
+  x.hops = c->x_hops.p;
+  x.dx = c->x_dx.p;
+  x.dy = c->x_dy.p;
+  x.dz = c->x_dz.p;
+  x.drains = c->x_drains.p;
+  x.hash = c->x_hash.p;
+  x.k_rec = c->x_krec.p;
+  x.heading = c->x_heading.p;
+  return x;
+}
+
+void reset_downstream(kmc_ctx* c, int level) {
+  // level 0: sites changed; 1: cells/tables changed; 2: CSR changed
+  if (level <= 0) c->cells_ok = false;
+  if (level <= 1) c->csr_state = 0;
+  c->n_exc = 0;
+  c->n_bins = 0;
+  c->n_trace = 0;
+}
+
+int copy_in(kmc_ctx* c, void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return KMC_OK;
+  KMC_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  return KMC_OK;
+}
+
+int copy_out(kmc_ctx* c, void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return KMC_OK;
+  KMC_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  KMC_HIP(c, hipStreamSynchronize(c->stream));
+  return KMC_OK;
+}
+
+int check_err_flag(kmc_ctx* c, const char* what) {
+  int32_t h = 0;
+  KMC_TRY(copy_out(c, &h, c->d_err.p, sizeof(int32_t)));
+  if (h != 0) return fail(c, KMC_ERR_RATE, std::string(what) + ": negative, NaN or infinite rate");
+  return KMC_OK;
+}
+
+}  // namespace
+
+// =============================================================================================
+extern "C" {
+
+const char* kmc_version(void) { return "cntkmc 0.1 (gfx950)"; }
+
+const char* kmc_last_error(const kmc_ctx* ctx) {
+  return ctx ? ctx->err.c_str() : g_last_error.c_str();
+}
+
+int kmc_create(const kmc_config* cfg, kmc_ctx** out) {
+  if (!cfg || !out) return fail(nullptr, KMC_ERR_ARG, "kmc_create: null argument");
+  *out = nullptr;
+  if (!(cfg->cutoff > 0.0) || !std::isfinite(cfg->cutoff) || !(cfg->min_distance >= 0.0) ||
+      !(cfg->min_distance < cfg->cutoff))
+    return fail(nullptr, KMC_ERR_ARG, "kmc_create: need 0 <= min_distance < cutoff < inf");
+  for (int a = 0; a < 3; ++a) {
+    if (cfg->periodic_mask & (1 << a)) {
+      if (!(cfg->period_len[a] > 2.0 * cfg->cutoff) || !std::isfinite(cfg->period_len[a]) ||
+          !std::isfinite(cfg->period_lo[a]))
+        return fail(nullptr, KMC_ERR_GEOMETRY,
+                    "kmc_create: a periodic axis needs period_len > 2 * cutoff");
+    }
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+    return fail(nullptr, KMC_ERR_NODEVICE, "kmc_create: no HIP device available");
+  if (cfg->device < 0 || cfg->device >= ndev)
+    return fail(nullptr, KMC_ERR_ARG, "kmc_create: device ordinal out of range");
+  kmc_ctx* c = new kmc_ctx();
+  c->cfg = *cfg;
+  hipError_t e = hipSetDevice(cfg->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+  if (e == hipSuccess) e = hipMalloc((void**)&c->d_err.p, sizeof(int32_t));
+  if (e != hipSuccess) {
+    int code = hip_fail(nullptr, e, "kmc_create");
+    kmc_destroy(c);
+    return code;
+  }
+  c->d_err.n = 1;
+  *out = c;
+  return KMC_OK;
+}
+
+int kmc_destroy(kmc_ctx* c) {
+  if (!c) return KMC_OK;
+  (void)hipSetDevice(c->cfg.device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (int a = 0; a < 3; ++a) {
+    dfree(c->d_pos[a]);
+    dfree(c->d_ori[a]);
+  }
+  dfree(c->d_chir); dfree(c->d_flags); dfree(c->d_left); dfree(c->d_right); dfree(c->d_src);
+  dfree(c->d_cell_of); dfree(c->d_cell_sites); dfree(c->d_cell_count); dfree(c->d_cell_start);
+  dfree(c->d_scan_tmp); dfree(c->d_tab_rates); dfree(c->d_tab_grid); dfree(c->d_deg);
+  dfree(c->d_row_ptr); dfree(c->d_hdr); dfree(c->d_cum); dfree(c->d_pair_rates);
+  dfree(c->d_edge);
+  for (int q = 0; q < 4; ++q) dfree(c->d_geo[q]);
+  dfree(c->d_err); dfree(c->x_site); dfree(c->x_t); dfree(c->x_dx); dfree(c->x_dy);
+  dfree(c->x_dz); dfree(c->x_hops); dfree(c->x_hash); dfree(c->x_drains); dfree(c->x_krec);
+  dfree(c->x_heading); dfree(c->d_site_count); dfree(c->r_dx); dfree(c->r_dy); dfree(c->r_dz);
+  dfree(c->d_reduced); dfree(c->r_drain); dfree(c->tr_site); dfree(c->tr_time); dfree(c->tr_n);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return KMC_OK;
+}
+
+int kmc_sync(kmc_ctx* c) {
+  if (!c) return fail(nullptr, KMC_ERR_ARG, "kmc_sync: null context");
+  KMC_HIP(c, hipStreamSynchronize(c->stream));
+  return KMC_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+int kmc_set_sites(kmc_ctx* c, int64_t n, const double* x, const double* y, const double* z,
+                  const double* ox, const double* oy, const double* oz, const int32_t* left,
+                  const int32_t* right, const uint8_t* chirality, const uint8_t* flags) {
+  if (!c) return fail(nullptr, KMC_ERR_ARG, "kmc_set_sites: null context");
+  if (n <= 0 || n >= (int64_t)EDGE_COL_MASK || !x || !y || !z || !ox || !oy || !oz)
+    return fail(c, KMC_ERR_ARG, "kmc_set_sites: need 0 < n < 2^31 and non-null x,y,z,ox,oy,oz");
+  KMC_HIP(c, hipSetDevice(c->cfg.device));
+  const double* pin[3] = {x, y, z};
+  const double* oin[3] = {ox, oy, oz};
+  for (int a = 0; a < 3; ++a) {
+    c->h_pos[a].assign(pin[a], pin[a] + n);
+    c->h_ori[a].assign(oin[a], oin[a] + n);
+  }
+  c->h_left.assign(n, -1);
+  c->h_right.assign(n, -1);
+  c->h_chir.assign(n, 0);
+  c->h_flags.assign(n, 0);
+  if (left) c->h_left.assign(left, left + n);
+  if (right) c->h_right.assign(right, right + n);
+  if (chirality) c->h_chir.assign(chirality, chirality + n);
+  if (flags) c->h_flags.assign(flags, flags + n);
+  for (int64_t i = 0; i < n; ++i) {
+    for (int a = 0; a < 3; ++a) {
+      const double p = c->h_pos[a][i];
+      if (!std::isfinite(p) || !std::isfinite(c->h_ori[a][i]))
+        return fail(c, KMC_ERR_ARG, "kmc_set_sites: non-finite position or orientation");
+      if (c->cfg.periodic_mask & (1 << a)) {
+        if (!(p >= c->cfg.period_lo[a] && p < c->cfg.period_lo[a] + c->cfg.period_len[a]))
+          return fail(c, KMC_ERR_GEOMETRY,
+                      "kmc_set_sites: periodic coordinate outside [period_lo, period_lo+len)");
+      }
+    }
+    if (c->h_left[i] < -1 || c->h_left[i] >= n || c->h_right[i] < -1 || c->h_right[i] >= n)
+      return fail(c, KMC_ERR_ARG, "kmc_set_sites: link index out of range");
+    if (c->h_chir[i] >= MAX_CHIR)
+      return fail(c, KMC_ERR_ARG, "kmc_set_sites: chirality index too large");
+  }
+  // bounding box (find_simulation_domain, monte_carlo.h:328-340)
+  for (int a = 0; a < 3; ++a) {
+    c->bb_lo[a] = c->bb_hi[a] = c->h_pos[a][0];
+    for (int64_t i = 0; i < n; ++i) {
+      const double p = c->h_pos[a][i];
+      c->bb_lo[a] = c->bb_lo[a] > p ? p : c->bb_lo[a];
+      c->bb_hi[a] = c->bb_hi[a] < p ? p : c->bb_hi[a];
+    }
+  }
+  c->h_src.clear();
+  for (int64_t i = 0; i < n; ++i)
+    if (c->h_flags[i] & KMC_SITE_SOURCE) c->h_src.push_back((int32_t)i);
+  c->n_sites = n;
+  for (int a = 0; a < 3; ++a) {
+    KMC_TRY(dalloc(c, c->d_pos[a], n));
+    KMC_TRY(dalloc(c, c->d_ori[a], n));
+    KMC_TRY(copy_in(c, c->d_pos[a].p, c->h_pos[a].data(), n * sizeof(double)));
+    KMC_TRY(copy_in(c, c->d_ori[a].p, c->h_ori[a].data(), n * sizeof(double)));
+  }
+  KMC_TRY(dalloc(c, c->d_chir, n));
+  KMC_TRY(dalloc(c, c->d_flags, n));
+  KMC_TRY(dalloc(c, c->d_left, n));
+  KMC_TRY(dalloc(c, c->d_right, n));
+  KMC_TRY(copy_in(c, c->d_chir.p, c->h_chir.data(), n));
+  KMC_TRY(copy_in(c, c->d_flags.p, c->h_flags.data(), n));
+  KMC_TRY(copy_in(c, c->d_left.p, c->h_left.data(), n * sizeof(int32_t)));
+  KMC_TRY(copy_in(c, c->d_right.p, c->h_right.data(), n * sizeof(int32_t)));
+  KMC_TRY(dalloc(c, c->d_src, c->h_src.size()));
+  KMC_TRY(copy_in(c, c->d_src.p, c->h_src.data(), c->h_src.size() * sizeof(int32_t)));
+  KMC_HIP(c, hipStreamSynchronize(c->stream));
+  reset_downstream(c, 0);
+  return KMC_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+int kmc_build_cells(kmc_ctx* c) {
+  if (!c) return fail(nullptr, KMC_ERR_ARG, "kmc_build_cells: null context");
+  if (c->n_sites == 0) return fail(c, KMC_ERR_STATE, "kmc_build_cells: call kmc_set_sites first");
+  KMC_HIP(c, hipSetDevice(c->cfg.device));
+  cell_desc& d = c->cells;
+  d = cell_desc{};
+  d.periodic_mask = c->cfg.periodic_mask;
+  d.cutoff = c->cfg.cutoff;
+  d.min_distance = c->cfg.min_distance;
+  int32_t dims[3];
+  for (int a = 0; a < 3; ++a) {
+    if (d.periodic_mask & (1 << a)) {
+      const int n = (int)std::floor(c->cfg.period_len[a] / c->cfg.cutoff);
+      dims[a] = n < 1 ? 1 : n;
+      d.lo[a] = c->cfg.period_lo[a];
+      d.period[a] = c->cfg.period_len[a];
+      d.cell[a] = c->cfg.period_len[a] / dims[a];
+    } else {
+      // n = ceil(extent / radius) + 1 (monte_carlo.h:377-385)
+      const double ext = (c->bb_hi[a] - c->bb_lo[a]) / c->cfg.cutoff;
+      if (!(ext < 1.0e9)) return fail(c, KMC_ERR_GEOMETRY, "kmc_build_cells: domain too large");
+      dims[a] = (int)std::ceil(ext) + 1;
+      d.lo[a] = c->bb_lo[a];
+      d.cell[a] = c->cfg.cutoff;
+      d.period[a] = 0.0;
+    }
+  }
+  d.nx = dims[0];
+  d.ny = dims[1];
+  d.nz = dims[2];
+  const int64_t n_cells = (int64_t)d.nx * d.ny * d.nz;
+  if (n_cells >= (int64_t)1 << 31)
+    return fail(c, KMC_ERR_GEOMETRY, "kmc_build_cells: more than 2^31 cells");
+  c->n_cells = n_cells;
+  const int64_t n = c->n_sites;
+  KMC_TRY(dalloc(c, c->d_cell_of, n));
+  KMC_TRY(dalloc(c, c->d_cell_sites, n));
+  KMC_TRY(dalloc(c, c->d_cell_count, n_cells));
+  KMC_TRY(dalloc(c, c->d_cell_start, n_cells + 1));
+  KMC_TRY(dalloc(c, c->d_scan_tmp, std::max(scan_tmp_elems(n_cells), scan_tmp_elems(n))));
+  KMC_HIP(c, hipMemsetAsync(c->d_cell_count.p, 0, n_cells * sizeof(uint32_t), c->stream));
+  site_arrays s = sites_of(c);
+  KMC_HIP(c, launch_cell_assign(s, d, c->d_cell_of.p, c->d_cell_count.p, c->stream));
+  KMC_HIP(c, exclusive_scan(c->d_cell_count.p, c->d_cell_start.p, n_cells, c->d_scan_tmp.p,
+                            c->stream));
+  KMC_HIP(c, hipMemsetAsync(c->d_cell_count.p, 0, n_cells * sizeof(uint32_t), c->stream));
+  KMC_HIP(c, launch_cell_scatter(n, c->d_cell_of.p, c->d_cell_start.p, c->d_cell_count.p,
+                                 c->d_cell_sites.p, c->stream));
+  KMC_HIP(c, launch_cell_sort(n_cells, c->d_cell_start.p, c->d_cell_sites.p, c->stream));
+  KMC_HIP(c, hipStreamSynchronize(c->stream));
+  c->cells_ok = true;
+  reset_downstream(c, 1);
+  return KMC_OK;
+}
+
+int kmc_get_cells(kmc_ctx* c, int32_t dims[3], int64_t* n_cells, int64_t* cell_start,
+                  int32_t* cell_sites) {
+  if (!c) return fail(nullptr, KMC_ERR_ARG, "kmc_get_cells: null context");
+  if (!c->cells_ok) return fail(c, KMC_ERR_STATE, "kmc_get_cells: cells not built");
+  if (dims) {
+    dims[0] = c->cells.nx;
+    dims[1] = c->cells.ny;
+    dims[2] = c->cells.nz;
+  }
+  if (n_cells) *n_cells = c->n_cells;
+  if (cell_start)
+    KMC_TRY(copy_out(c, cell_start, c->d_cell_start.p, (c->n_cells + 1) * sizeof(int64_t)));
+  if (cell_sites)
+    KMC_TRY(copy_out(c, cell_sites, c->d_cell_sites.p, c->n_sites * sizeof(int32_t)));
+  return KMC_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+int kmc_set_rate_table(kmc_ctx* c, int32_t n_chir, int32_t n_theta, const double* theta,
+                       int32_t n_z, const double* z_shift, int32_t n_a1, const double* a1,
+                       int32_t n_a2, const double* a2, const double* rates) {
+  if (!c) return fail(nullptr, KMC_ERR_ARG, "kmc_set_rate_table: null context");
+  if (n_chir < 1 || n_chir > MAX_CHIR || n_theta < 1 || n_z < 1 || n_a1 < 1 || n_a2 < 1 ||
+      !theta || !z_shift || !a1 || !a2 || !rates)
+    return fail(c, KMC_ERR_ARG, "kmc_set_rate_table: bad sizes or null arrays");
+  KMC_HIP(c, hipSetDevice(c->cfg.device));
+  const int64_t nr = (int64_t)n_chir * n_chir * n_theta * n_z * n_a1 * n_a2;
+  for (int64_t i = 0; i < nr; ++i)
+    if (!(rates[i] >= 0.0) || !std::isfinite(rates[i]))
+      return fail(c, KMC_ERR_RATE, "kmc_set_rate_table: negative, NaN or infinite rate");
+  std::vector<double> grid;
+  grid.insert(grid.end(), theta, theta + n_theta);
+  grid.insert(grid.end(), z_shift, z_shift + n_z);
+  grid.insert(grid.end(), a1, a1 + n_a1);
+  grid.insert(grid.end(), a2, a2 + n_a2);
+  KMC_TRY(dalloc(c, c->d_tab_grid, grid.size()));
+  KMC_TRY(dalloc(c, c->d_tab_rates, nr));
+  KMC_TRY(copy_in(c, c->d_tab_grid.p, grid.data(), grid.size() * sizeof(double)));
+  KMC_TRY(copy_in(c, c->d_tab_rates.p, rates, nr * sizeof(double)));
+  KMC_HIP(c, hipStreamSynchronize(c->stream));
+  c->tab_n[0] = n_chir;
+  c->tab_n[1] = n_theta;
+  c->tab_n[2] = n_z;
+  c->tab_n[3] = n_a1;
+  c->tab_n[4] = n_a2;
+  c->table_ok = true;
+  return KMC_OK;
+}
+
+int kmc_build_rates(kmc_ctx* c, int32_t mode) {
+  if (!c) return fail(nullptr, KMC_ERR_ARG, "kmc_build_rates: null context");
+  if (mode != KMC_RATE_TABLE && mode != KMC_RATE_PAIR)
+    return fail(c, KMC_ERR_ARG, "kmc_build_rates: unknown mode");
+  if (!c->cells_ok) return fail(c, KMC_ERR_STATE, "kmc_build_rates: call kmc_build_cells first");
+  if (mode == KMC_RATE_TABLE) {
+    if (!c->table_ok) return fail(c, KMC_ERR_STATE, "kmc_build_rates: no rate table set");
+    for (int64_t i = 0; i < c->n_sites; ++i)
+      if (c->h_chir[i] >= c->tab_n[0])
+        return fail(c, KMC_ERR_ARG, "kmc_build_rates: site chirality outside the table set");
+  }
+  KMC_HIP(c, hipSetDevice(c->cfg.device));
+  const int64_t n = c->n_sites;
+  site_arrays s = sites_of(c);
+  KMC_TRY(dalloc(c, c->d_deg, n));
+  KMC_TRY(dalloc(c, c->d_row_ptr, n + 1));
+  KMC_HIP(c, launch_count_pairs(s, c->cells, c->d_cell_start.p, c->d_cell_sites.p, c->d_deg.p,
+                                c->stream));
+  KMC_HIP(c, exclusive_scan(c->d_deg.p, c->d_row_ptr.p, n, c->d_scan_tmp.p, c->stream));
+  int64_t nnz = 0;
+  KMC_TRY(copy_out(c, &nnz, c->d_row_ptr.p + n, sizeof(int64_t)));
+  if (nnz >= (int64_t)0xFFFFFFFFll)
+    return fail(c, KMC_ERR_CAPACITY, "kmc_build_rates: more than 2^32 pairs");
+  c->nnz = nnz;
+  KMC_TRY(dalloc(c, c->d_hdr, n));
+  KMC_TRY(dalloc(c, c->d_cum, std::max<int64_t>(nnz, 1)));
+  KMC_TRY(dalloc(c, c->d_edge, std::max<int64_t>(nnz, 1)));
+  pair_geom_arrays geo{nullptr, nullptr, nullptr, nullptr};
+  if (mode == KMC_RATE_PAIR) {
+    for (int q = 0; q < 4; ++q) KMC_TRY(dalloc(c, c->d_geo[q], std::max<int64_t>(nnz, 1)));
+    geo.theta = c->d_geo[0].p;
+    geo.z = c->d_geo[1].p;
+    geo.a1 = c->d_geo[2].p;
+    geo.a2 = c->d_geo[3].p;
+  }
+  table_desc t{};
+  if (c->table_ok) {
+    t.n_chir = c->tab_n[0];
+    t.n_th = c->tab_n[1];
+    t.n_z = c->tab_n[2];
+    t.n_a1 = c->tab_n[3];
+    t.n_a2 = c->tab_n[4];
+    t.rates = c->d_tab_rates.p;
+    t.th = c->d_tab_grid.p;
+    t.z = t.th + t.n_th;
+    t.a1 = t.z + t.n_z;
+    t.a2 = t.a1 + t.n_a1;
+  }
+  KMC_HIP(c, hipMemsetAsync(c->d_err.p, 0, sizeof(int32_t), c->stream));
+  KMC_HIP(c, launch_fill_pairs(mode, s, c->cells, c->d_cell_start.p, c->d_cell_sites.p,
+                               c->d_row_ptr.p, t, csr_of(c), geo, c->d_err.p, c->stream));
+  KMC_TRY(check_err_flag(c, "kmc_build_rates"));
+  c->csr_state = (mode == KMC_RATE_TABLE) ? 2 : 1;
+  reset_downstream(c, 2);
+  return KMC_OK;
+}
+
+int kmc_get_nnz(kmc_ctx* c, int64_t* nnz) {
+  if (!c || !nnz) return fail(c, KMC_ERR_ARG, "kmc_get_nnz: null argument");
+  if (c->csr_state == 0) return fail(c, KMC_ERR_STATE, "kmc_get_nnz: no CSR built");
+  *nnz = c->nnz;
+  return KMC_OK;
+}
+
+int kmc_get_pair_geometry(kmc_ctx* c, double* theta, double* z_shift, double* a1, double* a2,
+                          uint8_t* chir_src, uint8_t* chir_dst) {
+  if (!c) return fail(nullptr, KMC_ERR_ARG, "kmc_get_pair_geometry: null context");
+  if (c->csr_state == 0 || !c->d_geo[0].p)
+    return fail(c, KMC_ERR_STATE, "kmc_get_pair_geometry: build with KMC_RATE_PAIR first");
+  double* outs[4] = {theta, z_shift, a1, a2};
+  for (int q = 0; q < 4; ++q)
+    if (outs[q]) KMC_TRY(copy_out(c, outs[q], c->d_geo[q].p, c->nnz * sizeof(double)));
+  if (chir_src || chir_dst) {
+    std::vector<site_hdr> hdr(c->n_sites);
+    std::vector<edge_rec> edge(c->nnz);
+    KMC_TRY(copy_out(c, hdr.data(), c->d_hdr.p, c->n_sites * sizeof(site_hdr)));
+    KMC_TRY(copy_out(c, edge.data(), c->d_edge.p, c->nnz * sizeof(edge_rec)));
+    for (int64_t i = 0; i < c->n_sites; ++i) {
+      for (uint32_t k = 0; k < hdr[i].n; ++k) {
+        const int64_t p = hdr[i].rs + k;
+        if (chir_src) chir_src[p] = c->h_chir[i];
+        if (chir_dst) chir_dst[p] = c->h_chir[edge[p].col & EDGE_COL_MASK];
+      }
+    }
+  }
+  return KMC_OK;
+}
+
+int kmc_set_pair_rates_device(kmc_ctx* c, const double* d_rates) {
+  if (!c || !d_rates) return fail(c, KMC_ERR_ARG, "kmc_set_pair_rates: null argument");
+  if (c->csr_state == 0)
+    return fail(c, KMC_ERR_STATE, "kmc_set_pair_rates: build with KMC_RATE_PAIR first");
+  KMC_HIP(c, hipSetDevice(c->cfg.device));
+  KMC_HIP(c, hipMemsetAsync(c->d_err.p, 0, sizeof(int32_t), c->stream));
+  KMC_HIP(c, launch_finish_pair_rates(c->n_sites, d_rates, csr_of(c), c->d_err.p, c->stream));
+  KMC_TRY(check_err_flag(c, "kmc_set_pair_rates"));
+  c->csr_state = 2;
+  reset_downstream(c, 2);
+  return KMC_OK;
+}
+
+int kmc_set_pair_rates(kmc_ctx* c, const double* rates) {
+  if (!c || !rates) return fail(c, KMC_ERR_ARG, "kmc_set_pair_rates: null argument");
+  if (c->csr_state == 0)
+    return fail(c, KMC_ERR_STATE, "kmc_set_pair_rates: build with KMC_RATE_PAIR first");
+  KMC_HIP(c, hipSetDevice(c->cfg.device));
+  KMC_TRY(dalloc(c, c->d_pair_rates, std::max<int64_t>(c->nnz, 1)));
+  KMC_TRY(copy_in(c, c->d_pair_rates.p, rates, c->nnz * sizeof(double)));
+  return kmc_set_pair_rates_device(c, c->d_pair_rates.p);
+}
+
+int kmc_get_csr(kmc_ctx* c, int64_t* row_ptr, int32_t* col, double* cum, float* disp,
+                double* gamma) {
+  if (!c) return fail(nullptr, KMC_ERR_ARG, "kmc_get_csr: null context");
+  if (c->csr_state != 2) return fail(c, KMC_ERR_STATE, "kmc_get_csr: CSR not complete");
+  KMC_HIP(c, hipSetDevice(c->cfg.device));
+  const int64_t n = c->n_sites, nnz = c->nnz;
+  std::vector<site_hdr> hdr(n);
+  KMC_TRY(copy_out(c, hdr.data(), c->d_hdr.p, n * sizeof(site_hdr)));
+  if (row_ptr) {
+    for (int64_t i = 0; i < n; ++i) row_ptr[i] = hdr[i].rs;
+    row_ptr[n] = nnz;
+  }
+  if (gamma)
+    for (int64_t i = 0; i < n; ++i) gamma[i] = hdr[i].gamma;
+  if (cum) KMC_TRY(copy_out(c, cum, c->d_cum.p, nnz * sizeof(double)));
+  if (col || disp) {
+    std::vector<edge_rec> edge(nnz);
+    KMC_TRY(copy_out(c, edge.data(), c->d_edge.p, nnz * sizeof(edge_rec)));
+    for (int64_t p = 0; p < nnz; ++p) {
+      if (col) col[p] = (int32_t)(edge[p].col & EDGE_COL_MASK);
+      if (disp) {
+        disp[3 * p + 0] = edge[p].dx;
+        disp[3 * p + 1] = edge[p].dy;
+        disp[3 * p + 2] = edge[p].dz;
+      }
+    }
+  }
+  return KMC_OK;
+}
+
+int kmc_set_csr(kmc_ctx* c, int64_t n_sites, const int64_t* row_ptr, const int32_t* col,
+                const double* cum, const float* disp) {
+  if (!c || !row_ptr || !col || !cum || !disp)
+    return fail(c, KMC_ERR_ARG, "kmc_set_csr: null argument");
+  if (n_sites != c->n_sites || n_sites == 0)
+    return fail(c, KMC_ERR_STATE, "kmc_set_csr: site count differs from kmc_set_sites");
+  KMC_HIP(c, hipSetDevice(c->cfg.device));
+  const int64_t nnz = row_ptr[n_sites];
+  if (row_ptr[0] != 0 || nnz < 0 || nnz >= (int64_t)0xFFFFFFFFll)
+    return fail(c, KMC_ERR_ARG, "kmc_set_csr: bad row_ptr");
+  std::vector<site_hdr> hdr(n_sites);
+  std::vector<edge_rec> edge(std::max<int64_t>(nnz, 1));
+  for (int64_t i = 0; i < n_sites; ++i) {
+    if (row_ptr[i + 1] < row_ptr[i]) return fail(c, KMC_ERR_ARG, "kmc_set_csr: row_ptr decreases");
+    hdr[i].rs = (uint32_t)row_ptr[i];
+    hdr[i].n = (uint32_t)(row_ptr[i + 1] - row_ptr[i]);
+    hdr[i].gamma = hdr[i].n ? cum[row_ptr[i + 1] - 1] : 0.0;
+    double prev = 0.0;
+    for (int64_t p = row_ptr[i]; p < row_ptr[i + 1]; ++p) {
+      if (col[p] < 0 || col[p] >= n_sites) return fail(c, KMC_ERR_ARG, "kmc_set_csr: bad column");
+      if (!(cum[p] >= prev) || !std::isfinite(cum[p]))
+        return fail(c, KMC_ERR_RATE, "kmc_set_csr: cumulative rates must be finite and rise");
+      prev = cum[p];
+      edge[p].col = (uint32_t)col[p] | ((c->h_flags[col[p]] & KMC_SITE_DRAIN) ? EDGE_DRAIN_BIT : 0u);
+      edge[p].dx = disp[3 * p + 0];
+      edge[p].dy = disp[3 * p + 1];
+      edge[p].dz = disp[3 * p + 2];
+    }
+  }
+  c->nnz = nnz;
+  KMC_TRY(dalloc(c, c->d_hdr, n_sites));
+  KMC_TRY(dalloc(c, c->d_cum, std::max<int64_t>(nnz, 1)));
+  KMC_TRY(dalloc(c, c->d_edge, std::max<int64_t>(nnz, 1)));
+  KMC_TRY(copy_in(c, c->d_hdr.p, hdr.data(), n_sites * sizeof(site_hdr)));
+  KMC_TRY(copy_in(c, c->d_cum.p, cum, nnz * sizeof(double)));
+  KMC_TRY(copy_in(c, c->d_edge.p, edge.data(), nnz * sizeof(edge_rec)));
+  KMC_HIP(c, hipStreamSynchronize(c->stream));
+  c->csr_state = 2;
+  reset_downstream(c, 2);
+  return KMC_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+int kmc_inject(kmc_ctx* c, int64_t n_excitons) {
+  if (!c) return fail(nullptr, KMC_ERR_ARG, "kmc_inject: null context");
+  if (c->csr_state != 2) return fail(c, KMC_ERR_STATE, "kmc_inject: CSR not complete");
+  if (n_excitons <= 0 || n_excitons > (int64_t)0x7FFFFFFF)
+    return fail(c, KMC_ERR_ARG, "kmc_inject: need 0 < n_excitons < 2^31");
+  if (c->h_src.empty()) return fail(c, KMC_ERR_STATE, "kmc_inject: no source (injection) sites");
+  KMC_HIP(c, hipSetDevice(c->cfg.device));
+  const int64_t E = n_excitons;
+  KMC_TRY(dalloc(c, c->x_site, E));
+  KMC_TRY(dalloc(c, c->x_t, E));
+  KMC_TRY(dalloc(c, c->x_dx, E));
+  KMC_TRY(dalloc(c, c->x_dy, E));
+  KMC_TRY(dalloc(c, c->x_dz, E));
+  KMC_TRY(dalloc(c, c->x_hops, E));
+  KMC_TRY(dalloc(c, c->x_hash, E));
+  KMC_TRY(dalloc(c, c->x_drains, E));
+  KMC_TRY(dalloc(c, c->x_krec, E));
+  KMC_TRY(dalloc(c, c->x_heading, E));
+  KMC_TRY(dalloc(c, c->d_site_count, c->n_sites));
+  KMC_HIP(c, hipMemsetAsync(c->d_site_count.p, 0, c->n_sites * sizeof(uint32_t), c->stream));
+  c->n_exc = E;
+  c->n_bins = 0;
+  c->n_trace = 0;
+  KMC_HIP(c, launch_inject(excitons_of(c), c->d_hdr.p, c->d_src.p, (uint32_t)c->h_src.size(),
+                           c->cfg.seed, c->stream));
+  KMC_HIP(c, hipStreamSynchronize(c->stream));
+  return KMC_OK;
+}
+
+int kmc_set_samples(kmc_ctx* c, int32_t n_bins, double t_sample_max) {
+  if (!c) return fail(nullptr, KMC_ERR_ARG, "kmc_set_samples: null context");
+  if (c->n_exc == 0) return fail(c, KMC_ERR_STATE, "kmc_set_samples: call kmc_inject first");
+  if (n_bins < 0 || (n_bins > 0 && !(t_sample_max > 0.0 && std::isfinite(t_sample_max))))
+    return fail(c, KMC_ERR_ARG, "kmc_set_samples: need n_bins >= 0 and 0 < t_sample_max < inf");
+  KMC_HIP(c, hipSetDevice(c->cfg.device));
+  const size_t nrec = (size_t)n_bins * c->n_exc;
+  KMC_TRY(dalloc(c, c->r_dx, nrec));
+  KMC_TRY(dalloc(c, c->r_dy, nrec));
+  KMC_TRY(dalloc(c, c->r_dz, nrec));
+  KMC_TRY(dalloc(c, c->r_drain, nrec));
+  KMC_HIP(c, hipMemsetAsync(c->x_krec.p, 0, c->n_exc * sizeof(uint32_t), c->stream));
+  KMC_HIP(c, hipStreamSynchronize(c->stream));
+  c->n_bins = n_bins;
+  c->t_sample_max = t_sample_max;
+  return KMC_OK;
+}
+
+int kmc_set_trace(kmc_ctx* c, int64_t n_trace, int64_t max_hops) {
+  if (!c) return fail(nullptr, KMC_ERR_ARG, "kmc_set_trace: null context");
+  if (c->n_exc == 0) return fail(c, KMC_ERR_STATE, "kmc_set_trace: call kmc_inject first");
+  if (n_trace < 0 || max_hops < 0 || n_trace > c->n_exc)
+    return fail(c, KMC_ERR_ARG, "kmc_set_trace: need 0 <= n_trace <= n_excitons, max_hops >= 0");
+  KMC_HIP(c, hipSetDevice(c->cfg.device));
+  KMC_TRY(dalloc(c, c->tr_site, (size_t)n_trace * max_hops));
+  KMC_TRY(dalloc(c, c->tr_time, (size_t)n_trace * max_hops));
+  KMC_TRY(dalloc(c, c->tr_n, n_trace));
+  if (n_trace) KMC_HIP(c, hipMemsetAsync(c->tr_n.p, 0, n_trace * sizeof(int64_t), c->stream));
+  KMC_HIP(c, hipStreamSynchronize(c->stream));
+  c->n_trace = n_trace;
+  c->tr_max = max_hops;
+  return KMC_OK;
+}
+
+int kmc_run(kmc_ctx* c, int64_t hops_per_exciton, double t_max, int32_t count_sites,
+            double* kernel_ms) {
+  if (!c) return fail(nullptr, KMC_ERR_ARG, "kmc_run: null context");
+  if (c->n_exc == 0) return fail(c, KMC_ERR_STATE, "kmc_run: call kmc_inject first");
+  if (hops_per_exciton < 0 || std::isnan(t_max))
+    return fail(c, KMC_ERR_ARG, "kmc_run: need hops_per_exciton >= 0 and t_max not NaN");
+  KMC_HIP(c, hipSetDevice(c->cfg.device));
+  hop_args a{};
+  a.x = excitons_of(c);
+  a.hdr = c->d_hdr.p;
+  a.cum = c->d_cum.p;
+  a.edge = c->d_edge.p;
+  a.src = c->d_src.p;
+  a.n_src = (uint32_t)c->h_src.size();
+  a.seed = c->cfg.seed;
+  a.hop_budget = (uint64_t)hops_per_exciton;
+  a.t_max = t_max;
+  a.n_bins = c->n_bins;
+  a.dt_sample = c->n_bins > 0 ? c->t_sample_max / c->n_bins : 0.0;
+  a.rec_dx = c->r_dx.p;
+  a.rec_dy = c->r_dy.p;
+  a.rec_dz = c->r_dz.p;
+  a.rec_drain = c->r_drain.p;
+  a.site_count = c->d_site_count.p;
+  a.n_trace = c->n_trace;
+  a.tr_max = c->tr_max;
+  a.tr_site = c->tr_site.p;
+  a.tr_time = c->tr_time.p;
+  a.tr_n = c->tr_n.p;
+  KMC_HIP(c, hipEventRecord(c->ev0, c->stream));
+  KMC_HIP(c, launch_hop(a, count_sites != 0, c->n_bins > 0, c->n_trace > 0 && c->tr_max > 0,
+                        c->stream));
+  KMC_HIP(c, hipEventRecord(c->ev1, c->stream));
+  if (kernel_ms) {
+    KMC_HIP(c, hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    KMC_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    *kernel_ms = ms;
+  }
+  return KMC_OK;
+}
+
+int kmc_get_excitons(kmc_ctx* c, int32_t* site, double* t_next, uint64_t* hops, double* disp,
+                     uint32_t* drains, uint64_t* seq_hash) {
+  if (!c) return fail(nullptr, KMC_ERR_ARG, "kmc_get_excitons: null context");
+  if (c->n_exc == 0) return fail(c, KMC_ERR_STATE, "kmc_get_excitons: no excitons");
+  KMC_HIP(c, hipSetDevice(c->cfg.device));
+  const int64_t E = c->n_exc;
+  if (site) KMC_TRY(copy_out(c, site, c->x_site.p, E * sizeof(int32_t)));
+  if (t_next) KMC_TRY(copy_out(c, t_next, c->x_t.p, E * sizeof(double)));
+  if (hops) KMC_TRY(copy_out(c, hops, c->x_hops.p, E * sizeof(uint64_t)));
+  if (drains) KMC_TRY(copy_out(c, drains, c->x_drains.p, E * sizeof(uint32_t)));
+  if (seq_hash) KMC_TRY(copy_out(c, seq_hash, c->x_hash.p, E * sizeof(uint64_t)));
+  if (disp) {
+    std::vector<double> v(E);
+    const dbuf<double>* src[3] = {&c->x_dx, &c->x_dy, &c->x_dz};
+    for (int a = 0; a < 3; ++a) {
+      KMC_TRY(copy_out(c, v.data(), src[a]->p, E * sizeof(double)));
+      for (int64_t e = 0; e < E; ++e) disp[3 * e + a] = v[e];
+    }
+  }
+  return KMC_OK;
+}
+
+int kmc_get_hop_counts(kmc_ctx* c, uint64_t* counts) {
+  if (!c || !counts) return fail(c, KMC_ERR_ARG, "kmc_get_hop_counts: null argument");
+  if (c->n_exc == 0) return fail(c, KMC_ERR_STATE, "kmc_get_hop_counts: no excitons");
+  std::vector<uint32_t> v(c->n_sites);
+  KMC_TRY(copy_out(c, v.data(), c->d_site_count.p, c->n_sites * sizeof(uint32_t)));
+  for (int64_t i = 0; i < c->n_sites; ++i) counts[i] = v[i];
+  return KMC_OK;
+}
+
+int kmc_get_samples(kmc_ctx* c, double* disp, uint32_t* drains, uint32_t* n_recorded) {
+  if (!c) return fail(nullptr, KMC_ERR_ARG, "kmc_get_samples: null context");
+  if (c->n_bins == 0) return fail(c, KMC_ERR_STATE, "kmc_get_samples: no sampling grid");
+  const size_t nrec = (size_t)c->n_bins * c->n_exc;
+  if (disp) {
+    std::vector<double> v(nrec);
+    const dbuf<double>* src[3] = {&c->r_dx, &c->r_dy, &c->r_dz};
+    for (int a = 0; a < 3; ++a) {
+      KMC_TRY(copy_out(c, v.data(), src[a]->p, nrec * sizeof(double)));
+      for (size_t r = 0; r < nrec; ++r) disp[3 * r + a] = v[r];
+    }
+  }
+  if (drains) KMC_TRY(copy_out(c, drains, c->r_drain.p, nrec * sizeof(uint32_t)));
+  if (n_recorded) KMC_TRY(copy_out(c, n_recorded, c->x_krec.p, c->n_exc * sizeof(uint32_t)));
+  return KMC_OK;
+}
+
+int kmc_get_trace(kmc_ctx* c, int32_t* sites, double* times, int64_t* n_hops) {
+  if (!c) return fail(nullptr, KMC_ERR_ARG, "kmc_get_trace: null context");
+  if (c->n_trace == 0) return fail(c, KMC_ERR_STATE, "kmc_get_trace: no trace buffer");
+  const size_t nt = (size_t)c->n_trace * c->tr_max;
+  if (sites) KMC_TRY(copy_out(c, sites, c->tr_site.p, nt * sizeof(int32_t)));
+  if (times) KMC_TRY(copy_out(c, times, c->tr_time.p, nt * sizeof(double)));
+  if (n_hops) KMC_TRY(copy_out(c, n_hops, c->tr_n.p, c->n_trace * sizeof(int64_t)));
+  return KMC_OK;
+}
+
+int kmc_reduce_samples(kmc_ctx* c, int32_t n_batches, double* out, double* kernel_ms) {
+  if (!c || !out) return fail(c, KMC_ERR_ARG, "kmc_reduce_samples: null argument");
+  if (c->n_bins == 0) return fail(c, KMC_ERR_STATE, "kmc_reduce_samples: no sampling grid");
+  if (n_batches < 1 || n_batches > c->n_exc || n_batches > 65535)
+    return fail(c, KMC_ERR_ARG, "kmc_reduce_samples: need 1 <= n_batches <= min(E, 65535)");
+  KMC_HIP(c, hipSetDevice(c->cfg.device));
+  const size_t nout = (size_t)c->n_bins * n_batches * 5;
+  KMC_TRY(dalloc(c, c->d_reduced, nout));
+  reduce_args a{};
+  a.n_exc = c->n_exc;
+  a.n_bins = c->n_bins;
+  a.n_batches = n_batches;
+  a.k_rec = c->x_krec.p;
+  a.rec_dx = c->r_dx.p;
+  a.rec_dy = c->r_dy.p;
+  a.rec_dz = c->r_dz.p;
+  a.rec_drain = c->r_drain.p;
+  a.out = c->d_reduced.p;
+  KMC_HIP(c, hipEventRecord(c->ev0, c->stream));
+  KMC_HIP(c, launch_reduce(a, c->stream));
+  KMC_HIP(c, hipEventRecord(c->ev1, c->stream));
+  KMC_TRY(copy_out(c, out, c->d_reduced.p, nout * sizeof(double)));
+  if (kernel_ms) {
+    float ms = 0.f;
+    KMC_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    *kernel_ms = ms;
+  }
+  return KMC_OK;
+}
+
+}  // extern "C"

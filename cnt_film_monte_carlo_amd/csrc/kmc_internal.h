@@ -1,0 +1,184 @@
+// kmc_internal.h -- device data layout and shared host/device arithmetic of the KMC path.
+//
+// Layout in HBM (all arrays owned by one kmc_ctx):
+//   sites      SoA double px,py,pz, ox,oy,oz [N]; uint8 chir[N], flags[N]; int32 left/right[N]
+//   cells      int32 cell_of[N]; int64 cell_start[n_cells+1]; int32 cell_sites[N]
+//   CSR        site_hdr hdr[N] (16 B: row start, degree, total rate Gamma)
+//              double cum[nnz] (FP64 inclusive prefix sums, reference order)
+//              edge_rec edge[nnz] (16 B: target | drain bit, float3 displacement)
+//   excitons   SoA state [E]; MSD records [n_bins][E]; per-site counts [N]
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+namespace cntkmc {
+
+constexpr double RAND_MAX_D = 2147483647.0;  // glibc RAND_MAX, the divisor of scatterer.cpp:17
+constexpr uint32_t EDGE_DRAIN_BIT = 0x80000000u;
+constexpr uint32_t EDGE_COL_MASK = 0x7FFFFFFFu;
+constexpr uint64_t FNV_OFFSET = 0xcbf29ce484222325ull;
+constexpr uint64_t FNV_PRIME = 0x100000001b3ull;
+constexpr int MAX_CHIR = 16;
+
+struct __align__(16) site_hdr {
+  uint32_t rs;   // first CSR entry of the row
+  uint32_t n;    // out-degree
+  double gamma;  // total out-rate = last cumulative entry (scatterer.h:91)
+};
+
+struct __align__(16) edge_rec {
+  uint32_t col;  // target site | EDGE_DRAIN_BIT when the target is a drain site
+  float dx, dy, dz;
+};
+
+// Rate tables in device memory: grid axes (theta, z_shift, axis_shift_1, axis_shift_2)
+// concatenated in one small array, rates [n_chir*n_chir][n_th][n_z][n_a1][n_a2].
+struct table_desc {
+  int32_t n_chir, n_th, n_z, n_a1, n_a2;
+  const double* rates;
+  const double* th;
+  const double* z;
+  const double* a1;
+  const double* a2;
+};
+
+struct cell_desc {
+  int32_t nx, ny, nz;
+  int32_t periodic_mask;
+  double lo[3];       // origin per axis (bounding-box minimum or period_lo)
+  double cell[3];     // cell edge per axis (cutoff on open axes, period/n on periodic axes)
+  double period[3];   // period length (periodic axes only)
+  double cutoff, min_distance;
+};
+
+// ---------------------------------------------------------------------------------------------
+// FP64 vector arithmetic in Armadillo's order for 3-vectors (op_dot::direct_dot_arma and
+// op_norm::vec_norm_2_direct_std accumulate elements 0 and 2 in one accumulator and element 1
+// in the other, then add the two).  Contraction is off so no FMA changes the rounding.
+// ---------------------------------------------------------------------------------------------
+__host__ __device__ __forceinline__ double dot3(double ax, double ay, double az, double bx,
+                                                double by, double bz) {
+#pragma clang fp contract(off)
+  double v1 = ax * bx;
+  const double v2 = ay * by;
+  v1 = v1 + az * bz;
+  return v1 + v2;
+}
+
+__host__ __device__ __forceinline__ double norm3(double x, double y, double z) {
+#pragma clang fp contract(off)
+  double acc1 = x * x;
+  const double acc2 = y * y;
+  acc1 = acc1 + z * z;
+  return sqrt(acc1 + acc2);
+}
+
+// First index of the minimum of |grid - x| (arma::index_min on abs(grid - x),
+// scattering_struct.h:42-49); NaN x keeps index 0.
+__host__ __device__ __forceinline__ int nearest_index(const double* g, int n, double x) {
+#pragma clang fp contract(off)
+  double best = INFINITY;
+  int idx = 0;
+  for (int i = 0; i < n; ++i) {
+    const double v = fabs(g[i] - x);
+    if (v < best) {
+      best = v;
+      idx = i;
+    }
+  }
+  return idx;
+}
+
+struct pair_geom {
+  double theta, z, a1, a2;
+};
+
+// Pair geometry of scatterer::find_neighbors (src/monte_carlo/scatterer.cpp:44-61).  p_i is the
+// source, p_j the (image of the) target; o_i, o_j unit orientations.
+__host__ __device__ __forceinline__ pair_geom pair_geometry(double pix, double piy, double piz,
+                                                            double pjx, double pjy, double pjz,
+                                                            double oix, double oiy, double oiz,
+                                                            double ojx, double ojy, double ojz) {
+#pragma clang fp contract(off)
+  pair_geom g;
+  const double dx = pix - pjx, dy = piy - pjy, dz = piz - pjz;
+  const double c = dot3(oix, oiy, oiz, ojx, ojy, ojz);
+  if (c == 1.0) {
+    g.a1 = 0.0;
+    const double d = dot3(dx, dy, dz, oix, oiy, oiz);
+    g.a2 = d;
+    g.theta = 0.0;
+    g.z = norm3(dx - d * oix, dy - d * oiy, dz - d * oiz);
+  } else {
+    g.theta = acos(c);
+    const double y1 = dot3(oix, oiy, oiz, dx, dy, dz);
+    const double y2 = dot3(ojx, ojy, ojz, dx, dy, dz);
+    const double s = 1.0 - c * c;
+    g.a1 = (y1 + y2 * c) / s;
+    g.a2 = (y2 + y1 * c) / s;
+    g.z = norm3((g.a1 * oix + pix) - (g.a2 * ojx + pjx), (g.a1 * oiy + piy) - (g.a2 * ojy + pjy),
+                (g.a1 * oiz + piz) - (g.a2 * ojz + pjz));
+  }
+  return g;
+}
+
+__host__ __device__ __forceinline__ double table_rate(const table_desc& t, int ci, int cj,
+                                                      const pair_geom& g) {
+  const int ith = nearest_index(t.th, t.n_th, g.theta);
+  const int iz = nearest_index(t.z, t.n_z, g.z);
+  const int ia1 = nearest_index(t.a1, t.n_a1, g.a1);
+  const int ia2 = nearest_index(t.a2, t.n_a2, g.a2);
+  const int64_t tab = (int64_t)ci * t.n_chir + cj;
+  const int64_t idx = (((tab * t.n_th + ith) * t.n_z + iz) * t.n_a1 + ia1) * t.n_a2 + ia2;
+  return t.rates[idx];
+}
+
+// Minimum image of p_j seen from p_i on one periodic axis.
+__host__ __device__ __forceinline__ double image_of(double pi, double pj, double period) {
+#pragma clang fp contract(off)
+  const double d = pi - pj;
+  const double half = 0.5 * period;
+  if (d > half) return pj + period;
+  if (d < -half) return pj - period;
+  return pj;
+}
+
+// Cell coordinate along one axis.  Open axes follow create_scatterer_buckets
+// (monte_carlo.h:390-392): int((p - min) / radius).  Periodic axes: floor((p - lo)/cell), clamped.
+__host__ __device__ __forceinline__ int cell_coord(const cell_desc& c, int axis, double p) {
+#pragma clang fp contract(off)
+  const int n = axis == 0 ? c.nx : (axis == 1 ? c.ny : c.nz);
+  if (c.periodic_mask & (1 << axis)) {
+    int i = (int)floor((p - c.lo[axis]) / c.cell[axis]);
+    if (i < 0) i = 0;
+    if (i >= n) i = n - 1;
+    return i;
+  }
+  return (int)((p - c.lo[axis]) / c.cell[axis]);
+}
+
+// The up to three stencil coordinates along one axis, in the order i-1, i, i+1 with
+// out-of-range cells dropped on open axes (monte_carlo.h:402-405) and wrapped (duplicates
+// dropped) on periodic axes.  Returns the count.
+__host__ __device__ __forceinline__ int stencil_axis(const cell_desc& c, int axis, int i,
+                                                     int out[3]) {
+  const int n = axis == 0 ? c.nx : (axis == 1 ? c.ny : c.nz);
+  int k = 0;
+  if (c.periodic_mask & (1 << axis)) {
+    for (int d = -1; d <= 1; ++d) {
+      const int w = ((i + d) % n + n) % n;
+      bool dup = false;
+      for (int q = 0; q < k; ++q) dup = dup || (out[q] == w);
+      if (!dup) out[k++] = w;
+    }
+  } else {
+    for (int d = -1; d <= 1; ++d) {
+      const int w = i + d;
+      if (w > -1 && w < n) out[k++] = w;
+    }
+  }
+  return k;
+}
+
+}  // namespace cntkmc

@@ -1,0 +1,96 @@
+// kmc_kernels.h -- argument blocks and launchers of the KMC kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kmc_internal.h"
+
+namespace cntkmc {
+
+struct site_arrays {
+  int64_t n;
+  const double *px, *py, *pz;
+  const double *ox, *oy, *oz;
+  const uint8_t* chir;
+  const uint8_t* flags;
+};
+
+struct csr_arrays {
+  site_hdr* hdr;
+  double* cum;
+  edge_rec* edge;
+};
+
+struct pair_geom_arrays {
+  double *theta, *z, *a1, *a2;
+};
+
+struct exciton_arrays {
+  int64_t n;
+  int32_t* site;
+  double* t_next;  // time of the pending hop
+  uint64_t* hops;  // hops done so far (the generator's hop counter)
+  double *dx, *dy, *dz;
+  uint32_t* drains;
+  uint64_t* hash;
+  uint32_t* k_rec;  // MSD records written so far
+  uint8_t* heading;
+};
+
+struct hop_args {
+  exciton_arrays x;
+  const site_hdr* hdr;
+  const double* cum;
+  const edge_rec* edge;
+  const int32_t* src;
+  uint32_t n_src;
+  uint64_t seed;
+  uint64_t hop_budget;
+  double t_max;
+  // MSD records [n_bins][E]
+  int32_t n_bins;
+  double dt_sample;
+  double *rec_dx, *rec_dy, *rec_dz;
+  uint32_t* rec_drain;
+  // per-site departure counts
+  uint32_t* site_count;
+  // trace [n_trace][tr_max]
+  int64_t n_trace, tr_max;
+  int32_t* tr_site;
+  double* tr_time;
+  int64_t* tr_n;
+};
+
+struct reduce_args {
+  int64_t n_exc;
+  int32_t n_bins, n_batches;
+  const uint32_t* k_rec;
+  const double *rec_dx, *rec_dy, *rec_dz;
+  const uint32_t* rec_drain;
+  double* out;  // [n_bins][n_batches][5]
+};
+
+hipError_t exclusive_scan(const uint32_t* d_in, int64_t* d_out, int64_t n, int64_t* d_tmp,
+                          hipStream_t st);
+int64_t scan_tmp_elems(int64_t n);
+
+hipError_t launch_cell_assign(const site_arrays& s, const cell_desc& c, int32_t* cell_of,
+                              uint32_t* cell_count, hipStream_t st);
+hipError_t launch_cell_scatter(int64_t n, const int32_t* cell_of, const int64_t* cell_start,
+                               uint32_t* cell_fill, int32_t* cell_sites, hipStream_t st);
+hipError_t launch_cell_sort(int64_t n_cells, const int64_t* cell_start, int32_t* cell_sites,
+                            hipStream_t st);
+hipError_t launch_count_pairs(const site_arrays& s, const cell_desc& c, const int64_t* cell_start,
+                              const int32_t* cell_sites, uint32_t* deg, hipStream_t st);
+hipError_t launch_fill_pairs(int mode, const site_arrays& s, const cell_desc& c,
+                             const int64_t* cell_start, const int32_t* cell_sites,
+                             const int64_t* row_ptr, const table_desc& t, const csr_arrays& out,
+                             const pair_geom_arrays& geo, int32_t* err, hipStream_t st);
+hipError_t launch_finish_pair_rates(int64_t n, const double* rates, const csr_arrays& out,
+                                    int32_t* err, hipStream_t st);
+hipError_t launch_inject(const exciton_arrays& x, const site_hdr* hdr, const int32_t* src,
+                         uint32_t n_src, uint64_t seed, hipStream_t st);
+hipError_t launch_hop(const hop_args& a, bool count, bool sample, bool trace, hipStream_t st);
+hipError_t launch_reduce(const reduce_args& a, hipStream_t st);
+
+}  // namespace cntkmc

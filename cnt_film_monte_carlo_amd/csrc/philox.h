@@ -1,0 +1,82 @@
+// philox.h -- Philox4x32-10 counter-based generator (Salmon, Moraes, Dror, Shaw,
+// "Parallel random numbers: as easy as 1, 2, 3", SC'11), usable on host and device.
+//
+// It replaces the reference's global std::rand() stream (src/monte_carlo/scatterer.cpp:17,
+// src/monte_carlo/scatterer.h:76, src/monte_carlo/monte_carlo.cpp:311,331), whose order
+// depends on OpenMP scheduling.  Here every draw is a pure function of
+// (seed, exciton, hop, stream), so results do not depend on launch shape or thread order.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cntkmc {
+
+struct u32x4 {
+  uint32_t v[4];
+};
+
+__host__ __device__ __forceinline__ u32x4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2,
+                                                        uint32_t c3, uint32_t k0, uint32_t k1) {
+  constexpr uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+  constexpr uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = (uint64_t)M0 * c0;
+    const uint64_t p1 = (uint64_t)M1 * c2;
+    const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    const uint32_t n0 = hi1 ^ c1 ^ k0;
+    const uint32_t n2 = hi0 ^ c3 ^ k1;
+    c0 = n0;
+    c1 = lo1;
+    c2 = n2;
+    c3 = lo0;
+    k0 += W0;
+    k1 += W1;
+  }
+  u32x4 out;
+  out.v[0] = c0;
+  out.v[1] = c1;
+  out.v[2] = c2;
+  out.v[3] = c3;
+  return out;
+}
+
+// Draw streams.  Counter = {hop lo, hop hi, exciton, stream}; key = seed.
+enum : uint32_t {
+  STREAM_HOP = 0u,            // per hop: w0 selection, w1 dwell, w2 re-injection, w3 dwell spare
+  STREAM_INIT = 1u,           // injection: w0 site, w1 dwell, w2 heading, w3 dwell spare
+  STREAM_HOP_REDRAW = 2u,     // + k: further dwell candidates after two zero words
+  STREAM_INIT_REDRAW = 0x80000000u,
+  REDRAW_LIMIT = 16u,         // bounded redraw loop; after it the draw is taken as 1
+};
+
+__host__ __device__ __forceinline__ u32x4 draw4(uint64_t seed, uint32_t exciton, uint64_t hop,
+                                                uint32_t stream) {
+  return philox4x32_10((uint32_t)hop, (uint32_t)(hop >> 32), exciton, stream, (uint32_t)seed,
+                       (uint32_t)(seed >> 32));
+}
+
+// 31-bit draw in [0, 2^31-1] = [0, RAND_MAX] of glibc rand().
+__host__ __device__ __forceinline__ uint32_t r31(uint32_t w) { return w >> 1; }
+
+// First non-zero 31-bit draw for a dwell time (ff_time rejects rand()==0, scatterer.h:76-77):
+// candidates w_a, w_b, then words of the redraw streams.
+__host__ __device__ __forceinline__ uint32_t nonzero_draw(uint32_t wa, uint32_t wb, uint64_t seed,
+                                                          uint32_t exciton, uint64_t hop,
+                                                          uint32_t redraw_stream) {
+  uint32_t r = r31(wa);
+  if (r != 0u) return r;
+  r = r31(wb);
+  if (r != 0u) return r;
+  for (uint32_t k = 0; k < REDRAW_LIMIT; ++k) {
+    const u32x4 w = draw4(seed, exciton, hop, redraw_stream + k);
+    if (r31(w.v[0]) != 0u) return r31(w.v[0]);
+    if (r31(w.v[1]) != 0u) return r31(w.v[1]);
+    if (r31(w.v[2]) != 0u) return r31(w.v[2]);
+    if (r31(w.v[3]) != 0u) return r31(w.v[3]);
+  }
+  return 1u;
+}
+
+}  // namespace cntkmc

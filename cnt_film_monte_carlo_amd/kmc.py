@@ -1,0 +1,433 @@
+"""ctypes bindings of libcntkmc.so, the C ABI of include/cntkmc.h.
+
+The Python layer is plumbing over the native library: every array goes through the
+C ABI unchanged, and every call raises KmcError on a negative status.  The library is
+required: importing this module fails loudly when libcntkmc.so has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from dataclasses import dataclass, field
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libcntkmc.so"
+
+OK = 0
+ERR_ARG, ERR_STATE, ERR_HIP, ERR_ALLOC, ERR_NODEVICE, ERR_GEOMETRY, ERR_RATE, ERR_CAPACITY = (
+    -1, -2, -3, -4, -5, -6, -7, -8)
+SITE_SOURCE, SITE_DRAIN = 1, 2
+RATE_TABLE, RATE_PAIR = 0, 1
+PERIODIC_X, PERIODIC_Y, PERIODIC_Z = 1, 2, 4
+
+_P = C.c_void_p
+_D = C.POINTER(C.c_double)
+_F = C.POINTER(C.c_float)
+_I32 = C.POINTER(C.c_int32)
+_I64 = C.POINTER(C.c_int64)
+_U8 = C.POINTER(C.c_uint8)
+_U32 = C.POINTER(C.c_uint32)
+_U64 = C.POINTER(C.c_uint64)
+
+
+class KmcError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class _Config(C.Structure):
+    _fields_ = [
+        ("device", C.c_int32),
+        ("periodic_mask", C.c_int32),
+        ("cutoff", C.c_double),
+        ("min_distance", C.c_double),
+        ("period_lo", C.c_double * 3),
+        ("period_len", C.c_double * 3),
+        ("seed", C.c_uint64),
+    ]
+
+
+class _NetParams(C.Structure):
+    _fields_ = [
+        ("seed", C.c_uint64),
+        ("n_tubes", C.c_int64),
+        ("len_min", C.c_double),
+        ("len_max", C.c_double),
+        ("box_lo", C.c_double * 3),
+        ("box_hi", C.c_double * 3),
+        ("site_spacing", C.c_double),
+        ("n_chir", C.c_int32),
+        ("wrap_mask", C.c_int32),
+    ]
+
+
+_SIGS = {
+    "kmc_version": (C.c_char_p, []),
+    "kmc_last_error": (C.c_char_p, [_P]),
+    "kmc_create": (C.c_int, [C.POINTER(_Config), C.POINTER(_P)]),
+    "kmc_destroy": (C.c_int, [_P]),
+    "kmc_sync": (C.c_int, [_P]),
+    "kmc_set_sites": (C.c_int, [_P, C.c_int64, _D, _D, _D, _D, _D, _D, _I32, _I32, _U8, _U8]),
+    "kmc_build_cells": (C.c_int, [_P]),
+    "kmc_get_cells": (C.c_int, [_P, _I32, _I64, _I64, _I32]),
+    "kmc_set_rate_table": (C.c_int, [_P, C.c_int32, C.c_int32, _D, C.c_int32, _D, C.c_int32, _D,
+                                     C.c_int32, _D, _D]),
+    "kmc_build_rates": (C.c_int, [_P, C.c_int32]),
+    "kmc_get_nnz": (C.c_int, [_P, _I64]),
+    "kmc_get_pair_geometry": (C.c_int, [_P, _D, _D, _D, _D, _U8, _U8]),
+    "kmc_set_pair_rates": (C.c_int, [_P, _D]),
+    "kmc_set_pair_rates_device": (C.c_int, [_P, _P]),
+    "kmc_get_csr": (C.c_int, [_P, _I64, _I32, _D, _F, _D]),
+    "kmc_set_csr": (C.c_int, [_P, C.c_int64, _I64, _I32, _D, _F]),
+    "kmc_inject": (C.c_int, [_P, C.c_int64]),
+    "kmc_set_samples": (C.c_int, [_P, C.c_int32, C.c_double]),
+    "kmc_set_trace": (C.c_int, [_P, C.c_int64, C.c_int64]),
+    "kmc_run": (C.c_int, [_P, C.c_int64, C.c_double, C.c_int32, _D]),
+    "kmc_get_excitons": (C.c_int, [_P, _I32, _D, _U64, _D, _U32, _U64]),
+    "kmc_get_hop_counts": (C.c_int, [_P, _U64]),
+    "kmc_get_samples": (C.c_int, [_P, _D, _U32, _U32]),
+    "kmc_get_trace": (C.c_int, [_P, _I32, _D, _I64]),
+    "kmc_reduce_samples": (C.c_int, [_P, C.c_int32, _D, _D]),
+    "kmc_estimate_diffusion": (C.c_int, [_D, C.c_int32, C.c_int32, C.c_double, C.c_int32,
+                                         C.c_int32, _D, _D]),
+    "kmc_generate_network": (C.c_int, [C.POINTER(_NetParams), C.c_int64, _D, _D, _D, _D, _D, _D,
+                                       _I32, _I32, _U8, _I32, _I64]),
+    "kmc_forster_table": (C.c_int, [C.c_double, C.c_int32, _D, C.c_int32, _D, C.c_int32, _D,
+                                    C.c_int32, _D, _D]),
+    "kmc_linspace": (C.c_int, [C.c_double, C.c_double, C.c_int32, _D]),
+}
+
+_lib = None
+
+
+def library() -> C.CDLL:
+    """Load libcntkmc.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; "
+                f"g.build()'` (hipcc, gfx950)")
+        lib = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def exported_symbols() -> list[str]:
+    return list(_SIGS)
+
+
+def _ptr(a: np.ndarray | None, ct):
+    if a is None:
+        return None
+    return a.ctypes.data_as(ct)
+
+
+def _c(a, dtype) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def _check(code: int, ctx=None):
+    if code < 0:
+        msg = library().kmc_last_error(ctx).decode()
+        raise KmcError(code, msg)
+    return code
+
+
+# ---------------------------------------------------------------------------------------------
+@dataclass
+class Config:
+    cutoff: float
+    min_distance: float = 0.4e-9
+    periodic_mask: int = 0
+    period_lo: tuple = (0.0, 0.0, 0.0)
+    period_len: tuple = (0.0, 0.0, 0.0)
+    seed: int = 0
+    device: int = 0
+
+    def to_c(self) -> _Config:
+        c = _Config()
+        c.device = self.device
+        c.periodic_mask = self.periodic_mask
+        c.cutoff = self.cutoff
+        c.min_distance = self.min_distance
+        for a in range(3):
+            c.period_lo[a] = self.period_lo[a]
+            c.period_len[a] = self.period_len[a]
+        c.seed = self.seed & 0xFFFFFFFFFFFFFFFF
+        return c
+
+
+class Context:
+    """One KMC problem on one GPU (wraps kmc_ctx)."""
+
+    def __init__(self, cfg: Config):
+        self.lib = library()
+        self.cfg = cfg
+        h = _P()
+        _check(self.lib.kmc_create(C.byref(cfg.to_c()), C.byref(h)))
+        self.h = h
+        self.n_sites = 0
+        self.n_exc = 0
+        self.n_bins = 0
+        self.t_sample_max = 0.0
+        self.n_trace = 0
+        self.tr_max = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.kmc_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ck(self, code):
+        return _check(code, self.h)
+
+    # -- sites / cells / rates ---------------------------------------------------------------
+    def set_sites(self, x, y, z, ox, oy, oz, left=None, right=None, chirality=None, flags=None):
+        arrs = [_c(v, np.float64) for v in (x, y, z, ox, oy, oz)]
+        n = arrs[0].size
+        left = None if left is None else _c(left, np.int32)
+        right = None if right is None else _c(right, np.int32)
+        chir = None if chirality is None else _c(chirality, np.uint8)
+        fl = None if flags is None else _c(flags, np.uint8)
+        self._ck(self.lib.kmc_set_sites(self.h, n, *[_ptr(a, _D) for a in arrs], _ptr(left, _I32),
+                                        _ptr(right, _I32), _ptr(chir, _U8), _ptr(fl, _U8)))
+        self.n_sites = n
+
+    def build_cells(self):
+        self._ck(self.lib.kmc_build_cells(self.h))
+
+    def get_cells(self):
+        dims = np.zeros(3, np.int32)
+        nc = C.c_int64()
+        self._ck(self.lib.kmc_get_cells(self.h, _ptr(dims, _I32), C.byref(nc), None, None))
+        start = np.zeros(nc.value + 1, np.int64)
+        sites = np.zeros(self.n_sites, np.int32)
+        self._ck(self.lib.kmc_get_cells(self.h, None, None, _ptr(start, _I64), _ptr(sites, _I32)))
+        return dims, start, sites
+
+    def set_rate_table(self, theta, z_shift, a1, a2, rates):
+        g = [_c(v, np.float64) for v in (theta, z_shift, a1, a2)]
+        r = _c(rates, np.float64)
+        per = g[0].size * g[1].size * g[2].size * g[3].size
+        n2 = r.size // per
+        n_chir = int(round(math.sqrt(n2)))
+        if n_chir * n_chir * per != r.size:
+            raise ValueError("rates must hold n_chir^2 tables of the grid size")
+        self._ck(self.lib.kmc_set_rate_table(self.h, n_chir, g[0].size, _ptr(g[0], _D), g[1].size,
+                                             _ptr(g[1], _D), g[2].size, _ptr(g[2], _D), g[3].size,
+                                             _ptr(g[3], _D), _ptr(r, _D)))
+
+    def build_rates(self, mode=RATE_TABLE):
+        self._ck(self.lib.kmc_build_rates(self.h, mode))
+
+    @property
+    def nnz(self) -> int:
+        v = C.c_int64()
+        self._ck(self.lib.kmc_get_nnz(self.h, C.byref(v)))
+        return v.value
+
+    def get_pair_geometry(self):
+        m = self.nnz
+        out = [np.zeros(m) for _ in range(4)]
+        cs = np.zeros(m, np.uint8)
+        cd = np.zeros(m, np.uint8)
+        self._ck(self.lib.kmc_get_pair_geometry(self.h, *[_ptr(a, _D) for a in out], _ptr(cs, _U8),
+                                                _ptr(cd, _U8)))
+        return dict(theta=out[0], z=out[1], a1=out[2], a2=out[3], chir_src=cs, chir_dst=cd)
+
+    def set_pair_rates(self, rates):
+        r = _c(rates, np.float64)
+        self._ck(self.lib.kmc_set_pair_rates(self.h, _ptr(r, _D)))
+
+    def set_pair_rates_device(self, dev_ptr: int):
+        self._ck(self.lib.kmc_set_pair_rates_device(self.h, C.c_void_p(dev_ptr)))
+
+    def get_csr(self):
+        m = self.nnz
+        row = np.zeros(self.n_sites + 1, np.int64)
+        col = np.zeros(m, np.int32)
+        cum = np.zeros(m, np.float64)
+        disp = np.zeros((m, 3), np.float32)
+        gam = np.zeros(self.n_sites, np.float64)
+        self._ck(self.lib.kmc_get_csr(self.h, _ptr(row, _I64), _ptr(col, _I32), _ptr(cum, _D),
+                                      _ptr(disp, _F), _ptr(gam, _D)))
+        return dict(row_ptr=row, col=col, cum=cum, disp=disp, gamma=gam)
+
+    def set_csr(self, row_ptr, col, cum, disp):
+        row = _c(row_ptr, np.int64)
+        self._ck(self.lib.kmc_set_csr(self.h, row.size - 1, _ptr(row, _I64),
+                                      _ptr(_c(col, np.int32), _I32), _ptr(_c(cum, np.float64), _D),
+                                      _ptr(_c(disp, np.float32), _F)))
+
+    # -- excitons -----------------------------------------------------------------------------
+    def inject(self, n_excitons: int):
+        self._ck(self.lib.kmc_inject(self.h, int(n_excitons)))
+        self.n_exc = int(n_excitons)
+        self.n_bins = 0
+        self.n_trace = 0
+
+    def set_samples(self, n_bins: int, t_sample_max: float):
+        self._ck(self.lib.kmc_set_samples(self.h, int(n_bins), float(t_sample_max)))
+        self.n_bins = int(n_bins)
+        self.t_sample_max = float(t_sample_max)
+
+    def set_trace(self, n_trace: int, max_hops: int):
+        self._ck(self.lib.kmc_set_trace(self.h, int(n_trace), int(max_hops)))
+        self.n_trace, self.tr_max = int(n_trace), int(max_hops)
+
+    def run(self, hops_per_exciton: int, t_max: float = math.inf, count_sites: bool = True,
+            timed: bool = True) -> float:
+        ms = C.c_double(0.0)
+        self._ck(self.lib.kmc_run(self.h, int(hops_per_exciton), float(t_max), int(count_sites),
+                                  C.byref(ms) if timed else None))
+        return ms.value
+
+    def sync(self):
+        self._ck(self.lib.kmc_sync(self.h))
+
+    def get_excitons(self):
+        E = self.n_exc
+        site = np.zeros(E, np.int32)
+        t = np.zeros(E)
+        hops = np.zeros(E, np.uint64)
+        disp = np.zeros((E, 3))
+        dr = np.zeros(E, np.uint32)
+        hs = np.zeros(E, np.uint64)
+        self._ck(self.lib.kmc_get_excitons(self.h, _ptr(site, _I32), _ptr(t, _D), _ptr(hops, _U64),
+                                           _ptr(disp, _D), _ptr(dr, _U32), _ptr(hs, _U64)))
+        return dict(site=site, t_next=t, hops=hops, disp=disp, drains=dr, hash=hs)
+
+    def get_hop_counts(self):
+        c = np.zeros(self.n_sites, np.uint64)
+        self._ck(self.lib.kmc_get_hop_counts(self.h, _ptr(c, _U64)))
+        return c
+
+    def get_samples(self):
+        T, E = self.n_bins, self.n_exc
+        disp = np.zeros((T, E, 3))
+        dr = np.zeros((T, E), np.uint32)
+        kr = np.zeros(E, np.uint32)
+        self._ck(self.lib.kmc_get_samples(self.h, _ptr(disp, _D), _ptr(dr, _U32), _ptr(kr, _U32)))
+        return dict(disp=disp, drains=dr, n_recorded=kr)
+
+    def get_trace(self):
+        s = np.zeros((self.n_trace, self.tr_max), np.int32)
+        t = np.zeros((self.n_trace, self.tr_max))
+        n = np.zeros(self.n_trace, np.int64)
+        self._ck(self.lib.kmc_get_trace(self.h, _ptr(s, _I32), _ptr(t, _D), _ptr(n, _I64)))
+        return dict(sites=s, times=t, n_hops=n)
+
+    def reduce_samples(self, n_batches: int):
+        out = np.zeros((self.n_bins, n_batches, 5))
+        ms = C.c_double(0.0)
+        self._ck(self.lib.kmc_reduce_samples(self.h, int(n_batches), _ptr(out, _D), C.byref(ms)))
+        return out, ms.value
+
+
+# ---------------------------------------------------------------------------------------------
+# host-only helpers
+# ---------------------------------------------------------------------------------------------
+@dataclass
+class NetworkParams:
+    n_tubes: int
+    len_min: float
+    len_max: float
+    box_lo: tuple
+    box_hi: tuple
+    site_spacing: float = 10e-9
+    n_chir: int = 1
+    wrap_mask: int = 0
+    seed: int = 1001
+
+    def to_c(self) -> _NetParams:
+        p = _NetParams()
+        p.seed = self.seed
+        p.n_tubes = self.n_tubes
+        p.len_min = self.len_min
+        p.len_max = self.len_max
+        for a in range(3):
+            p.box_lo[a] = self.box_lo[a]
+            p.box_hi[a] = self.box_hi[a]
+        p.site_spacing = self.site_spacing
+        p.n_chir = self.n_chir
+        p.wrap_mask = self.wrap_mask
+        return p
+
+
+def generate_network(params: NetworkParams) -> dict:
+    lib = library()
+    cp = params.to_c()
+    n = C.c_int64()
+    _check(lib.kmc_generate_network(C.byref(cp), 0, *([None] * 10), C.byref(n)))
+    N = n.value
+    out = {k: np.zeros(N) for k in ("x", "y", "z", "ox", "oy", "oz")}
+    left = np.zeros(N, np.int32)
+    right = np.zeros(N, np.int32)
+    chir = np.zeros(N, np.uint8)
+    tube = np.zeros(N, np.int32)
+    _check(lib.kmc_generate_network(C.byref(cp), N, *[_ptr(out[k], _D) for k in
+                                                       ("x", "y", "z", "ox", "oy", "oz")],
+                                    _ptr(left, _I32), _ptr(right, _I32), _ptr(chir, _U8),
+                                    _ptr(tube, _I32), C.byref(n)))
+    out.update(left=left, right=right, chirality=chir, tube=tube, n=N)
+    return out
+
+
+def linspace(start: float, end: float, n: int) -> np.ndarray:
+    out = np.zeros(n)
+    _check(library().kmc_linspace(start, end, n, _ptr(out, _D)))
+    return out
+
+
+def forster_table(gamma_0: float, theta, z_shift, a1, a2) -> np.ndarray:
+    g = [_c(v, np.float64) for v in (theta, z_shift, a1, a2)]
+    out = np.zeros(tuple(v.size for v in g))
+    _check(library().kmc_forster_table(gamma_0, g[0].size, _ptr(g[0], _D), g[1].size,
+                                       _ptr(g[1], _D), g[2].size, _ptr(g[2], _D), g[3].size,
+                                       _ptr(g[3], _D), _ptr(out, _D)))
+    return out
+
+
+def estimate_diffusion(reduced: np.ndarray, t_sample_max: float, k_lo: int = 0,
+                       k_hi: int | None = None):
+    red = _c(reduced, np.float64)
+    T, B = red.shape[0], red.shape[1]
+    D = np.zeros(3)
+    ci = np.zeros(3)
+    _check(library().kmc_estimate_diffusion(_ptr(red, _D), T, B, float(t_sample_max), int(k_lo),
+                                            int(T if k_hi is None else k_hi), _ptr(D, _D),
+                                            _ptr(ci, _D)))
+    return D, ci
+
+
+def default_grids():
+    """The table grid of input.json:161-164 (z 1.5..10 nm x11, shifts -10..10 nm x11, theta
+    0..180 deg x21), built like create_*_scatt_table (monte_carlo.cpp:65-75)."""
+    pi_ref = 3.141592  # src/helper/constants.h:10
+    z = linspace(1.5e-9, 10e-9, 11)
+    a1 = linspace(-10e-9, 10e-9, 11)
+    a2 = linspace(-10e-9, 10e-9, 11)
+    th = linspace(0.0, 180.0, 21) * (pi_ref / 180)
+    return th, z, a1, a2

@@ -1,0 +1,786 @@
+/*
+ * kmc_oracle.c -- CPU oracle of the KMC exciton-hopping path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the cpu_baseline leg of
+ * bench.py may load this library, and only as the checker or the CPU baseline; the product
+ * (cnt_film_monte_carlo_amd/) never links, loads or calls it.
+ *
+ * A plain-C, FP64 restatement of the reference algorithm (amirhosseindavoody/
+ * cnt_film_monte_carlo), written independently of the HIP product.  Each function cites the
+ * reference lines it follows.  Build: oracle/Makefile (gcc -O2 -fopenmp -ffp-contract=off).
+ *
+ * Draw mapping shared with the product (the counter-based generator replaces std::rand()):
+ *   Philox4x32-10, key = {seed lo, seed hi}, counter = {hop lo, hop hi, exciton, stream}.
+ *   hop h    : stream 0 -> w0 selection, w1 dwell, w2 re-injection, w3 dwell spare
+ *   injection: stream 1 -> w0 site, w1 dwell, w2 heading, w3 dwell spare
+ *   a 31-bit draw is w >> 1; dwell draws of 0 move to the spare word, then to streams
+ *   2+k (hops) or 0x80000000+k (injection), k < 16, words in order; after that r = 1.
+ *
+ * Parity pins (see tests/): Philox against the published Random123 known-answer vectors;
+ * geometry against closed forms; cell lists and neighbour sets against brute force; the
+ * selection against a linear scan; the dwell against Exp(Gamma) statistics; MSD against the
+ * 1-D chain known answer D = k a^2.  Armadillo's summation order (dot, norm, linspace,
+ * index_min) cannot be pinned here (Armadillo is absent): parity unpinned for those ulps.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORC_RAND_MAX 2147483647.0
+
+/* ------------------------------------------------------------------------------------------
+ * Philox4x32-10 (Salmon et al., SC'11): 10 rounds of two 32x32->64 multiplies with Weyl key
+ * increments.
+ * ---------------------------------------------------------------------------------------- */
+static void mulhilo32(uint32_t a, uint32_t b, uint32_t* hi, uint32_t* lo) {
+  uint64_t p = (uint64_t)a * (uint64_t)b;
+  *hi = (uint32_t)(p >> 32);
+  *lo = (uint32_t)p;
+}
+
+void orc_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4]) {
+  uint32_t x0 = ctr_in[0], x1 = ctr_in[1], x2 = ctr_in[2], x3 = ctr_in[3];
+  uint32_t k0 = key_in[0], k1 = key_in[1];
+  int round;
+  for (round = 0; round < 10; ++round) {
+    uint32_t hi0, lo0, hi1, lo1, y0, y1, y2, y3;
+    if (round > 0) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    mulhilo32(0xD2511F53u, x0, &hi0, &lo0);
+    mulhilo32(0xCD9E8D57u, x2, &hi1, &lo1);
+    y0 = hi1 ^ x1 ^ k0;
+    y1 = lo1;
+    y2 = hi0 ^ x3 ^ k1;
+    y3 = lo0;
+    x0 = y0; x1 = y1; x2 = y2; x3 = y3;
+  }
+  out[0] = x0; out[1] = x1; out[2] = x2; out[3] = x3;
+}
+
+static void draws(uint64_t seed, uint32_t exciton, uint64_t hop, uint32_t stream, uint32_t w[4]) {
+  uint32_t ctr[4], key[2];
+  ctr[0] = (uint32_t)(hop & 0xFFFFFFFFu);
+  ctr[1] = (uint32_t)(hop >> 32);
+  ctr[2] = exciton;
+  ctr[3] = stream;
+  key[0] = (uint32_t)(seed & 0xFFFFFFFFu);
+  key[1] = (uint32_t)(seed >> 32);
+  orc_philox4x32_10(ctr, key, w);
+}
+
+/* a draw for ff_time, which loops while rand() == 0 (scatterer.h:76-77) */
+static uint32_t dwell_draw(uint64_t seed, uint32_t e, uint64_t hop, uint32_t w1, uint32_t w3,
+                           uint32_t redraw_base) {
+  uint32_t k, i, w[4];
+  if ((w1 >> 1) != 0u) return w1 >> 1;
+  if ((w3 >> 1) != 0u) return w3 >> 1;
+  for (k = 0; k < 16u; ++k) {
+    draws(seed, e, hop, redraw_base + k, w);
+    for (i = 0; i < 4; ++i)
+      if ((w[i] >> 1) != 0u) return w[i] >> 1;
+  }
+  return 1u;
+}
+
+/* ------------------------------------------------------------------------------------------
+ * Armadillo-order small-vector arithmetic (op_dot::direct_dot_arma, vec_norm_2_direct_std):
+ * two accumulators over even / odd elements, summed at the end.
+ * ---------------------------------------------------------------------------------------- */
+static double arma_dot(const double* a, const double* b, int n) {
+  double v1 = 0.0, v2 = 0.0;
+  int i, j;
+  for (i = 0, j = 1; j < n; i += 2, j += 2) {
+    v1 += a[i] * b[i];
+    v2 += a[j] * b[j];
+  }
+  if (i < n) v1 += a[i] * b[i];
+  return v1 + v2;
+}
+
+static double arma_norm(const double* a, int n) {
+  double acc1 = 0.0, acc2 = 0.0;
+  int i, j;
+  for (i = 0, j = 1; j < n; i += 2, j += 2) {
+    acc1 += a[i] * a[i];
+    acc2 += a[j] * a[j];
+  }
+  if (i < n) acc1 += a[i] * a[i];
+  return sqrt(acc1 + acc2);
+}
+
+/* arma::linspace (used by create_*_scatt_table, monte_carlo.cpp:66-75, 158-167) */
+int orc_linspace(double start, double end, int n, double* out) {
+  int i;
+  double delta;
+  if (n < 1) return -1;
+  if (n == 1) {
+    out[0] = end;
+    return 0;
+  }
+  delta = (end - start) / (double)(n - 1);
+  for (i = 0; i < n - 1; ++i) out[i] = start + (double)i * delta;
+  out[n - 1] = end;
+  return 0;
+}
+
+/* arma::normalise: divide by the norm unless it is zero */
+static void arma_normalise(const double* v, double* out) {
+  double nv = arma_norm(v, 3);
+  double d = (nv != 0.0) ? nv : 1.0;
+  out[0] = v[0] / d;
+  out[1] = v[1] / d;
+  out[2] = v[2] / d;
+}
+
+/* create_forster_scatt_table (src/monte_carlo/monte_carlo.cpp:156-200) */
+int orc_forster_table(double gamma_0, int n_th, const double* th, int n_z, const double* zs,
+                      int n_a1, const double* a1, int n_a2, const double* a2, double* rates) {
+  int it, iz, i1, i2;
+  size_t idx = 0;
+  for (it = 0; it < n_th; ++it)
+    for (iz = 0; iz < n_z; ++iz)
+      for (i1 = 0; i1 < n_a1; ++i1)
+        for (i2 = 0; i2 < n_a2; ++i2) {
+          double r1[3], r2[3], dR[3], n1[3], n2[3], nd[3], af;
+          r1[0] = a1[i1]; r1[1] = 0.0; r1[2] = 0.0;
+          r2[0] = a2[i2] * cos(th[it]); r2[1] = a2[i2] * sin(th[it]); r2[2] = zs[iz];
+          dR[0] = r1[0] - r2[0]; dR[1] = r1[1] - r2[1]; dR[2] = r1[2] - r2[2];
+          arma_normalise(r1, n1);
+          arma_normalise(r2, n2);
+          arma_normalise(dR, nd);
+          af = cos(th[it]) - 3 * arma_dot(n1, nd, 3) * arma_dot(n2, nd, 3);
+          rates[idx++] = gamma_0 * pow(af, 2) * pow(1.e-9 / arma_norm(dR, 3), 6);
+        }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------------------------
+ * Network: sites, cell list, rate table
+ * ---------------------------------------------------------------------------------------- */
+typedef struct orc_net {
+  int64_t n;
+  double *p[3], *o[3];
+  uint8_t *chir, *flags;
+  double cutoff, min_dist;
+  int periodic;
+  double period_lo[3], period_len[3];
+  /* cells */
+  int dims[3];
+  double lo[3], cell[3];
+  int64_t n_cells;
+  int64_t* cell_start;
+  int32_t* cell_sites;
+  /* table */
+  int t_nchir, t_n[4];
+  double* t_grid[4];
+  double* t_rates;
+  /* source list */
+  int64_t n_src;
+  int32_t* src;
+} orc_net;
+
+static int axis_cell(const orc_net* w, int a, double p) {
+  if (w->periodic & (1 << a)) {
+    int i = (int)floor((p - w->lo[a]) / w->cell[a]);
+    if (i < 0) i = 0;
+    if (i >= w->dims[a]) i = w->dims[a] - 1;
+    return i;
+  }
+  /* int ix = (s.pos(0) - xmin) / radius;  (monte_carlo.h:390) */
+  return (int)((p - w->lo[a]) / w->cell[a]);
+}
+
+void orc_net_free(orc_net* w) {
+  int a;
+  if (!w) return;
+  for (a = 0; a < 3; ++a) {
+    free(w->p[a]);
+    free(w->o[a]);
+  }
+  for (a = 0; a < 4; ++a) free(w->t_grid[a]);
+  free(w->chir); free(w->flags); free(w->cell_start); free(w->cell_sites); free(w->t_rates);
+  free(w->src);
+  free(w);
+}
+
+/* create_scatterer_buckets (src/monte_carlo/monte_carlo.h:368-416), plus a periodic axis
+ * option (period split into floor(period / cutoff) equal cells). */
+orc_net* orc_net_create(int64_t n, const double* x, const double* y, const double* z,
+                        const double* ox, const double* oy, const double* oz,
+                        const uint8_t* chir, const uint8_t* flags, double cutoff, double min_dist,
+                        int periodic, const double* period_lo, const double* period_len) {
+  orc_net* w = (orc_net*)calloc(1, sizeof(orc_net));
+  const double* pin[3];
+  const double* oin[3];
+  int64_t i, *fill;
+  int a;
+  if (!w || n <= 0) return NULL;
+  pin[0] = x; pin[1] = y; pin[2] = z;
+  oin[0] = ox; oin[1] = oy; oin[2] = oz;
+  w->n = n;
+  for (a = 0; a < 3; ++a) {
+    w->p[a] = (double*)malloc(n * sizeof(double));
+    w->o[a] = (double*)malloc(n * sizeof(double));
+    memcpy(w->p[a], pin[a], n * sizeof(double));
+    memcpy(w->o[a], oin[a], n * sizeof(double));
+  }
+  w->chir = (uint8_t*)calloc(n, 1);
+  w->flags = (uint8_t*)calloc(n, 1);
+  if (chir) memcpy(w->chir, chir, n);
+  if (flags) memcpy(w->flags, flags, n);
+  w->cutoff = cutoff;
+  w->min_dist = min_dist;
+  w->periodic = periodic;
+  for (a = 0; a < 3; ++a) {
+    w->period_lo[a] = period_lo ? period_lo[a] : 0.0;
+    w->period_len[a] = period_len ? period_len[a] : 0.0;
+  }
+  /* domain: bounding box of the sites (find_simulation_domain, monte_carlo.h:328-340) */
+  for (a = 0; a < 3; ++a) {
+    double mn = w->p[a][0], mx = w->p[a][0];
+    for (i = 0; i < n; ++i) {
+      mn = mn > w->p[a][i] ? w->p[a][i] : mn;
+      mx = mx < w->p[a][i] ? w->p[a][i] : mx;
+    }
+    if (periodic & (1 << a)) {
+      int m = (int)floor(w->period_len[a] / cutoff);
+      w->dims[a] = m < 1 ? 1 : m;
+      w->lo[a] = w->period_lo[a];
+      w->cell[a] = w->period_len[a] / w->dims[a];
+    } else {
+      w->dims[a] = (int)ceil((mx - mn) / cutoff) + 1; /* monte_carlo.h:377 */
+      w->lo[a] = mn;
+      w->cell[a] = cutoff;
+    }
+  }
+  w->n_cells = (int64_t)w->dims[0] * w->dims[1] * w->dims[2];
+  w->cell_start = (int64_t*)calloc(w->n_cells + 1, sizeof(int64_t));
+  w->cell_sites = (int32_t*)malloc(n * sizeof(int32_t));
+  fill = (int64_t*)calloc(w->n_cells, sizeof(int64_t));
+  /* bucket contents in site order (monte_carlo.h:389-395) */
+  for (i = 0; i < n; ++i) {
+    int64_t c = (int64_t)axis_cell(w, 0, w->p[0][i]) +
+                (int64_t)axis_cell(w, 1, w->p[1][i]) * w->dims[0] +
+                (int64_t)axis_cell(w, 2, w->p[2][i]) * w->dims[0] * w->dims[1];
+    w->cell_start[c + 1]++;
+  }
+  for (i = 0; i < w->n_cells; ++i) w->cell_start[i + 1] += w->cell_start[i];
+  for (i = 0; i < n; ++i) {
+    int64_t c = (int64_t)axis_cell(w, 0, w->p[0][i]) +
+                (int64_t)axis_cell(w, 1, w->p[1][i]) * w->dims[0] +
+                (int64_t)axis_cell(w, 2, w->p[2][i]) * w->dims[0] * w->dims[1];
+    w->cell_sites[w->cell_start[c] + fill[c]++] = (int32_t)i;
+  }
+  free(fill);
+  w->src = (int32_t*)malloc(n * sizeof(int32_t));
+  for (i = 0; i < n; ++i)
+    if (w->flags[i] & 1u) w->src[w->n_src++] = (int32_t)i;
+  return w;
+}
+
+int orc_net_cells(const orc_net* w, int32_t dims[3], int64_t* n_cells, int64_t* cell_start,
+                  int32_t* cell_sites) {
+  if (dims) {
+    dims[0] = w->dims[0]; dims[1] = w->dims[1]; dims[2] = w->dims[2];
+  }
+  if (n_cells) *n_cells = w->n_cells;
+  if (cell_start) memcpy(cell_start, w->cell_start, (w->n_cells + 1) * sizeof(int64_t));
+  if (cell_sites) memcpy(cell_sites, w->cell_sites, w->n * sizeof(int32_t));
+  return 0;
+}
+
+int orc_net_set_table(orc_net* w, int n_chir, int n_th, const double* th, int n_z,
+                      const double* zs, int n_a1, const double* a1, int n_a2, const double* a2,
+                      const double* rates) {
+  const double* g[4];
+  int n[4], a;
+  size_t nr;
+  g[0] = th; g[1] = zs; g[2] = a1; g[3] = a2;
+  n[0] = n_th; n[1] = n_z; n[2] = n_a1; n[3] = n_a2;
+  w->t_nchir = n_chir;
+  for (a = 0; a < 4; ++a) {
+    free(w->t_grid[a]);
+    w->t_n[a] = n[a];
+    w->t_grid[a] = (double*)malloc(n[a] * sizeof(double));
+    memcpy(w->t_grid[a], g[a], n[a] * sizeof(double));
+  }
+  nr = (size_t)n_chir * n_chir * n_th * n_z * n_a1 * n_a2;
+  free(w->t_rates);
+  w->t_rates = (double*)malloc(nr * sizeof(double));
+  memcpy(w->t_rates, rates, nr * sizeof(double));
+  return 0;
+}
+
+/* tmp = abs(grid - x); tmp.index_min()  (scattering_struct.h:42-49) */
+static unsigned arma_index_min_abs(const double* grid, int n, double x) {
+  double best = INFINITY;
+  unsigned idx = 0;
+  int i;
+  for (i = 0; i < n; ++i) {
+    double v = fabs(grid[i] - x);
+    if (v < best) {
+      best = v;
+      idx = (unsigned)i;
+    }
+  }
+  return idx;
+}
+
+/* scattering_struct::get_rate (scattering_struct.h:38-52), per chirality pair */
+static double get_rate(const orc_net* w, int ci, int cj, double theta, double zsh, double a1,
+                       double a2) {
+  unsigned ith = arma_index_min_abs(w->t_grid[0], w->t_n[0], theta);
+  unsigned iz = arma_index_min_abs(w->t_grid[1], w->t_n[1], zsh);
+  unsigned i1 = arma_index_min_abs(w->t_grid[2], w->t_n[2], a1);
+  unsigned i2 = arma_index_min_abs(w->t_grid[3], w->t_n[3], a2);
+  size_t tab = (size_t)ci * w->t_nchir + cj;
+  size_t idx = (((tab * w->t_n[0] + ith) * w->t_n[1] + iz) * w->t_n[2] + i1) * w->t_n[3] + i2;
+  return w->t_rates[idx];
+}
+
+/* check_neighbor geometry (src/monte_carlo/scatterer.cpp:44-61) */
+static void orc_geometry(const double* pi, const double* pj, const double* a1, const double* a2,
+                         double* theta, double* zsh, double* ash1, double* ash2) {
+  double dR[3], cosT, y1, y2, s2, v[3];
+  int k;
+  for (k = 0; k < 3; ++k) dR[k] = pi[k] - pj[k];
+  cosT = arma_dot(a1, a2, 3);
+  if (cosT == 1) {
+    double d;
+    *ash1 = 0;
+    *ash2 = arma_dot(dR, a1, 3);
+    *theta = 0;
+    d = arma_dot(dR, a1, 3);
+    for (k = 0; k < 3; ++k) v[k] = dR[k] - d * a1[k];
+    *zsh = arma_norm(v, 3);
+  } else {
+    *theta = acos(cosT);
+    y1 = arma_dot(a1, dR, 3);
+    y2 = arma_dot(a2, dR, 3);
+    s2 = 1 - pow(cosT, 2);
+    *ash1 = (y1 + y2 * cosT) / s2;
+    *ash2 = (y2 + y1 * cosT) / s2;
+    for (k = 0; k < 3; ++k) v[k] = (*ash1 * a1[k] + pi[k]) - (*ash2 * a2[k] + pj[k]);
+    *zsh = arma_norm(v, 3);
+  }
+}
+
+/* image of coordinate q seen from p on a periodic axis of length L */
+static double orc_image(double p, double q, double L) {
+  double d = p - q;
+  if (d > 0.5 * L) return q + L;
+  if (d < -0.5 * L) return q - L;
+  return q;
+}
+
+typedef struct {
+  int32_t j;
+  double pj[3];
+} orc_cand;
+
+/* Enumerate the neighbours of site i in reference order: stencil cells x-outer, y, z-inner
+ * (monte_carlo.h:402-411), buckets in site order, keep min_dist < |dR| < cutoff
+ * (scatterer.cpp:40-43).  Returns the count; fills out (capacity cap) when non-NULL. */
+static int64_t orc_neighbors(const orc_net* w, int64_t i, orc_cand* out, int64_t cap) {
+  int st[3][3], ns[3], a, b, c, d, k;
+  int64_t cnt = 0, q;
+  double pi[3];
+  for (a = 0; a < 3; ++a) pi[a] = w->p[a][i];
+  for (a = 0; a < 3; ++a) {
+    int ia = axis_cell(w, a, pi[a]);
+    ns[a] = 0;
+    for (d = -1; d <= 1; ++d) {
+      int v = ia + d;
+      if (w->periodic & (1 << a)) {
+        int dup = 0;
+        v = ((v % w->dims[a]) + w->dims[a]) % w->dims[a];
+        for (k = 0; k < ns[a]; ++k) dup |= (st[a][k] == v);
+        if (!dup) st[a][ns[a]++] = v;
+      } else if (v > -1 && v < w->dims[a]) {
+        st[a][ns[a]++] = v;
+      }
+    }
+  }
+  for (a = 0; a < ns[0]; ++a)
+    for (b = 0; b < ns[1]; ++b)
+      for (c = 0; c < ns[2]; ++c) {
+        int64_t cell = (int64_t)st[0][a] + (int64_t)st[1][b] * w->dims[0] +
+                       (int64_t)st[2][c] * w->dims[0] * w->dims[1];
+        for (q = w->cell_start[cell]; q < w->cell_start[cell + 1]; ++q) {
+          int32_t j = w->cell_sites[q];
+          double pj[3], dR[3], dist;
+          for (k = 0; k < 3; ++k) {
+            pj[k] = w->p[k][j];
+            if (w->periodic & (1 << k)) pj[k] = orc_image(pi[k], pj[k], w->period_len[k]);
+            dR[k] = pi[k] - pj[k];
+          }
+          dist = arma_norm(dR, 3);
+          if ((dist < w->cutoff) && (dist > w->min_dist)) {
+            if (out && cnt < cap) {
+              out[cnt].j = j;
+              for (k = 0; k < 3; ++k) out[cnt].pj[k] = pj[k];
+            }
+            ++cnt;
+          }
+        }
+      }
+  return cnt;
+}
+
+int64_t orc_net_count_pairs(const orc_net* w, int64_t* row_ptr) {
+  int64_t i;
+  row_ptr[0] = 0;
+  for (i = 0; i < w->n; ++i) row_ptr[i + 1] = row_ptr[i] + orc_neighbors(w, i, NULL, 0);
+  return row_ptr[w->n];
+}
+
+/* find_neighbors with inclusive FP64 prefix sum (scatterer.cpp:34-83).  mode 0: table
+ * rates; mode 1: pair_rates[nnz] given in CSR order.  Any output may be NULL. */
+int orc_net_fill(const orc_net* w, int mode, const int64_t* row_ptr, const double* pair_rates,
+                 int32_t* col, double* cum, float* disp, double* g_th, double* g_z,
+                 double* g_a1, double* g_a2) {
+  int64_t i;
+  int64_t maxdeg = 0;
+  for (i = 0; i < w->n; ++i)
+    if (row_ptr[i + 1] - row_ptr[i] > maxdeg) maxdeg = row_ptr[i + 1] - row_ptr[i];
+#pragma omp parallel for schedule(dynamic, 256)
+  for (i = 0; i < w->n; ++i) {
+    orc_cand* nb = (orc_cand*)malloc((maxdeg + 1) * sizeof(orc_cand));
+    int64_t m = orc_neighbors(w, i, nb, maxdeg + 1), q;
+    double pi[3], ai[3];
+    int k;
+    for (k = 0; k < 3; ++k) {
+      pi[k] = w->p[k][i];
+      ai[k] = w->o[k][i];
+    }
+    for (q = 0; q < m; ++q) {
+      int64_t pos = row_ptr[i] + q;
+      int32_t j = nb[q].j;
+      double aj[3], th, zs, s1, s2, rate;
+      for (k = 0; k < 3; ++k) aj[k] = w->o[k][j];
+      orc_geometry(pi, nb[q].pj, ai, aj, &th, &zs, &s1, &s2);
+      if (col) col[pos] = j;
+      if (disp)
+        for (k = 0; k < 3; ++k) disp[3 * pos + k] = (float)(nb[q].pj[k] - pi[k]);
+      if (g_th) g_th[pos] = th;
+      if (g_z) g_z[pos] = zs;
+      if (g_a1) g_a1[pos] = s1;
+      if (g_a2) g_a2[pos] = s2;
+      if (cum) {
+        rate = (mode == 0) ? get_rate(w, w->chir[i], w->chir[j], th, zs, s1, s2) : pair_rates[pos];
+        cum[pos] = (q == 0) ? rate : rate + cum[pos - 1];
+      }
+    }
+    free(nb);
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------------------------
+ * Excitons
+ * ---------------------------------------------------------------------------------------- */
+/* scatterer::update_state binary search (scatterer.cpp:17-30); cum = the row, n >= 1 */
+static unsigned orc_select(const double* cum, unsigned n, double dice) {
+  unsigned left = (unsigned)-1;
+  unsigned right = n - 1;
+  while (left + 1 < right) {
+    unsigned mid = (left + right) / 2;
+    if (cum[mid] <= dice)
+      left = mid;
+    else
+      right = mid;
+  }
+  return right;
+}
+
+int orc_select_index(const double* cum, int n, double dice) {
+  return (int)orc_select(cum, (unsigned)n, dice);
+}
+
+/* ff_time (scatterer.h:74-80) with the cached inverse rate of set_max_rate (scatterer.h:92) */
+static double orc_dwell(const double* row, int64_t n, uint32_t r) {
+  double max_rate, inv;
+  if (n == 0) return INFINITY;
+  max_rate = row[n - 1];
+  inv = 1. / max_rate;
+  return -inv * log((double)r / (double)ORC_RAND_MAX);
+}
+
+double orc_dwell_time(double max_rate, uint32_t r) {
+  double inv = 1. / max_rate;
+  return -inv * log((double)r / (double)ORC_RAND_MAX);
+}
+
+#define FNV_OFF 0xcbf29ce484222325ull
+#define FNV_PR 0x100000001b3ull
+
+/* kubo_create_particles + particle constructor (monte_carlo.cpp:308-316, particle.h:48-52) */
+int orc_inject(int64_t n_sites, const int64_t* row_ptr, const double* cum, int64_t n_src,
+               const int32_t* src, uint64_t seed, int64_t E, int32_t* site, double* t_next,
+               uint64_t* hops, double* disp, uint32_t* drains, uint64_t* hash, uint32_t* k_rec) {
+  int64_t e;
+  (void)n_sites;
+  if (n_src <= 0) return -1;
+#pragma omp parallel for
+  for (e = 0; e < E; ++e) {
+    uint32_t w[4], r;
+    int32_t s;
+    draws(seed, (uint32_t)e, 0, 1u, w);
+    s = src[(w[0] >> 1) % (uint32_t)n_src];
+    r = dwell_draw(seed, (uint32_t)e, 0, w[1], w[3], 0x80000000u);
+    site[e] = s;
+    t_next[e] = 0.0 + orc_dwell(cum + row_ptr[s], row_ptr[s + 1] - row_ptr[s], r);
+    hops[e] = 0;
+    disp[3 * e] = disp[3 * e + 1] = disp[3 * e + 2] = 0.0;
+    drains[e] = 0;
+    hash[e] = (FNV_OFF ^ (uint64_t)(uint32_t)s) * FNV_PR;
+    if (k_rec) k_rec[e] = 0;
+  }
+  return 0;
+}
+
+typedef struct orc_run_args {
+  /* CSR */
+  int64_t n_sites;
+  const int64_t* row_ptr;
+  const int32_t* col;
+  const double* cum;
+  const float* disp;
+  const uint8_t* flags;
+  int64_t n_src;
+  const int32_t* src;
+  uint64_t seed;
+  /* state [E] */
+  int64_t E;
+  int32_t* site;
+  double* t_next;
+  uint64_t* hops;
+  double* dsp; /* [E][3] */
+  uint32_t* drains;
+  uint64_t* hash;
+  uint32_t* k_rec;
+  /* budget */
+  int64_t hop_budget;
+  double t_max;
+  /* samples [T][E] */
+  int32_t n_bins;
+  double t_sample_max;
+  double* rec_disp; /* [T][E][3] */
+  uint32_t* rec_drain;
+  /* counts [n_sites], may be NULL */
+  uint64_t* site_count;
+  /* trace [n_trace][tr_max] */
+  int64_t n_trace, tr_max;
+  int32_t* tr_site;
+  double* tr_time;
+  int64_t* tr_n;
+  /* faithful mode: recompute neighbours per hop from this network (NULL = use the CSR) */
+  const orc_net* net;
+  int n_threads;
+} orc_run_args;
+
+/* One exciton through its budget: particle::step (particle.cpp:57-80) in event form,
+ * update_state (scatterer.cpp:9-31), ff_time (scatterer.h:74-80), re-injection on a drain
+ * site keeping delta_pos and ff_time (monte_carlo.cpp:330-336). */
+static void orc_one(const orc_run_args* a, int64_t e, orc_cand* nb, double* rowbuf,
+                    float* dispbuf, int32_t* colbuf, int64_t cap) {
+  int32_t s = a->site[e];
+  double t = a->t_next[e];
+  uint64_t h = a->hops[e];
+  double dx = a->dsp[3 * e], dy = a->dsp[3 * e + 1], dz = a->dsp[3 * e + 2];
+  uint32_t nd = a->drains[e];
+  uint64_t hs = a->hash[e];
+  uint32_t k = a->k_rec ? a->k_rec[e] : 0;
+  int64_t tr = (a->tr_n && e < a->n_trace) ? a->tr_n[e] : 0;
+  const uint64_t h_end = h + (uint64_t)a->hop_budget;
+  const double dts = a->n_bins > 0 ? a->t_sample_max / a->n_bins : 0.0;
+
+  for (;;) {
+    const double* row;
+    const int32_t* rcol;
+    const float* rdisp;
+    int64_t n;
+    uint32_t w[4], r;
+    unsigned idx;
+    double dice;
+    int32_t j;
+    int drain;
+    double t_hop;
+
+    while ((int32_t)k < a->n_bins) {
+      double tk = (double)(k + 1) * dts;
+      size_t rr;
+      if (!(tk < t)) break;
+      rr = (size_t)k * a->E + e;
+      a->rec_disp[3 * rr] = dx;
+      a->rec_disp[3 * rr + 1] = dy;
+      a->rec_disp[3 * rr + 2] = dz;
+      a->rec_drain[rr] = nd;
+      ++k;
+    }
+    /* current site row: cached CSR or recomputed (find_neighbors on every hop, scatterer.cpp:12) */
+    if (a->net) {
+      const orc_net* w0 = a->net;
+      int64_t q;
+      double pi[3], ai[3];
+      int kk;
+      n = orc_neighbors(w0, s, nb, cap);
+      for (kk = 0; kk < 3; ++kk) {
+        pi[kk] = w0->p[kk][s];
+        ai[kk] = w0->o[kk][s];
+      }
+      for (q = 0; q < n; ++q) {
+        double aj[3], th, zs, s1, s2, rate;
+        int32_t jj = nb[q].j;
+        for (kk = 0; kk < 3; ++kk) aj[kk] = w0->o[kk][jj];
+        orc_geometry(pi, nb[q].pj, ai, aj, &th, &zs, &s1, &s2);
+        rate = get_rate(w0, w0->chir[s], w0->chir[jj], th, zs, s1, s2);
+        rowbuf[q] = (q == 0) ? rate : rate + rowbuf[q - 1];
+        colbuf[q] = jj;
+        for (kk = 0; kk < 3; ++kk) dispbuf[3 * q + kk] = (float)(nb[q].pj[kk] - pi[kk]);
+      }
+      row = rowbuf;
+      rcol = colbuf;
+      rdisp = dispbuf;
+    } else {
+      row = a->cum + a->row_ptr[s];
+      rcol = a->col + a->row_ptr[s];
+      rdisp = a->disp + 3 * a->row_ptr[s];
+      n = a->row_ptr[s + 1] - a->row_ptr[s];
+    }
+    if (h >= h_end || n == 0 || !(t <= a->t_max)) break;
+
+    draws(a->seed, (uint32_t)e, h, 0u, w);
+    dice = row[n - 1] * (double)(w[0] >> 1) / ORC_RAND_MAX;
+    idx = orc_select(row, (unsigned)n, dice);
+    j = rcol[idx];
+    if (a->site_count) {
+#pragma omp atomic
+      a->site_count[s] += 1;
+    }
+    dx = dx + (double)rdisp[3 * idx];
+    dy = dy + (double)rdisp[3 * idx + 1];
+    dz = dz + (double)rdisp[3 * idx + 2];
+    hs = (hs ^ (uint64_t)(uint32_t)j) * FNV_PR;
+    r = dwell_draw(a->seed, (uint32_t)e, h, w[1], w[3], 2u);
+    t_hop = t;
+    {
+      double tj;
+      if (a->net) {
+        /* the dwell uses the new site's cached 1/Gamma (set_max_rate) */
+        int64_t nj = orc_neighbors(a->net, j, nb, cap), q;
+        double gj = 0.0;
+        const orc_net* w0 = a->net;
+        double pj[3], aj[3];
+        int kk;
+        for (kk = 0; kk < 3; ++kk) {
+          pj[kk] = w0->p[kk][j];
+          aj[kk] = w0->o[kk][j];
+        }
+        for (q = 0; q < nj; ++q) {
+          double ak[3], th, zs, s1, s2, rate;
+          int32_t jj = nb[q].j;
+          for (kk = 0; kk < 3; ++kk) ak[kk] = w0->o[kk][jj];
+          orc_geometry(pj, nb[q].pj, aj, ak, &th, &zs, &s1, &s2);
+          rate = get_rate(w0, w0->chir[j], w0->chir[jj], th, zs, s1, s2);
+          gj = (q == 0) ? rate : rate + gj;
+        }
+        tj = (nj == 0) ? INFINITY : orc_dwell_time(gj, r);
+      } else {
+        tj = orc_dwell(a->cum + a->row_ptr[j], a->row_ptr[j + 1] - a->row_ptr[j], r);
+      }
+      t = t + tj;
+    }
+    s = j;
+    drain = (a->flags[j] & 2u) != 0;
+    if (drain) {
+      ++nd;
+      s = a->src[(w[2] >> 1) % (uint32_t)a->n_src];
+      hs = (hs ^ ((uint64_t)(uint32_t)s | (1ull << 32))) * FNV_PR;
+    }
+    if (a->tr_site && e < a->n_trace && tr < a->tr_max) {
+      a->tr_site[(size_t)e * a->tr_max + tr] = s;
+      a->tr_time[(size_t)e * a->tr_max + tr] = t_hop;
+      ++tr;
+    }
+    ++h;
+  }
+  a->site[e] = s;
+  a->t_next[e] = t;
+  a->hops[e] = h;
+  a->dsp[3 * e] = dx;
+  a->dsp[3 * e + 1] = dy;
+  a->dsp[3 * e + 2] = dz;
+  a->drains[e] = nd;
+  a->hash[e] = hs;
+  if (a->k_rec) a->k_rec[e] = k;
+  if (a->tr_n && e < a->n_trace) a->tr_n[e] = tr;
+}
+
+int orc_run(const orc_run_args* a) {
+  int64_t e, cap = 0;
+  if (a->net) {
+    int64_t i;
+    for (i = 0; i < a->net->n; ++i) {
+      int64_t m = orc_neighbors(a->net, i, NULL, 0);
+      if (m > cap) cap = m;
+    }
+  }
+  cap += 1;
+#ifdef _OPENMP
+  if (a->n_threads > 0) omp_set_num_threads(a->n_threads);
+#endif
+#pragma omp parallel
+  {
+    orc_cand* nb = (orc_cand*)malloc(cap * sizeof(orc_cand));
+    double* rowbuf = (double*)malloc(cap * sizeof(double));
+    float* dispbuf = (float*)malloc(3 * cap * sizeof(float));
+    int32_t* colbuf = (int32_t*)malloc(cap * sizeof(int32_t));
+#pragma omp for schedule(dynamic, 16)
+    for (e = 0; e < a->E; ++e) orc_one(a, e, nb, rowbuf, dispbuf, colbuf, cap);
+    free(nb);
+    free(rowbuf);
+    free(dispbuf);
+    free(colbuf);
+  }
+  return 0;
+}
+
+/* Serial per-bin, per-batch sums of the MSD records (kubo_save_avg_dispalcement_squared,
+ * monte_carlo.cpp:382-409): out[T][B][5] = {count, sum dx^2, sum dy^2, sum dz^2, sum drains}. */
+int orc_reduce_samples(int64_t E, int32_t n_bins, int32_t n_batches, const uint32_t* k_rec,
+                       const double* rec_disp, const uint32_t* rec_drain, double* out) {
+  int32_t k, b;
+  for (k = 0; k < n_bins; ++k)
+    for (b = 0; b < n_batches; ++b) {
+      int64_t e0 = (int64_t)b * E / n_batches, e1 = (int64_t)(b + 1) * E / n_batches, e;
+      double acc[5] = {0, 0, 0, 0, 0};
+      for (e = e0; e < e1; ++e) {
+        if ((uint32_t)k < k_rec[e]) {
+          size_t r = (size_t)k * E + e;
+          double x = rec_disp[3 * r], y = rec_disp[3 * r + 1], z = rec_disp[3 * r + 2];
+          acc[0] += 1.0;
+          acc[1] += pow(x, 2);
+          acc[2] += pow(y, 2);
+          acc[3] += pow(z, 2);
+          acc[4] += (double)rec_drain[r];
+        }
+      }
+      memcpy(out + ((size_t)k * n_batches + b) * 5, acc, sizeof(acc));
+    }
+  return 0;
+}
+
+int orc_max_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
