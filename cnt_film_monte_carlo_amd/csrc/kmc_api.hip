@@ -68,6 +68,11 @@ struct kmc_ctx {
   dbuf<edge_rec> d_edge;
   dbuf<double> d_geo[4];
   dbuf<int32_t> d_err;
+  // row blocks read by the hop kernel
+  int64_t n_units = 0;
+  dbuf<uint32_t> d_units, d_src_roff;
+  dbuf<int64_t> d_roff;
+  dbuf<uint4> d_blocks;
 
   // excitons
   int64_t n_exc = 0;
@@ -76,7 +81,9 @@ struct kmc_ctx {
   dbuf<uint64_t> x_hops, x_hash;
   dbuf<uint32_t> x_drains, x_krec;
   dbuf<uint8_t> x_heading;
-  dbuf<uint32_t> d_site_count;
+  dbuf<uint32_t> d_site_count;  // KMC_COUNT_COPIES copies, count_stride apart
+  dbuf<uint64_t> d_counts64;
+  int64_t count_stride = 0;
 
   // MSD records
   int32_t n_bins = 0;
@@ -198,6 +205,31 @@ int copy_out(kmc_ctx* c, void* dst, const void* src, size_t bytes) {
   return KMC_OK;
 }
 
+// Row blocks of the hop kernel from the finished CSR (hdr, cum, edge), plus the block offsets of
+// the source sites.
+int pack_rows(kmc_ctx* c) {
+  const int64_t n = c->n_sites;
+  KMC_TRY(dalloc(c, c->d_units, n));
+  KMC_TRY(dalloc(c, c->d_roff, n + 1));
+  KMC_HIP(c, launch_block_sizes(n, c->d_hdr.p, c->d_units.p, c->stream));
+  KMC_HIP(c, exclusive_scan(c->d_units.p, c->d_roff.p, n, c->d_scan_tmp.p, c->stream));
+  int64_t units = 0;
+  KMC_TRY(copy_out(c, &units, c->d_roff.p + n, sizeof(int64_t)));
+  if (units >= (int64_t)EDGE_DRAIN_BIT)
+    return fail(c, KMC_ERR_CAPACITY, "pack_rows: CSR too large for 31-bit block offsets");
+  c->n_units = units;
+  // 8 spare units: the hop kernel reads eight cumulative rates past short rows
+  KMC_TRY(dalloc(c, c->d_blocks, units + 8));
+  KMC_HIP(c, hipMemsetAsync(c->d_blocks.p, 0, (units + 8) * sizeof(uint4), c->stream));
+  KMC_HIP(c, launch_fill_blocks(n, c->d_hdr.p, c->d_cum.p, c->d_edge.p, c->d_roff.p,
+                                c->d_blocks.p, c->stream));
+  KMC_TRY(dalloc(c, c->d_src_roff, c->h_src.size()));
+  KMC_HIP(c, launch_gather_offsets((int64_t)c->h_src.size(), c->d_src.p, c->d_roff.p,
+                                   c->d_src_roff.p, c->stream));
+  KMC_HIP(c, hipStreamSynchronize(c->stream));
+  return KMC_OK;
+}
+
 int check_err_flag(kmc_ctx* c, const char* what) {
   int32_t h = 0;
   KMC_TRY(copy_out(c, &h, c->d_err.p, sizeof(int32_t)));
@@ -266,6 +298,8 @@ int kmc_destroy(kmc_ctx* c) {
   dfree(c->d_row_ptr); dfree(c->d_hdr); dfree(c->d_cum); dfree(c->d_pair_rates);
   dfree(c->d_edge);
   for (int q = 0; q < 4; ++q) dfree(c->d_geo[q]);
+  dfree(c->d_units); dfree(c->d_src_roff); dfree(c->d_roff); dfree(c->d_blocks);
+  dfree(c->d_counts64);
   dfree(c->d_err); dfree(c->x_site); dfree(c->x_t); dfree(c->x_dx); dfree(c->x_dy);
   dfree(c->x_dz); dfree(c->x_hops); dfree(c->x_hash); dfree(c->x_drains); dfree(c->x_krec);
   dfree(c->x_heading); dfree(c->d_site_count); dfree(c->r_dx); dfree(c->r_dy); dfree(c->r_dz);
@@ -512,6 +546,7 @@ int kmc_build_rates(kmc_ctx* c, int32_t mode) {
   KMC_HIP(c, launch_fill_pairs(mode, s, c->cells, c->d_cell_start.p, c->d_cell_sites.p,
                                c->d_row_ptr.p, t, csr_of(c), geo, c->d_err.p, c->stream));
   KMC_TRY(check_err_flag(c, "kmc_build_rates"));
+  if (mode == KMC_RATE_TABLE) KMC_TRY(pack_rows(c));
   c->csr_state = (mode == KMC_RATE_TABLE) ? 2 : 1;
   reset_downstream(c, 2);
   return KMC_OK;
@@ -556,6 +591,7 @@ int kmc_set_pair_rates_device(kmc_ctx* c, const double* d_rates) {
   KMC_HIP(c, hipMemsetAsync(c->d_err.p, 0, sizeof(int32_t), c->stream));
   KMC_HIP(c, launch_finish_pair_rates(c->n_sites, d_rates, csr_of(c), c->d_err.p, c->stream));
   KMC_TRY(check_err_flag(c, "kmc_set_pair_rates"));
+  KMC_TRY(pack_rows(c));
   c->csr_state = 2;
   reset_downstream(c, 2);
   return KMC_OK;
@@ -638,6 +674,9 @@ int kmc_set_csr(kmc_ctx* c, int64_t n_sites, const int64_t* row_ptr, const int32
   KMC_TRY(copy_in(c, c->d_cum.p, cum, nnz * sizeof(double)));
   KMC_TRY(copy_in(c, c->d_edge.p, edge.data(), nnz * sizeof(edge_rec)));
   KMC_HIP(c, hipStreamSynchronize(c->stream));
+  if (!c->d_scan_tmp.p || (int64_t)c->d_scan_tmp.n < scan_tmp_elems(n_sites))
+    KMC_TRY(dalloc(c, c->d_scan_tmp, scan_tmp_elems(n_sites)));
+  KMC_TRY(pack_rows(c));
   c->csr_state = 2;
   reset_downstream(c, 2);
   return KMC_OK;
@@ -662,13 +701,15 @@ int kmc_inject(kmc_ctx* c, int64_t n_excitons) {
   KMC_TRY(dalloc(c, c->x_drains, E));
   KMC_TRY(dalloc(c, c->x_krec, E));
   KMC_TRY(dalloc(c, c->x_heading, E));
-  KMC_TRY(dalloc(c, c->d_site_count, c->n_sites));
-  KMC_HIP(c, hipMemsetAsync(c->d_site_count.p, 0, c->n_sites * sizeof(uint32_t), c->stream));
+  c->count_stride = (c->n_sites + COUNT_ALIGN - 1) / COUNT_ALIGN * COUNT_ALIGN;
+  KMC_TRY(dalloc(c, c->d_site_count, (size_t)KMC_COUNT_COPIES * c->count_stride));
+  KMC_HIP(c, hipMemsetAsync(c->d_site_count.p, 0, c->d_site_count.n * sizeof(uint32_t),
+                            c->stream));
   c->n_exc = E;
   c->n_bins = 0;
   c->n_trace = 0;
-  KMC_HIP(c, launch_inject(excitons_of(c), c->d_hdr.p, c->d_src.p, (uint32_t)c->h_src.size(),
-                           c->cfg.seed, c->stream));
+  KMC_HIP(c, launch_inject(excitons_of(c), c->d_blocks.p, c->d_src_roff.p,
+                           (uint32_t)c->h_src.size(), c->cfg.seed, c->stream));
   KMC_HIP(c, hipStreamSynchronize(c->stream));
   return KMC_OK;
 }
@@ -685,6 +726,12 @@ int kmc_set_samples(kmc_ctx* c, int32_t n_bins, double t_sample_max) {
   KMC_TRY(dalloc(c, c->r_dz, nrec));
   KMC_TRY(dalloc(c, c->r_drain, nrec));
   KMC_HIP(c, hipMemsetAsync(c->x_krec.p, 0, c->n_exc * sizeof(uint32_t), c->stream));
+  if (nrec) {
+    KMC_HIP(c, hipMemsetAsync(c->r_dx.p, 0, nrec * sizeof(double), c->stream));
+    KMC_HIP(c, hipMemsetAsync(c->r_dy.p, 0, nrec * sizeof(double), c->stream));
+    KMC_HIP(c, hipMemsetAsync(c->r_dz.p, 0, nrec * sizeof(double), c->stream));
+    KMC_HIP(c, hipMemsetAsync(c->r_drain.p, 0, nrec * sizeof(uint32_t), c->stream));
+  }
   KMC_HIP(c, hipStreamSynchronize(c->stream));
   c->n_bins = n_bins;
   c->t_sample_max = t_sample_max;
@@ -701,6 +748,11 @@ int kmc_set_trace(kmc_ctx* c, int64_t n_trace, int64_t max_hops) {
   KMC_TRY(dalloc(c, c->tr_time, (size_t)n_trace * max_hops));
   KMC_TRY(dalloc(c, c->tr_n, n_trace));
   if (n_trace) KMC_HIP(c, hipMemsetAsync(c->tr_n.p, 0, n_trace * sizeof(int64_t), c->stream));
+  if (n_trace && max_hops) {
+    const size_t nt = (size_t)n_trace * max_hops;
+    KMC_HIP(c, hipMemsetAsync(c->tr_site.p, 0, nt * sizeof(int32_t), c->stream));
+    KMC_HIP(c, hipMemsetAsync(c->tr_time.p, 0, nt * sizeof(double), c->stream));
+  }
   KMC_HIP(c, hipStreamSynchronize(c->stream));
   c->n_trace = n_trace;
   c->tr_max = max_hops;
@@ -716,10 +768,9 @@ int kmc_run(kmc_ctx* c, int64_t hops_per_exciton, double t_max, int32_t count_si
   KMC_HIP(c, hipSetDevice(c->cfg.device));
   hop_args a{};
   a.x = excitons_of(c);
-  a.hdr = c->d_hdr.p;
-  a.cum = c->d_cum.p;
-  a.edge = c->d_edge.p;
-  a.src = c->d_src.p;
+  a.blocks = c->d_blocks.p;
+  a.roff = c->d_roff.p;
+  a.src_roff = c->d_src_roff.p;
   a.n_src = (uint32_t)c->h_src.size();
   a.seed = c->cfg.seed;
   a.hop_budget = (uint64_t)hops_per_exciton;
@@ -731,6 +782,7 @@ int kmc_run(kmc_ctx* c, int64_t hops_per_exciton, double t_max, int32_t count_si
   a.rec_dz = c->r_dz.p;
   a.rec_drain = c->r_drain.p;
   a.site_count = c->d_site_count.p;
+  a.count_stride = c->count_stride;
   a.n_trace = c->n_trace;
   a.tr_max = c->tr_max;
   a.tr_site = c->tr_site.p;
@@ -774,9 +826,11 @@ int kmc_get_excitons(kmc_ctx* c, int32_t* site, double* t_next, uint64_t* hops, 
 int kmc_get_hop_counts(kmc_ctx* c, uint64_t* counts) {
   if (!c || !counts) return fail(c, KMC_ERR_ARG, "kmc_get_hop_counts: null argument");
   if (c->n_exc == 0) return fail(c, KMC_ERR_STATE, "kmc_get_hop_counts: no excitons");
-  std::vector<uint32_t> v(c->n_sites);
-  KMC_TRY(copy_out(c, v.data(), c->d_site_count.p, c->n_sites * sizeof(uint32_t)));
-  for (int64_t i = 0; i < c->n_sites; ++i) counts[i] = v[i];
+  KMC_HIP(c, hipSetDevice(c->cfg.device));
+  KMC_TRY(dalloc(c, c->d_counts64, c->n_sites));
+  KMC_HIP(c, launch_sum_counts(c->n_sites, c->count_stride, c->d_site_count.p, c->d_counts64.p,
+                               c->stream));
+  KMC_TRY(copy_out(c, counts, c->d_counts64.p, c->n_sites * sizeof(uint64_t)));
   return KMC_OK;
 }
 

@@ -20,6 +20,8 @@ constexpr uint32_t EDGE_COL_MASK = 0x7FFFFFFFu;
 constexpr uint64_t FNV_OFFSET = 0xcbf29ce484222325ull;
 constexpr uint64_t FNV_PRIME = 0x100000001b3ull;
 constexpr int MAX_CHIR = 16;
+#define KMC_COUNT_COPIES 8u  // one departure-count array per XCD
+constexpr int64_t COUNT_ALIGN = 1024;  // each copy starts on a 4 KB boundary
 
 struct __align__(16) site_hdr {
   uint32_t rs;   // first CSR entry of the row

@@ -39,10 +39,9 @@ struct exciton_arrays {
 
 struct hop_args {
   exciton_arrays x;
-  const site_hdr* hdr;
-  const double* cum;
-  const edge_rec* edge;
-  const int32_t* src;
+  const uint4* blocks;      // row blocks (16-byte units)
+  const int64_t* roff;      // block offset of every site
+  const uint32_t* src_roff; // block offsets of the source sites
   uint32_t n_src;
   uint64_t seed;
   uint64_t hop_budget;
@@ -52,8 +51,9 @@ struct hop_args {
   double dt_sample;
   double *rec_dx, *rec_dy, *rec_dz;
   uint32_t* rec_drain;
-  // per-site departure counts
+  // per-site departure counts: KMC_COUNT_COPIES arrays, count_stride apart
   uint32_t* site_count;
+  int64_t count_stride;
   // trace [n_trace][tr_max]
   int64_t n_trace, tr_max;
   int32_t* tr_site;
@@ -88,8 +88,16 @@ hipError_t launch_fill_pairs(int mode, const site_arrays& s, const cell_desc& c,
                              const pair_geom_arrays& geo, int32_t* err, hipStream_t st);
 hipError_t launch_finish_pair_rates(int64_t n, const double* rates, const csr_arrays& out,
                                     int32_t* err, hipStream_t st);
-hipError_t launch_inject(const exciton_arrays& x, const site_hdr* hdr, const int32_t* src,
+hipError_t launch_inject(const exciton_arrays& x, const uint4* blocks, const uint32_t* src_roff,
                          uint32_t n_src, uint64_t seed, hipStream_t st);
+hipError_t launch_block_sizes(int64_t n, const site_hdr* hdr, uint32_t* units, hipStream_t st);
+hipError_t launch_fill_blocks(int64_t n, const site_hdr* hdr, const double* cum,
+                              const edge_rec* edge, const int64_t* roff, uint4* blocks,
+                              hipStream_t st);
+hipError_t launch_gather_offsets(int64_t n, const int32_t* idx, const int64_t* roff, uint32_t* out,
+                                 hipStream_t st);
+hipError_t launch_sum_counts(int64_t n, int64_t stride, const uint32_t* copies, uint64_t* out,
+                             hipStream_t st);
 hipError_t launch_hop(const hop_args& a, bool count, bool sample, bool trace, hipStream_t st);
 hipError_t launch_reduce(const reduce_args& a, hipStream_t st);
 

@@ -296,48 +296,157 @@ __device__ __forceinline__ double dwell_time(uint32_t n, double gamma, uint32_t 
   return -inv * log((double)r / RAND_MAX_D);
 }
 
-// Site header as three scalars from one 16-byte load.
-struct hdr_regs {
-  uint32_t rs, n;
+// ---------------------------------------------------------------------------------------------
+// Row blocks: the hop kernel's copy of the CSR.  Each site owns one contiguous block of 16-byte
+// units: [n | site | Gamma] [cum[0..n) (+8 B pad when n is odd)] [payload[0..n)], where a
+// payload entry is {target block offset | drain bit, float dx, dy, dz}.  One hop touches one
+// block: the header and the first eight cumulative rates arrive with five independent 16-byte
+// loads, and the chosen payload sits a few lines further in the same block.
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_block_sizes(int64_t n, const site_hdr* __restrict__ hdr,
+                                                     uint32_t* __restrict__ units) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t d = hdr[i].n;
+  units[i] = 1u + (d + 1u) / 2u + d;
+}
+
+__global__ __launch_bounds__(256) void k_pack_rows(int64_t n, const site_hdr* __restrict__ hdr,
+                                                   const double* __restrict__ cum,
+                                                   const edge_rec* __restrict__ edge,
+                                                   const int64_t* __restrict__ roff,
+                                                   uint4* __restrict__ blocks) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const site_hdr h = hdr[i];
+  const uint32_t R = (uint32_t)roff[i];
+  uint4 head;
+  head.x = h.n;
+  head.y = (uint32_t)i;
+  head.z = (uint32_t)__double2loint(h.gamma);
+  head.w = (uint32_t)__double2hiint(h.gamma);
+  blocks[R] = head;
+  double* c = reinterpret_cast<double*>(blocks + R + 1);
+  for (uint32_t k = 0; k < h.n; ++k) c[k] = cum[h.rs + k];
+  if (h.n & 1u) c[h.n] = 0.0;
+  uint4* pay = blocks + R + 1 + (h.n + 1u) / 2u;
+  for (uint32_t k = 0; k < h.n; ++k) {
+    const edge_rec e = edge[h.rs + k];
+    const uint32_t j = e.col & EDGE_COL_MASK;
+    uint4 p;
+    p.x = (uint32_t)roff[j] | (e.col & EDGE_DRAIN_BIT);
+    p.y = __float_as_uint(e.dx);
+    p.z = __float_as_uint(e.dy);
+    p.w = __float_as_uint(e.dz);
+    pay[k] = p;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_gather_offsets(int64_t n, const int32_t* __restrict__ idx,
+                                                        const int64_t* __restrict__ roff,
+                                                        uint32_t* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = (uint32_t)roff[idx[i]];
+}
+
+struct row_regs {
+  uint32_t n, site;
   double gamma;
+  double c[8];
 };
 
-__device__ __forceinline__ hdr_regs load_hdr(const site_hdr* __restrict__ hdr, int32_t s) {
-  const uint4 v = *(reinterpret_cast<const uint4*>(hdr) + s);
-  hdr_regs h;
-  h.rs = v.x;
-  h.n = v.y;
-  h.gamma = __hiloint2double((int)v.w, (int)v.z);
-  return h;
+__device__ __forceinline__ row_regs load_row(const uint4* __restrict__ blocks, uint32_t R) {
+  const uint4 h = blocks[R];
+  const uint4 q0 = blocks[R + 1], q1 = blocks[R + 2], q2 = blocks[R + 3], q3 = blocks[R + 4];
+  row_regs r;
+  r.n = h.x;
+  r.site = h.y;
+  r.gamma = __hiloint2double((int)h.w, (int)h.z);
+  r.c[0] = __hiloint2double((int)q0.y, (int)q0.x);
+  r.c[1] = __hiloint2double((int)q0.w, (int)q0.z);
+  r.c[2] = __hiloint2double((int)q1.y, (int)q1.x);
+  r.c[3] = __hiloint2double((int)q1.w, (int)q1.z);
+  r.c[4] = __hiloint2double((int)q2.y, (int)q2.x);
+  r.c[5] = __hiloint2double((int)q2.w, (int)q2.z);
+  r.c[6] = __hiloint2double((int)q3.y, (int)q3.x);
+  r.c[7] = __hiloint2double((int)q3.w, (int)q3.z);
+  return r;
 }
 
-__device__ __forceinline__ void load_edge(const edge_rec* __restrict__ edge, uint32_t p,
-                                          uint32_t& col, float& ex, float& ey, float& ez) {
-  const uint4 v = *(reinterpret_cast<const uint4*>(edge) + p);
-  col = v.x;
-  ex = __uint_as_float(v.y);
-  ey = __uint_as_float(v.z);
-  ez = __uint_as_float(v.w);
+// Index picked by update_state's binary search (scatterer.cpp:17-30): the first entry whose
+// cumulative rate exceeds dice, or n-1.  Rows are non-decreasing (rates >= 0 are enforced when
+// the CSR is built), so that index equals the number of leading entries k < n-1 with
+// cum[k] <= dice, counted here without a dependent load chain.
+__device__ __forceinline__ uint32_t select_entry(const uint4* __restrict__ blocks, uint32_t R,
+                                                 const row_regs& r, double dice) {
+  const uint32_t lim = r.n - 1u;
+  uint32_t idx = 0;
+#pragma unroll
+  for (uint32_t q = 0; q < 8; ++q) idx += (q < lim && r.c[q] <= dice) ? 1u : 0u;
+  if (idx == 8u && lim > 8u) {
+    const double* cg = reinterpret_cast<const double*>(blocks + R + 1);
+    uint32_t k = 8;
+    while (k < lim && cg[k] <= dice) ++k;
+    idx = k;
+  }
+  return idx;
 }
 
-__global__ __launch_bounds__(256) void k_inject(exciton_arrays x, const site_hdr* __restrict__ hdr,
-                                                const int32_t* __restrict__ src, uint32_t n_src,
-                                                uint64_t seed) {
+// Per-thread write-back cache of departure counts: four (site, count) entries in LRU order,
+// flushed with one atomic add into this XCD's private copy of the count array when evicted.
+struct count_cache {
+  uint32_t s0 = 0xFFFFFFFFu, s1 = 0xFFFFFFFFu, s2 = 0xFFFFFFFFu, s3 = 0xFFFFFFFFu;
+  uint32_t n0 = 0, n1 = 0, n2 = 0, n3 = 0;
+
+  __device__ __forceinline__ void add(uint32_t s, uint32_t* __restrict__ base) {
+    if (s == s0) {
+      ++n0;
+    } else if (s == s1) {
+      const uint32_t t = n1;
+      s1 = s0; n1 = n0; s0 = s; n0 = t + 1u;
+    } else if (s == s2) {
+      const uint32_t t = n2;
+      s2 = s1; n2 = n1; s1 = s0; n1 = n0; s0 = s; n0 = t + 1u;
+    } else if (s == s3) {
+      const uint32_t t = n3;
+      s3 = s2; n3 = n2; s2 = s1; n2 = n1; s1 = s0; n1 = n0; s0 = s; n0 = t + 1u;
+    } else {
+      if (n3) atomicAdd(base + s3, n3);
+      s3 = s2; n3 = n2; s2 = s1; n2 = n1; s1 = s0; n1 = n0; s0 = s; n0 = 1u;
+    }
+  }
+  __device__ __forceinline__ void flush(uint32_t* __restrict__ base) {
+    if (n0) atomicAdd(base + s0, n0);
+    if (n1) atomicAdd(base + s1, n1);
+    if (n2) atomicAdd(base + s2, n2);
+    if (n3) atomicAdd(base + s3, n3);
+  }
+};
+
+// Hardware id of the XCD running this wave (gfx950: 8 XCDs).  Only picks which private copy
+// of the count array a wave updates; correctness does not depend on it (device-scope atomics).
+__device__ __forceinline__ uint32_t xcd_id() {
+  return (uint32_t)__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & (KMC_COUNT_COPIES - 1u);
+}
+
+__global__ __launch_bounds__(256) void k_inject(exciton_arrays x, const uint4* __restrict__ blocks,
+                                                const uint32_t* __restrict__ src_roff,
+                                                uint32_t n_src, uint64_t seed) {
 #pragma clang fp contract(off)
   const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= x.n) return;
   const u32x4 w = draw4(seed, e, 0, STREAM_INIT);
-  const int32_t s = src[r31(w.v[0]) % n_src];
-  const hdr_regs h = load_hdr(hdr, s);
+  const uint4 h = blocks[src_roff[r31(w.v[0]) % n_src]];
   const uint32_t r = nonzero_draw(w.v[1], w.v[3], seed, e, 0, STREAM_INIT_REDRAW);
-  x.site[e] = s;
-  x.t_next[e] = 0.0 + dwell_time(h.n, h.gamma, r);
+  x.site[e] = (int32_t)h.y;
+  x.t_next[e] = 0.0 + dwell_time(h.x, __hiloint2double((int)h.w, (int)h.z), r);
   x.hops[e] = 0;
   x.dx[e] = 0.0;
   x.dy[e] = 0.0;
   x.dz[e] = 0.0;
   x.drains[e] = 0;
-  x.hash[e] = (FNV_OFFSET ^ (uint64_t)(uint32_t)s) * FNV_PRIME;
+  x.hash[e] = (FNV_OFFSET ^ (uint64_t)h.y) * FNV_PRIME;
   x.k_rec[e] = 0;
   x.heading[e] = (uint8_t)(r31(w.v[2]) & 1u);
 }
@@ -349,7 +458,6 @@ __global__ __launch_bounds__(256) void k_hop(hop_args a) {
 #pragma clang fp contract(off)
   const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= a.x.n) return;
-  int32_t s = a.x.site[e];
   double t = a.x.t_next[e];
   uint64_t h = a.x.hops[e];
   double dx = a.x.dx[e], dy = a.x.dy[e], dz = a.x.dz[e];
@@ -358,9 +466,12 @@ __global__ __launch_bounds__(256) void k_hop(hop_args a) {
   uint32_t k = SAMPLE ? a.x.k_rec[e] : 0u;
   int64_t tr = 0;
   if (TRACE && e < a.n_trace) tr = a.tr_n[e];
-  hdr_regs H = load_hdr(a.hdr, s);
+  uint32_t R = a.roff[a.x.site[e]];
+  row_regs row = load_row(a.blocks, R);
   const uint64_t h_end = h + a.hop_budget;
   const size_t E = a.x.n;
+  count_cache cc;
+  uint32_t* cbase = COUNT ? a.site_count + (size_t)xcd_id() * a.count_stride : nullptr;
 
   while (true) {
     if (SAMPLE) {
@@ -375,53 +486,43 @@ __global__ __launch_bounds__(256) void k_hop(hop_args a) {
         ++k;
       }
     }
-    if (h >= h_end || H.n == 0u || !(t <= a.t_max)) break;
+    if (h >= h_end || row.n == 0u || !(t <= a.t_max)) break;
 
     // event selection (scatterer::update_state, scatterer.cpp:17-30)
     const u32x4 w = draw4(a.seed, e, h, STREAM_HOP);
-    const double dice = (H.gamma * (double)r31(w.v[0])) / RAND_MAX_D;
-    const double* row = a.cum + H.rs;
-    uint32_t left = 0xFFFFFFFFu, right = H.n - 1u;
-    while (left + 1u < right) {
-      const uint32_t mid = (left + right) >> 1;
-      if (row[mid] <= dice)
-        left = mid;
-      else
-        right = mid;
-    }
-    uint32_t col;
-    float ex, ey, ez;
-    load_edge(a.edge, H.rs + right, col, ex, ey, ez);
-    if (COUNT) atomicAdd(&a.site_count[s], 1u);
-    const int32_t j = (int32_t)(col & EDGE_COL_MASK);
-    dx = dx + (double)ex;
-    dy = dy + (double)ey;
-    dz = dz + (double)ez;
-    hs = (hs ^ (uint64_t)(uint32_t)j) * FNV_PRIME;
+    const double dice = (row.gamma * (double)r31(w.v[0])) / RAND_MAX_D;
+    const uint32_t idx = select_entry(a.blocks, R, row, dice);
+    const uint4 p = a.blocks[R + 1u + (row.n + 1u) / 2u + idx];
+    if (COUNT) cc.add(row.site, cbase);
+    dx = dx + (double)__uint_as_float(p.y);
+    dy = dy + (double)__uint_as_float(p.z);
+    dz = dz + (double)__uint_as_float(p.w);
     const double t_hop = t;
 
     // dwell at the new site (particle.cpp:183 -> ff_time)
-    H = load_hdr(a.hdr, j);
+    R = p.x & EDGE_COL_MASK;
+    row = load_row(a.blocks, R);
+    hs = (hs ^ (uint64_t)row.site) * FNV_PRIME;
     const uint32_t r = nonzero_draw(w.v[1], w.v[3], a.seed, e, h, STREAM_HOP_REDRAW);
-    t = t + dwell_time(H.n, H.gamma, r);
-    s = j;
+    t = t + dwell_time(row.n, row.gamma, r);
     // drain: count current and re-inject, keeping displacement and dwell (kubo_step,
     // src/monte_carlo/monte_carlo.cpp:330-336)
-    if (col & EDGE_DRAIN_BIT) {
+    if (p.x & EDGE_DRAIN_BIT) {
       ++nd;
-      s = a.src[r31(w.v[2]) % a.n_src];
-      hs = (hs ^ ((uint64_t)(uint32_t)s | (1ull << 32))) * FNV_PRIME;
-      H = load_hdr(a.hdr, s);
+      R = a.src_roff[r31(w.v[2]) % a.n_src];
+      row = load_row(a.blocks, R);
+      hs = (hs ^ ((uint64_t)row.site | (1ull << 32))) * FNV_PRIME;
     }
     if (TRACE && e < a.n_trace && tr < a.tr_max) {
-      a.tr_site[(size_t)e * a.tr_max + tr] = s;
+      a.tr_site[(size_t)e * a.tr_max + tr] = (int32_t)row.site;
       a.tr_time[(size_t)e * a.tr_max + tr] = t_hop;
       ++tr;
     }
     ++h;
   }
+  if (COUNT) cc.flush(cbase);
 
-  a.x.site[e] = s;
+  a.x.site[e] = (int32_t)row.site;
   a.x.t_next[e] = t;
   a.x.hops[e] = h;
   a.x.dx[e] = dx;
@@ -450,6 +551,18 @@ hipError_t launch_hop(const hop_args& a, bool count, bool sample, bool trace, hi
   KMC_HOP_CASE(true, true, true)
 #undef KMC_HOP_CASE
   return hipErrorInvalidValue;
+}
+
+// Sum of the per-XCD copies of the departure counts.
+__global__ __launch_bounds__(256) void k_sum_counts(int64_t n, int64_t stride,
+                                                    const uint32_t* __restrict__ copies,
+                                                    uint64_t* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t s = 0;
+#pragma unroll
+  for (int x = 0; x < KMC_COUNT_COPIES; ++x) s += copies[(size_t)x * stride + i];
+  out[i] = s;
 }
 
 // =============================================================================================
@@ -555,10 +668,38 @@ hipError_t launch_finish_pair_rates(int64_t n, const double* rates, const csr_ar
   return hipGetLastError();
 }
 
-hipError_t launch_inject(const exciton_arrays& x, const site_hdr* hdr, const int32_t* src,
+hipError_t launch_inject(const exciton_arrays& x, const uint4* blocks, const uint32_t* src_roff,
                          uint32_t n_src, uint64_t seed, hipStream_t st) {
   if (x.n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_inject, grid_for(x.n), dim3(256), 0, st, x, hdr, src, n_src, seed);
+  hipLaunchKernelGGL(k_inject, grid_for(x.n), dim3(256), 0, st, x, blocks, src_roff, n_src, seed);
+  return hipGetLastError();
+}
+
+hipError_t launch_block_sizes(int64_t n, const site_hdr* hdr, uint32_t* units, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_block_sizes, grid_for(n), dim3(256), 0, st, n, hdr, units);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_blocks(int64_t n, const site_hdr* hdr, const double* cum,
+                              const edge_rec* edge, const int64_t* roff, uint4* blocks,
+                              hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pack_rows, grid_for(n), dim3(256), 0, st, n, hdr, cum, edge, roff, blocks);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_offsets(int64_t n, const int32_t* idx, const int64_t* roff, uint32_t* out,
+                                 hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gather_offsets, grid_for(n), dim3(256), 0, st, n, idx, roff, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_sum_counts(int64_t n, int64_t stride, const uint32_t* copies, uint64_t* out,
+                             hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sum_counts, grid_for(n), dim3(256), 0, st, n, stride, copies, out);
   return hipGetLastError();
 }
 

@@ -73,8 +73,13 @@ def test_pair_geometry_and_pair_rates_bitwise(kmc, orc, nw, small):
     ctx.build_rates(kmc.RATE_PAIR)
     geo = ctx.get_pair_geometry()
     og = small["onet"].csr(mode=1, pair_rates=None, geometry=True)
-    for k in ("theta", "z", "a1", "a2"):
+    for k in ("z", "a1", "a2"):
         assert np.array_equal(geo[k], og[k]), k
+    # theta = acos(cos theta): the device acos and glibc's may differ by one ulp
+    th_g, th_o = geo["theta"], og["theta"]
+    ulp = np.abs(th_g.view(np.int64) - th_o.view(np.int64))
+    say(f"  theta: {np.count_nonzero(ulp)} of {ulp.size} differ, max {ulp.max()} ulp")
+    assert ulp.max() <= 1
     rng = np.random.default_rng(4)
     rates = rng.random(ctx.nnz) * 1e12
     rates[::7] = 0.0
@@ -116,8 +121,11 @@ def test_hops_bitwise_small_film(kmc, orc, nw, small):
     assert int(ox.site_count.sum()) == E * H
     s = ctx.get_samples()
     assert np.array_equal(s["n_recorded"], ox.k_rec)
-    assert np.array_equal(s["disp"], ox.rec_disp)
-    assert np.array_equal(s["drains"], ox.rec_drain)
+    valid = np.arange(32)[:, None] < ox.k_rec[None, :]
+    say(f"  MSD records: {valid.sum()} of {valid.size} reached")
+    assert np.array_equal(s["disp"][valid], ox.rec_disp[valid])
+    assert np.array_equal(s["drains"][valid], ox.rec_drain[valid])
+    assert np.all(s["disp"][~valid] == 0) and np.all(s["drains"][~valid] == 0)
     red, _ = ctx.reduce_samples(16)
     ored = ox.reduce_samples(16)
     np.testing.assert_allclose(red, ored, rtol=1e-12, atol=0)
