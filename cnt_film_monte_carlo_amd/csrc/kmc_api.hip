@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -73,6 +74,10 @@ struct kmc_ctx {
   dbuf<uint32_t> d_units, d_src_roff;
   dbuf<int64_t> d_roff;
   dbuf<uint4> d_blocks;
+  // per-chirality-class pair lists for externally evaluated rates
+  dbuf<uint8_t> d_cls;
+  dbuf<uint32_t> d_cls_count;
+  dbuf<int64_t> d_cls_start, d_sel;
 
   // excitons
   int64_t n_exc = 0;
@@ -299,7 +304,8 @@ int kmc_destroy(kmc_ctx* c) {
   dfree(c->d_edge);
   for (int q = 0; q < 4; ++q) dfree(c->d_geo[q]);
   dfree(c->d_units); dfree(c->d_src_roff); dfree(c->d_roff); dfree(c->d_blocks);
-  dfree(c->d_counts64);
+  dfree(c->d_counts64); dfree(c->d_cls); dfree(c->d_cls_count); dfree(c->d_cls_start);
+  dfree(c->d_sel);
   dfree(c->d_err); dfree(c->x_site); dfree(c->x_t); dfree(c->x_dx); dfree(c->x_dy);
   dfree(c->x_dz); dfree(c->x_hops); dfree(c->x_hash); dfree(c->x_drains); dfree(c->x_krec);
   dfree(c->x_heading); dfree(c->d_site_count); dfree(c->r_dx); dfree(c->r_dy); dfree(c->r_dz);
@@ -595,6 +601,67 @@ int kmc_set_pair_rates_device(kmc_ctx* c, const double* d_rates) {
   c->csr_state = 2;
   reset_downstream(c, 2);
   return KMC_OK;
+}
+
+int kmc_pair_classes(kmc_ctx* c, int32_t n_chir, int64_t* class_start, const int64_t** d_sel) {
+  if (!c || n_chir < 1 || n_chir > MAX_CHIR || !class_start)
+    return fail(c, KMC_ERR_ARG, "kmc_pair_classes: bad argument");
+  if (c->csr_state == 0) return fail(c, KMC_ERR_STATE, "kmc_pair_classes: no pair geometry");
+  for (int64_t i = 0; i < c->n_sites; ++i)
+    if (c->h_chir[i] >= n_chir) return fail(c, KMC_ERR_ARG, "kmc_pair_classes: chirality >= n_chir");
+  KMC_HIP(c, hipSetDevice(c->cfg.device));
+  const int nc = n_chir * n_chir;
+  const int64_t nnz = std::max<int64_t>(c->nnz, 1);
+  KMC_TRY(dalloc(c, c->d_cls, nnz));
+  KMC_TRY(dalloc(c, c->d_cls_count, nc));
+  KMC_TRY(dalloc(c, c->d_cls_start, nc + 1));
+  KMC_TRY(dalloc(c, c->d_sel, nnz));
+  KMC_HIP(c, hipMemsetAsync(c->d_cls_count.p, 0, nc * sizeof(uint32_t), c->stream));
+  KMC_HIP(c, launch_pair_classes(c->n_sites, c->nnz, c->d_hdr.p, c->d_edge.p, c->d_chir.p, n_chir,
+                                 c->d_cls.p, c->d_cls_count.p, c->stream));
+  std::vector<uint32_t> cnt(nc);
+  KMC_TRY(copy_out(c, cnt.data(), c->d_cls_count.p, nc * sizeof(uint32_t)));
+  class_start[0] = 0;
+  for (int k = 0; k < nc; ++k) class_start[k + 1] = class_start[k] + cnt[k];
+  KMC_TRY(copy_in(c, c->d_cls_start.p, class_start, (nc + 1) * sizeof(int64_t)));
+  KMC_HIP(c, hipMemsetAsync(c->d_cls_count.p, 0, nc * sizeof(uint32_t), c->stream));
+  KMC_HIP(c, launch_pair_scatter(c->nnz, c->d_cls.p, c->d_cls_start.p, c->d_cls_count.p,
+                                 c->d_sel.p, c->stream));
+  KMC_HIP(c, hipStreamSynchronize(c->stream));
+  if (d_sel) *d_sel = c->d_sel.p;
+  return KMC_OK;
+}
+
+int kmc_build_rates_exact(kmc_ctx* c, int32_t n_chir, kmc_transfer* const* transfers,
+                          double* elapsed_ms) {
+  if (!c || !transfers) return fail(c, KMC_ERR_ARG, "kmc_build_rates_exact: null argument");
+  if (c->csr_state != 1 || !c->d_geo[0].p)
+    return fail(c, KMC_ERR_STATE, "kmc_build_rates_exact: build with KMC_RATE_PAIR first");
+  const auto t0 = std::chrono::steady_clock::now();
+  std::vector<int64_t> start(n_chir * n_chir + 1);
+  const int64_t* sel = nullptr;
+  KMC_TRY(kmc_pair_classes(c, n_chir, start.data(), &sel));
+  KMC_TRY(dalloc(c, c->d_pair_rates, std::max<int64_t>(c->nnz, 1)));
+  KMC_HIP(c, hipMemsetAsync(c->d_pair_rates.p, 0, c->nnz * sizeof(double), c->stream));
+  KMC_HIP(c, hipStreamSynchronize(c->stream));
+  for (int k = 0; k < n_chir * n_chir; ++k) {
+    const int64_t n = start[k + 1] - start[k];
+    if (n == 0 || !transfers[k]) continue;
+    const int s = kmc_transfer_rates_device(transfers[k], n, c->d_geo[0].p, c->d_geo[1].p,
+                                            c->d_geo[2].p, c->d_geo[3].p, sel + start[k],
+                                            c->d_pair_rates.p);
+    if (s != KMC_OK) return fail(c, s, std::string("kmc_build_rates_exact: ") + kmc_transfer_last_error());
+  }
+  KMC_TRY(kmc_set_pair_rates_device(c, c->d_pair_rates.p));
+  if (elapsed_ms)
+    *elapsed_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return KMC_OK;
+}
+
+int kmc_get_pair_rates(kmc_ctx* c, double* rates) {
+  if (!c || !rates) return fail(c, KMC_ERR_ARG, "kmc_get_pair_rates: null argument");
+  if (!c->d_pair_rates.p) return fail(c, KMC_ERR_STATE, "kmc_get_pair_rates: no pair rates");
+  return copy_out(c, rates, c->d_pair_rates.p, c->nnz * sizeof(double));
 }
 
 int kmc_set_pair_rates(kmc_ctx* c, const double* rates) {

@@ -288,10 +288,13 @@ __global__ __launch_bounds__(256) void k_finish_pair_rates(int64_t n, const doub
 // (c) Injection and the persistent hop kernel
 // =============================================================================================
 // Dwell time of ff_time (scatterer.h:74-80): -(1/Gamma) * log(r / RAND_MAX), r in [1, RAND_MAX].
-// A site without neighbours never hops again (the reference's set_max_rate is undefined there).
+// A site without neighbours (the reference's set_max_rate is undefined there) or with total
+// rate 0 (1/Gamma = inf in the reference, so ff_time never falls below dt) never hops again.
+__device__ __forceinline__ bool trapped(uint32_t n, double gamma) { return n == 0u || !(gamma > 0.0); }
+
 __device__ __forceinline__ double dwell_time(uint32_t n, double gamma, uint32_t r) {
 #pragma clang fp contract(off)
-  if (n == 0u) return INFINITY;
+  if (trapped(n, gamma)) return INFINITY;
   const double inv = 1.0 / gamma;
   return -inv * log((double)r / RAND_MAX_D);
 }
@@ -486,7 +489,7 @@ __global__ __launch_bounds__(256) void k_hop(hop_args a) {
         ++k;
       }
     }
-    if (h >= h_end || row.n == 0u || !(t <= a.t_max)) break;
+    if (h >= h_end || trapped(row.n, row.gamma) || !(t <= a.t_max)) break;
 
     // event selection (scatterer::update_state, scatterer.cpp:17-30)
     const u32x4 w = draw4(a.seed, e, h, STREAM_HOP);
@@ -686,6 +689,52 @@ hipError_t launch_fill_blocks(int64_t n, const site_hdr* hdr, const double* cum,
                               hipStream_t st) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_pack_rows, grid_for(n), dim3(256), 0, st, n, hdr, cum, edge, roff, blocks);
+  return hipGetLastError();
+}
+
+// Chirality class (donor * n_chir + acceptor) of every CSR entry, and per-class index lists.
+__global__ __launch_bounds__(256) void k_pair_class(int64_t n, const site_hdr* __restrict__ hdr,
+                                                    const edge_rec* __restrict__ edge,
+                                                    const uint8_t* __restrict__ chir, int32_t n_chir,
+                                                    uint8_t* __restrict__ cls,
+                                                    uint32_t* __restrict__ counts) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const site_hdr h = hdr[i];
+  const int ci = chir[i];
+  for (uint32_t k = 0; k < h.n; ++k) {
+    const int cj = chir[edge[h.rs + k].col & EDGE_COL_MASK];
+    const int c = ci * n_chir + cj;
+    cls[h.rs + k] = (uint8_t)c;
+    atomicAdd(&counts[c], 1u);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_pair_scatter(int64_t nnz, const uint8_t* __restrict__ cls,
+                                                      const int64_t* __restrict__ starts,
+                                                      uint32_t* __restrict__ fill,
+                                                      int64_t* __restrict__ sel) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nnz) return;
+  const int c = cls[p];
+  const uint32_t slot = atomicAdd(&fill[c], 1u);
+  sel[starts[c] + slot] = p;
+}
+
+hipError_t launch_pair_classes(int64_t n, int64_t nnz, const site_hdr* hdr, const edge_rec* edge,
+                               const uint8_t* chir, int32_t n_chir, uint8_t* cls, uint32_t* counts,
+                               hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pair_class, grid_for(n), dim3(256), 0, st, n, hdr, edge, chir, n_chir, cls,
+                     counts);
+  (void)nnz;
+  return hipGetLastError();
+}
+
+hipError_t launch_pair_scatter(int64_t nnz, const uint8_t* cls, const int64_t* starts, uint32_t* fill,
+                               int64_t* sel, hipStream_t st) {
+  if (nnz == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pair_scatter, grid_for(nnz), dim3(256), 0, st, nnz, cls, starts, fill, sel);
   return hipGetLastError();
 }
 

@@ -1,0 +1,510 @@
+// kmc_transfer.hip -- exact first-order exciton-transfer rate (exciton_transfer::first_order,
+// src/exciton_transfer/exciton_transfer.cpp:380-441) for batches of pair geometries.
+//
+// Host: relevant states, partition function, energy matching and the Q factors of every
+// matched pair (exciton_transfer.cpp:226-304, exciton_transfer.h:167-196) collapse into one
+// coefficient per pair of centre-of-mass momenta (a, b):
+//     rate = sum_ab C_ab |J_ab|^2,
+//     C_ab = sum_{matched (i,f) with momenta (a,b)} (2 pi / hb) (exp(-E_i/kT) / Z)
+//                 |Q_if|^2 / (L_i L_f) * Lorentzian(E_i - E_f),
+// since M = |Q J| / sqrt(L_i L_f) (exciton_transfer.cpp:415) and J depends on the two states only
+// through their momenta.  Device: J_ab = sum_i sum_j u_a(i) v_b(j) / |R_i - R_j| over the A-site
+// unit cells of both tubes (exciton_transfer.cpp:307-377), one workgroup per geometry, the
+// acceptor atoms staged in LDS tiles after the shift / rotation of make_Ru_3d.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cntkmc.h"
+#include "cnt_model.h"
+
+using namespace cntkmc;
+
+
+namespace {
+
+constexpr int MAXMOM = 16;    // centre-of-mass momenta per tube handled by the kernel
+constexpr int MAXTERMS = 256; // nonzero C_ab terms (one owning thread each)
+constexpr int TILE = 128;     // acceptor atoms per LDS tile
+
+struct ex_state {
+  int ik_cm_idx, n;
+  double energy;
+  int ik_cm;
+};
+
+// get_relevant_states (exciton_transfer.cpp:226-270)
+std::vector<ex_state> relevant_states(const cnt_model& c, double min_energy, double T) {
+  const double thr = min_energy + std::abs(std::log(1.e-3) * phys::kb * T);
+  std::vector<ex_state> out;
+  for (int n = 0; n < c.nk_relev; n++)
+    for (int ic = 0; ic < c.nk_cm; ic++) {
+      const double e = c.ex_energy[(size_t)ic * c.nk_relev + n];
+      if (e <= thr) out.push_back({ic, n, e, c.ik_cm0 + ic});
+    }
+  std::stable_sort(out.begin(), out.end(),
+                   [](const ex_state& a, const ex_state& b) { return a.energy < b.energy; });
+  return out;
+}
+
+// Q_partial of calculate_Q (exciton_transfer.cpp:276-296)
+cplx q_partial(const cnt_model& c, const ex_state& s) {
+  const double kx = s.ik_cm * c.dk_l.x, ky = s.ik_cm * c.dk_l.y;
+  const cplx ef0 = std::exp(cplx(0., +1.) * (kx * 0.0 + ky * 0.0));
+  const cplx ef1 = std::exp(cplx(0., +1.) * (kx * c.aCC.x + ky * c.aCC.y));
+  cplx qp = 0;
+  const cplx* psi = &c.ex_psi[((size_t)s.ik_cm_idx * c.nk_relev + s.n) * c.nk_c];
+  for (int k = 0; k < c.nk_c; k++) {
+    const int32_t* id = &c.ex_idx[((size_t)s.ik_cm_idx * c.nk_c + k) * 4];
+    const cplx ccA = c.wf(id[1], id[0], 1, 0), ccB = c.wf(id[1], id[0], 1, 1);
+    const cplx cvA = c.wf(id[3], id[2], 0, 0), cvB = c.wf(id[3], id[2], 0, 1);
+    qp += psi[k] * (ccA * std::conj(cvA) * ef0 + ccB * std::conj(cvB) * ef1);
+  }
+  return qp;
+}
+
+// make_Ru_3d before shift and rotation (exciton_transfer.cpp:310-322): all unit cells, centred
+// in y.  make_Ru_2d (exciton_transfer.cpp:341-351): the unrolled positions.
+void tube_atoms(const cnt_model& c, std::vector<double>& r3, std::vector<vec2>& r2) {
+  const int n = c.Nu * c.length;
+  r3.assign((size_t)n * 4, 0.0);
+  r2.assign(n, vec2{});
+  for (int i = 0; i < c.length; i++)
+    for (int a = 0; a < c.Nu; a++) {
+      const int k = i * c.Nu + a;
+      r3[4 * k + 0] = c.pos_u_3d[a].x + i * c.t_vec_3d.x;
+      r3[4 * k + 1] = c.pos_u_3d[a].y + i * c.t_vec_3d.y;
+      r3[4 * k + 2] = c.pos_u_3d[a].z + i * c.t_vec_3d.z;
+      r2[k] = {c.pos_a[a].x + i * c.t_vec.x, c.pos_a[a].y + i * c.t_vec.y};
+    }
+  double ymax = r3[1], ymin = r3[1];
+  for (int k = 0; k < n; k++) {
+    ymax = std::max(ymax, r3[4 * k + 1]);
+    ymin = std::min(ymin, r3[4 * k + 1]);
+  }
+  const double mid = (ymax + ymin) / 2.;
+  for (int k = 0; k < n; k++) r3[4 * k + 1] -= mid;
+}
+
+// ---------------------------------------------------------------------------------------------
+// device: one workgroup per geometry
+// ---------------------------------------------------------------------------------------------
+struct exact_args {
+  int32_t na, nb;                 // atoms per tube
+  int32_t nmb;                    // acceptor momenta (padded to the kernel's NMB)
+  int32_t n_terms;                // nonzero (a, b) terms
+  const double4* donor;           // [na] centred positions (w unused)
+  const double4* acceptor;        // [nb]
+  const double2* u;               // [nma][na]  exp(-i k_a . r)
+  const double2* v;               // [nmb][nb]  exp(+i k_b . r)
+  const int32_t* term_a;          // [n_terms] donor momentum slot
+  const int32_t* term_b;          // [n_terms] acceptor momentum slot
+  const double* term_c;           // [n_terms] C_ab
+  int64_t n_geom;
+  const double *theta, *z, *a1, *a2;
+  const int64_t* sel;             // optional: geometry g reads entry sel[g] and writes out[sel[g]]
+  double* out;
+  double clamp_lo[3], clamp_hi[3];  // z, a1, a2 box
+};
+
+__device__ __forceinline__ double rsqrt_f64(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  y = y * (1.5 - 0.5 * x * y * y);
+  y = y * (1.5 - 0.5 * x * y * y);
+  return y;
+}
+
+// One workgroup of 256 threads per geometry.  Donor atoms go in chunks of 256 (one per
+// thread); for its atom i a thread accumulates W_b(i) = sum_j v_b(j) / |R_i - R_j| over
+// acceptor tiles staged in LDS (positions after make_Ru_3d's shift and rotation).  The chunk's
+// W goes to LDS and thread t < n_terms adds sum_i u_a(i) W_b(i) of its term (a, b) to J_ab.
+template <int NMB>
+__global__ __launch_bounds__(256) void k_exact_rate(exact_args a) {
+  __shared__ double4 s_pos[TILE];
+  __shared__ double2 s_v[NMB][TILE];
+  __shared__ double2 s_w[NMB][256];
+  __shared__ double s_red[4];
+  const int tid = threadIdx.x;
+  for (int64_t g = blockIdx.x; g < a.n_geom; g += gridDim.x) {
+    const int64_t e = a.sel ? a.sel[g] : g;
+    const double th = a.theta[e];
+    const double zs = fmin(fmax(a.z[e], a.clamp_lo[0]), a.clamp_hi[0]);
+    const double s1 = fmin(fmax(a.a1[e], a.clamp_lo[1]), a.clamp_hi[1]);
+    const double s2 = fmin(fmax(a.a2[e], a.clamp_lo[2]), a.clamp_hi[2]);
+    double sth, cth;
+    sincos(th, &sth, &cth);
+    const bool owner = tid < a.n_terms;
+    const int ta = owner ? a.term_a[tid] : 0, tb = owner ? a.term_b[tid] : 0;
+    double jr = 0.0, ji = 0.0;
+    for (int i0 = 0; i0 < a.na; i0 += 256) {
+      const int i = i0 + tid;
+      const bool live = i < a.na;
+      double4 ri = live ? a.donor[i] : make_double4(0, 0, 0, 0);
+      ri.y += s1;
+      double wr[NMB], wi[NMB];
+#pragma unroll
+      for (int q = 0; q < NMB; q++) wr[q] = wi[q] = 0.0;
+      for (int j0 = 0; j0 < a.nb; j0 += TILE) {
+        __syncthreads();
+        if (tid < TILE && j0 + tid < a.nb) {
+          const int j = j0 + tid;
+          const double4 b = a.acceptor[j];
+          const double by = b.y + s2;
+          s_pos[tid] = make_double4(b.x * cth - by * sth, b.x * sth + by * cth, b.z + zs, 0.0);
+#pragma unroll
+          for (int q = 0; q < NMB; q++) s_v[q][tid] = a.v[(size_t)q * a.nb + j];
+        }
+        __syncthreads();
+        const int tn = min(TILE, a.nb - j0);
+        if (live) {
+          for (int t = 0; t < tn; t++) {
+            const double4 p = s_pos[t];
+            const double dx = ri.x - p.x, dy = ri.y - p.y, dz = ri.z - p.z;
+            const double r = rsqrt_f64(dx * dx + dy * dy + dz * dz);
+#pragma unroll
+            for (int q = 0; q < NMB; q++) {
+              wr[q] += s_v[q][t].x * r;
+              wi[q] += s_v[q][t].y * r;
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < NMB; q++) s_w[q][tid] = live ? make_double2(wr[q], wi[q]) : make_double2(0, 0);
+      __syncthreads();
+      if (owner) {
+        const int cn = min(256, a.na - i0);
+        const double2* ua = a.u + (size_t)ta * a.na + i0;
+        for (int k = 0; k < cn; k++) {
+          const double2 uu = ua[k];
+          const double2 w = s_w[tb][k];
+          jr += uu.x * w.x - uu.y * w.y;
+          ji += uu.x * w.y + uu.y * w.x;
+        }
+      }
+    }
+    // rate = sum over terms of C_ab |J_ab|^2
+    double part = owner ? a.term_c[tid] * (jr * jr + ji * ji) : 0.0;
+    for (int off = 32; off > 0; off >>= 1) part += __shfl_down(part, off, 64);
+    if ((tid & 63) == 0) s_red[tid >> 6] = part;
+    __syncthreads();
+    if (tid == 0) a.out[e] = (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]);
+    __syncthreads();
+  }
+}
+
+int padded_nmb(int nmb) { return nmb <= 2 ? 2 : nmb <= 4 ? 4 : nmb <= 8 ? 8 : nmb <= 12 ? 12 : 16; }
+
+hipError_t launch_exact(const exact_args& a, hipStream_t st) {
+  if (a.n_geom == 0) return hipSuccess;
+  const unsigned grid = (unsigned)std::min<int64_t>(a.n_geom, 65536);
+  switch (a.nmb) {
+    case 2: hipLaunchKernelGGL(k_exact_rate<2>, dim3(grid), dim3(256), 0, st, a); break;
+    case 4: hipLaunchKernelGGL(k_exact_rate<4>, dim3(grid), dim3(256), 0, st, a); break;
+    case 8: hipLaunchKernelGGL(k_exact_rate<8>, dim3(grid), dim3(256), 0, st, a); break;
+    case 12: hipLaunchKernelGGL(k_exact_rate<12>, dim3(grid), dim3(256), 0, st, a); break;
+    case 16: hipLaunchKernelGGL(k_exact_rate<16>, dim3(grid), dim3(256), 0, st, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+thread_local std::string g_tr_err;
+int tfail(int code, const std::string& m) {
+  g_tr_err = m;
+  return code;
+}
+
+}  // namespace
+
+struct kmc_transfer {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  kmc_transfer_info info{};
+  std::vector<int> mom_a, mom_b;            // ik_cm of each momentum slot
+  std::vector<int32_t> term_a, term_b;      // nonzero terms
+  std::vector<double> term_c;
+  int nmb_pad = 2;
+  double clamp_lo[3] = {-1e300, -1e300, -1e300}, clamp_hi[3] = {1e300, 1e300, 1e300};
+  std::vector<double> h_donor, h_acc;
+  std::vector<vec2> r2d, r2a;
+  std::vector<cplx> h_u, h_v;
+  bool on_device = false;
+  double4 *d_donor = nullptr, *d_acc = nullptr;
+  double2 *d_u = nullptr, *d_v = nullptr;
+  int32_t *d_ta = nullptr, *d_tb = nullptr;
+  double* d_tc = nullptr;
+  double* d_geo = nullptr;  // scratch for host-array calls: 4 n doubles
+  double* d_out = nullptr;
+  size_t geo_cap = 0;
+};
+
+namespace {
+
+template <typename T>
+hipError_t put(T** d, const std::vector<T>& h) {
+  hipError_t e = hipMalloc((void**)d, std::max<size_t>(1, h.size()) * sizeof(T));
+  if (e == hipSuccess && !h.empty()) e = hipMemcpy(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice);
+  return e;
+}
+
+int upload(kmc_transfer* t) {
+  if (t->on_device) return KMC_OK;
+  hipError_t e = hipSetDevice(t->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&t->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&t->ev1);
+  if (e == hipSuccess) e = put((double**)&t->d_donor, t->h_donor);
+  if (e == hipSuccess) e = put((double**)&t->d_acc, t->h_acc);
+  if (e == hipSuccess) e = put((cplx**)&t->d_u, t->h_u);
+  if (e == hipSuccess) e = put((cplx**)&t->d_v, t->h_v);
+  if (e == hipSuccess) e = put(&t->d_ta, t->term_a);
+  if (e == hipSuccess) e = put(&t->d_tb, t->term_b);
+  if (e == hipSuccess) e = put(&t->d_tc, t->term_c);
+  if (e != hipSuccess)
+    return tfail(e == hipErrorNoDevice ? KMC_ERR_NODEVICE : KMC_ERR_HIP,
+                 std::string("kmc_transfer: device setup failed: ") + hipGetErrorString(e));
+  t->on_device = true;
+  return KMC_OK;
+}
+
+exact_args args_of(const kmc_transfer* t) {
+  exact_args a{};
+  a.na = t->info.n_atoms_donor;
+  a.nb = t->info.n_atoms_acceptor;
+  a.nmb = t->nmb_pad;
+  a.n_terms = t->info.n_terms;
+  a.donor = t->d_donor;
+  a.acceptor = t->d_acc;
+  a.u = t->d_u;
+  a.v = t->d_v;
+  a.term_a = t->d_ta;
+  a.term_b = t->d_tb;
+  a.term_c = t->d_tc;
+  for (int k = 0; k < 3; k++) {
+    a.clamp_lo[k] = t->clamp_lo[k];
+    a.clamp_hi[k] = t->clamp_hi[k];
+  }
+  return a;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* kmc_transfer_last_error(void) { return g_tr_err.c_str(); }
+
+int kmc_transfer_create(int32_t device, const kmc_cnt* donor, const kmc_cnt* acceptor,
+                        double temperature, double broadening, kmc_transfer** out) {
+  if (!donor || !acceptor || !out || !(temperature > 0) || !(broadening > 0))
+    return tfail(KMC_ERR_ARG, "kmc_transfer_create: bad argument");
+  *out = nullptr;
+  const cnt_model& D = donor->m;
+  const cnt_model& A = acceptor->m;
+  using namespace phys;
+  // first_order (exciton_transfer.cpp:385-400)
+  double min_energy = D.ex_energy[0];
+  for (double e : D.ex_energy) min_energy = std::min(min_energy, e);
+  const std::vector<ex_state> ds = relevant_states(D, min_energy, temperature);
+  const std::vector<ex_state> as = relevant_states(A, min_energy, temperature);
+  double Z = 0;
+  for (const auto& s : ds) Z += std::exp(-s.energy / (kb * temperature));
+  auto lorentzian = [&](double e) { return inv_pi * broadening / (e * e + broadening * broadening); };
+  kmc_transfer* t = new kmc_transfer();
+  t->device = device;
+  // calculate_Q coefficient (exciton_transfer.cpp:298-300)
+  const double coeff = (std::pow(q0, 2) * D.Au() * A.Au()) /
+                       (16 * std::pow(pi, 3) * eps0 * D.radius * A.radius *
+                        std::sqrt(D.length_in_meter() * A.length_in_meter()));
+  const double LiLf = D.length_in_meter() * A.length_in_meter();
+  std::vector<cplx> qd(ds.size()), qa(as.size());
+  for (size_t i = 0; i < ds.size(); i++) qd[i] = q_partial(D, ds[i]);
+  for (size_t f = 0; f < as.size(); f++) qa[f] = q_partial(A, as[f]);
+  auto slot = [](std::vector<int>& v, int k) {
+    for (size_t i = 0; i < v.size(); i++)
+      if (v[i] == k) return (int)i;
+    v.push_back(k);
+    return (int)v.size() - 1;
+  };
+  std::vector<double> C((size_t)MAXMOM * MAXMOM, 0.0);
+  int n_pairs = 0;
+  // match_states (exciton_transfer.h:167-196) and the sum of first_order (:411-427)
+  for (size_t i = 0; i < ds.size(); i++)
+    for (size_t f = 0; f < as.size(); f++) {
+      const double de = ds[i].energy - as[f].energy;
+      if (!(lorentzian(de) > 1.e-2 * lorentzian(0))) continue;
+      ++n_pairs;
+      const cplx Q = cplx(coeff) * std::conj(qd[i]) * qa[f];
+      const double w = (2 * pi / hb) * (std::exp(-ds[i].energy / (kb * temperature)) / Z) *
+                       std::norm(Q) / LiLf * lorentzian(de);
+      const int p = slot(t->mom_a, ds[i].ik_cm), q = slot(t->mom_b, as[f].ik_cm);
+      if (p >= MAXMOM || q >= MAXMOM) {
+        delete t;
+        return tfail(KMC_ERR_CAPACITY, "kmc_transfer_create: more than 16 centre-of-mass "
+                                       "momenta per tube among the matched states");
+      }
+      C[(size_t)p * MAXMOM + q] += w;
+    }
+  if (t->mom_a.empty()) t->mom_a.push_back(0);
+  if (t->mom_b.empty()) t->mom_b.push_back(0);
+  const int nma = (int)t->mom_a.size(), nmb = (int)t->mom_b.size();
+  for (int p = 0; p < nma; p++)
+    for (int q = 0; q < nmb; q++)
+      if (C[(size_t)p * MAXMOM + q] != 0.0) {
+        t->term_a.push_back(p);
+        t->term_b.push_back(q);
+        t->term_c.push_back(C[(size_t)p * MAXMOM + q]);
+      }
+  if ((int)t->term_c.size() > MAXTERMS) {
+    delete t;
+    return tfail(KMC_ERR_CAPACITY, "kmc_transfer_create: more than 256 momentum terms");
+  }
+  t->nmb_pad = padded_nmb(nmb);
+  kmc_transfer_info& in = t->info;
+  in.n_donor_states = (int)ds.size();
+  in.n_acceptor_states = (int)as.size();
+  in.n_pairs = n_pairs;
+  in.n_mom_donor = nma;
+  in.n_mom_acceptor = nmb;
+  in.n_terms = (int)t->term_c.size();
+  in.n_atoms_donor = D.Nu * D.length;
+  in.n_atoms_acceptor = A.Nu * A.length;
+  in.partition_Z = Z;
+  in.min_energy = min_energy;
+  // host copies of the device data; uploaded on the first rate call
+  tube_atoms(D, t->h_donor, t->r2d);
+  tube_atoms(A, t->h_acc, t->r2a);
+  t->h_u.assign((size_t)nma * t->r2d.size(), cplx(0, 0));
+  t->h_v.assign((size_t)t->nmb_pad * t->r2a.size(), cplx(0, 0));
+  for (int p = 0; p < nma; p++) {
+    const double kx = t->mom_a[p] * D.dk_l.x, ky = t->mom_a[p] * D.dk_l.y;
+    for (size_t i = 0; i < t->r2d.size(); i++)
+      t->h_u[p * t->r2d.size() + i] = std::exp(-cplx(0, 1) * (kx * t->r2d[i].x + ky * t->r2d[i].y));
+  }
+  for (int q = 0; q < nmb; q++) {
+    const double kx = t->mom_b[q] * A.dk_l.x, ky = t->mom_b[q] * A.dk_l.y;
+    for (size_t j = 0; j < t->r2a.size(); j++)
+      t->h_v[q * t->r2a.size() + j] = std::exp(+cplx(0, 1) * (kx * t->r2a[j].x + ky * t->r2a[j].y));
+  }
+  *out = t;
+  return KMC_OK;
+}
+
+int kmc_transfer_destroy(kmc_transfer* t) {
+  if (!t) return KMC_OK;
+  if (t->on_device) {
+    (void)hipSetDevice(t->device);
+    if (t->stream) (void)hipStreamSynchronize(t->stream);
+    void* bufs[] = {t->d_donor, t->d_acc, t->d_u, t->d_v, t->d_ta, t->d_tb, t->d_tc, t->d_geo, t->d_out};
+    for (void* b : bufs)
+      if (b) (void)hipFree(b);
+    if (t->ev0) (void)hipEventDestroy(t->ev0);
+    if (t->ev1) (void)hipEventDestroy(t->ev1);
+    if (t->stream) (void)hipStreamDestroy(t->stream);
+  }
+  delete t;
+  return KMC_OK;
+}
+
+int kmc_transfer_get_info(const kmc_transfer* t, kmc_transfer_info* info) {
+  if (!t || !info) return tfail(KMC_ERR_ARG, "kmc_transfer_get_info: null argument");
+  *info = t->info;
+  return KMC_OK;
+}
+
+int kmc_transfer_terms(const kmc_transfer* t, int32_t* ik_cm_donor, int32_t* ik_cm_acceptor,
+                       double* coef) {
+  if (!t) return tfail(KMC_ERR_ARG, "kmc_transfer_terms: null argument");
+  for (int k = 0; k < t->info.n_terms; k++) {
+    if (ik_cm_donor) ik_cm_donor[k] = t->mom_a[t->term_a[k]];
+    if (ik_cm_acceptor) ik_cm_acceptor[k] = t->mom_b[t->term_b[k]];
+    if (coef) coef[k] = t->term_c[k];
+  }
+  return KMC_OK;
+}
+
+int kmc_transfer_set_clamp(kmc_transfer* t, const double lo[3], const double hi[3]) {
+  if (!t || !lo || !hi) return tfail(KMC_ERR_ARG, "kmc_transfer_set_clamp: null argument");
+  for (int k = 0; k < 3; k++) {
+    if (!(lo[k] <= hi[k])) return tfail(KMC_ERR_ARG, "kmc_transfer_set_clamp: lo > hi");
+    t->clamp_lo[k] = lo[k];
+    t->clamp_hi[k] = hi[k];
+  }
+  return KMC_OK;
+}
+
+int kmc_transfer_rates_device(kmc_transfer* t, int64_t n, const double* d_theta, const double* d_z,
+                              const double* d_a1, const double* d_a2, const int64_t* d_sel,
+                              double* d_rate) {
+  if (!t || n < 0 || (n > 0 && (!d_theta || !d_z || !d_a1 || !d_a2 || !d_rate)))
+    return tfail(KMC_ERR_ARG, "kmc_transfer_rates_device: bad argument");
+  if (n == 0) return KMC_OK;
+  {
+    const int s = upload(t);
+    if (s != KMC_OK) return s;
+  }
+  exact_args a = args_of(t);
+  a.n_geom = n;
+  a.theta = d_theta;
+  a.z = d_z;
+  a.a1 = d_a1;
+  a.a2 = d_a2;
+  a.sel = d_sel;
+  a.out = d_rate;
+  hipError_t e = hipSetDevice(t->device);
+  if (e == hipSuccess) e = launch_exact(a, t->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(t->stream);
+  if (e != hipSuccess) return tfail(KMC_ERR_HIP, std::string("kmc_transfer_rates: ") + hipGetErrorString(e));
+  return KMC_OK;
+}
+
+int kmc_transfer_rates(kmc_transfer* t, int64_t n, const double* theta, const double* z,
+                       const double* a1, const double* a2, double* rate, double* kernel_ms) {
+  if (!t || n < 0 || (n > 0 && (!theta || !z || !a1 || !a2 || !rate)))
+    return tfail(KMC_ERR_ARG, "kmc_transfer_rates: bad argument");
+  if (n == 0) return KMC_OK;
+  {
+    const int s = upload(t);
+    if (s != KMC_OK) return s;
+  }
+  hipError_t e = hipSetDevice(t->device);
+  if (e == hipSuccess && (size_t)n > t->geo_cap) {
+    if (t->d_geo) (void)hipFree(t->d_geo);
+    if (t->d_out) (void)hipFree(t->d_out);
+    t->d_geo = nullptr;
+    t->d_out = nullptr;
+    e = hipMalloc((void**)&t->d_geo, 4 * n * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void**)&t->d_out, n * sizeof(double));
+    t->geo_cap = e == hipSuccess ? (size_t)n : 0;
+  }
+  const double* src[4] = {theta, z, a1, a2};
+  for (int q = 0; q < 4 && e == hipSuccess; q++)
+    e = hipMemcpyAsync(t->d_geo + q * n, src[q], n * sizeof(double), hipMemcpyHostToDevice, t->stream);
+  exact_args a = args_of(t);
+  a.n_geom = n;
+  a.theta = t->d_geo;
+  a.z = t->d_geo + n;
+  a.a1 = t->d_geo + 2 * n;
+  a.a2 = t->d_geo + 3 * n;
+  a.sel = nullptr;
+  a.out = t->d_out;
+  if (e == hipSuccess) e = hipEventRecord(t->ev0, t->stream);
+  if (e == hipSuccess) e = launch_exact(a, t->stream);
+  if (e == hipSuccess) e = hipEventRecord(t->ev1, t->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(rate, t->d_out, n * sizeof(double), hipMemcpyDeviceToHost, t->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(t->stream);
+  if (e == hipSuccess && kernel_ms) {
+    float ms = 0.f;
+    e = hipEventElapsedTime(&ms, t->ev0, t->ev1);
+    *kernel_ms = ms;
+  }
+  if (e != hipSuccess) return tfail(KMC_ERR_HIP, std::string("kmc_transfer_rates: ") + hipGetErrorString(e));
+  return KMC_OK;
+}
+
+}  // extern "C"

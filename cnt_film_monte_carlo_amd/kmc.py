@@ -100,6 +100,9 @@ _SIGS = {
     "kmc_forster_table": (C.c_int, [C.c_double, C.c_int32, _D, C.c_int32, _D, C.c_int32, _D,
                                     C.c_int32, _D, _D]),
     "kmc_linspace": (C.c_int, [C.c_double, C.c_double, C.c_int32, _D]),
+    "kmc_build_rates_exact": (C.c_int, [_P, C.c_int32, C.POINTER(_P), _D]),
+    "kmc_pair_classes": (C.c_int, [_P, C.c_int32, _I64, C.POINTER(_P)]),
+    "kmc_get_pair_rates": (C.c_int, [_P, _D]),
 }
 
 _lib = None
@@ -263,6 +266,26 @@ class Context:
 
     def set_pair_rates_device(self, dev_ptr: int):
         self._ck(self.lib.kmc_set_pair_rates_device(self.h, C.c_void_p(dev_ptr)))
+
+    def build_rates_exact(self, transfers) -> float:
+        """Per-pair exact rates; transfers[ci * n_chir + cj] (None -> rate 0).  Returns ms."""
+        n2 = len(transfers)
+        n_chir = int(round(math.sqrt(n2)))
+        arr = (_P * n2)(*[None if t is None else t.h for t in transfers])
+        ms = C.c_double(0.0)
+        self._ck(self.lib.kmc_build_rates_exact(self.h, n_chir, arr, C.byref(ms)))
+        return ms.value
+
+    def pair_classes(self, n_chir: int):
+        start = np.zeros(n_chir * n_chir + 1, np.int64)
+        dsel = _P()
+        self._ck(self.lib.kmc_pair_classes(self.h, n_chir, _ptr(start, _I64), C.byref(dsel)))
+        return start, dsel.value
+
+    def get_pair_rates(self):
+        r = np.zeros(self.nnz)
+        self._ck(self.lib.kmc_get_pair_rates(self.h, _ptr(r, _D)))
+        return r
 
     def get_csr(self):
         m = self.nnz

@@ -194,6 +194,100 @@ int kmc_forster_table(double gamma_0, int32_t n_theta, const double* theta, int3
 /* arma::linspace as used for the table grids (monte_carlo.cpp:66-75).  Host only. */
 int kmc_linspace(double start, double end, int32_t n, double* out);
 
+/* ---- CNT exciton model (host only; no device work) ------------------------------------
+ * Restates cnt::calculate_exciton_dispersion (src/exciton_transfer/cnt.cpp:1056-1081):
+ * get_parameters (:16-117), get_atom_coordinates (:120-209), electron_energy in the
+ * K2-extended zone (:301-358), find_valleys (:362-416), find_relev_ik_range (:419-517),
+ * calculate_vq (:520-631), calculate_polarization (:634-732), calculate_dielectric
+ * (:735-780) and the A2 singlet exciton of calculate_A_excitons (:783-1053). */
+typedef struct kmc_cnt kmc_cnt;
+
+typedef struct kmc_cnt_info {
+  int32_t n, m, length_cells;           /* chirality (n,m), length in cnt unit cells        */
+  int32_t Nu, Q, M, t1, t2;             /* cnt.h:35-53                                       */
+  int32_t nk, n_mu;                     /* K2-extended zone: ik in [0,nk), mu in [0,n_mu)   */
+  int32_t nk_relev, nk_cm, nk_c;        /* exciton basis sizes (cnt.cpp:876-878)             */
+  int32_t ik_cm0;                       /* ik_cm of dispersion row 0 (= -nk_relev)          */
+  double radius, length_m, Au, a_l;     /* [m], [m], [m^2], [m]                              */
+  double ch_vec[2], t_vec[2], aCC[2], dk_l[2], K1[2], K2[2]; /* rotated frame (cnt.cpp:44-58) */
+  double e_min;                         /* lowest A2 singlet energy [J]                      */
+} kmc_cnt_info;
+
+/* Build the model of a (n,m) tube of length_cells unit cells (threads <= 0: all host cores). */
+int kmc_cnt_create(int32_t n, int32_t m, int32_t length_cells, int32_t threads, kmc_cnt** out);
+int kmc_cnt_destroy(kmc_cnt* c);
+const char* kmc_cnt_last_error(void);
+int kmc_cnt_get_info(const kmc_cnt* c, kmc_cnt_info* info);
+/* A2 singlet energies [nk_cm][nk_relev] (ascending per ik_cm) [J]. */
+int kmc_cnt_dispersion(const kmc_cnt* c, double* energy);
+/* Electron energies [2][nk][n_mu] (band 0 valence, 1 conduction) and wavefunctions
+ * [n_mu][nk][band][atom] as interleaved complex doubles. */
+int kmc_cnt_electrons(const kmc_cnt* c, double* energy, double* wavefunc);
+/* v(q) [nq][2 n_mu - 1][4] complex, PI and epsilon [nq][2 n_mu - 1], nq = 2 nk - 1. */
+int kmc_cnt_interaction(const kmc_cnt* c, double* vq, double* PI, double* eps);
+/* A2 singlet wavefunctions [nk_cm][nk_relev][nk_c] complex and their electron-state indices
+ * [nk_cm][nk_c][4] = {ik_c, mu_c, ik_v, mu_v} (cnt.h:136-146). */
+int kmc_cnt_exciton_states(const kmc_cnt* c, double* psi, int32_t* idx);
+/* The Hermitian A2 singlet kernel at one ik_cm index, column-major complex nk_relev^2. */
+int kmc_cnt_singlet_kernel(const kmc_cnt* c, int32_t ik_cm_idx, double* kernel);
+/* A and B atom positions of one unit cell [Nu][2] (unrolled) and rolled unit cells [Nu][3]. */
+int kmc_cnt_atoms(const kmc_cnt* c, double* pos_a, double* pos_b, double* pos_u_3d);
+/* Eigen-decomposition of a Hermitian matrix (column-major complex n x n): ascending
+ * eigenvalues and eigenvectors in columns; stands in for arma::eig_sym (cnt.cpp:950-965). */
+int kmc_hermitian_eig(int32_t n, const double* a, double* evals, double* vecs);
+
+/* ---- exact first-order transfer rate ---------------------------------------------------
+ * exciton_transfer::first_order (src/exciton_transfer/exciton_transfer.cpp:380-441) for a
+ * donor / acceptor tube pair at temperature T [K] and Lorentzian broadening [J] (the reference
+ * hard-codes 300 K and 4 meV, exciton_transfer.h:41-42).  Matched states collapse into
+ * coefficients C_ab per pair of centre-of-mass momenta; rate = sum_ab C_ab |J_ab|^2 with the
+ * Coulomb sum J_ab of calculate_J (exciton_transfer.cpp:307-377) evaluated on the device. */
+typedef struct kmc_transfer kmc_transfer;
+
+typedef struct kmc_transfer_info {
+  int32_t n_donor_states, n_acceptor_states, n_pairs; /* relevant states, matched pairs     */
+  int32_t n_mom_donor, n_mom_acceptor;                /* distinct ik_cm per side (<= 16)      */
+  int32_t n_terms;                                    /* nonzero C_ab (<= 256)                */
+  int32_t n_atoms_donor, n_atoms_acceptor;            /* A-site unit cells per tube           */
+  double partition_Z, min_energy;                     /* exciton_transfer.cpp:385, 394-397    */
+} kmc_transfer_info;
+
+int kmc_transfer_create(int32_t device, const kmc_cnt* donor, const kmc_cnt* acceptor,
+                        double temperature, double broadening, kmc_transfer** out);
+int kmc_transfer_destroy(kmc_transfer* t);
+const char* kmc_transfer_last_error(void);
+int kmc_transfer_get_info(const kmc_transfer* t, kmc_transfer_info* info);
+/* The n_terms nonzero terms: donor / acceptor centre-of-mass momenta (ik_cm) and C_ab [1/s/m^2]. */
+int kmc_transfer_terms(const kmc_transfer* t, int32_t* ik_cm_donor, int32_t* ik_cm_acceptor,
+                       double* coef);
+/* Clamp (z_shift, axis_shift_1, axis_shift_2) into [lo, hi] before evaluation (default: none).
+ * With the table grid's box this is the continuous counterpart of the nearest-node clamp of
+ * scattering_struct::get_rate (scattering_struct.h:42-49). */
+int kmc_transfer_set_clamp(kmc_transfer* t, const double lo[3], const double hi[3]);
+/* Rates for n geometries (host arrays); kernel_ms (may be NULL) = device time. */
+int kmc_transfer_rates(kmc_transfer* t, int64_t n, const double* theta, const double* z_shift,
+                       const double* axis_shift_1, const double* axis_shift_2, double* rate,
+                       double* kernel_ms);
+/* Same on device arrays; with d_sel != NULL geometry g is entry d_sel[g] (read and written). */
+int kmc_transfer_rates_device(kmc_transfer* t, int64_t n, const double* d_theta,
+                              const double* d_z, const double* d_a1, const double* d_a2,
+                              const int64_t* d_sel, double* d_rate);
+
+/* Per-pair exact rates for a KMC_RATE_PAIR build: CSR entries are grouped by chirality class
+ * (donor * n_chir + acceptor) on the device and class k is evaluated with transfers[k] (NULL:
+ * rate 0) at the pair's (theta, z_shift, axis_shift_1, axis_shift_2); then the FP64 prefix sums
+ * finish the CSR.  This replaces both the table fill of create_davoody_scatt_table
+ * (src/monte_carlo/monte_carlo.cpp:108-134) and its nearest-node lookup: every pair gets
+ * first_order at its own geometry (clamped as set by kmc_transfer_set_clamp).  elapsed_ms
+ * (may be NULL) = wall time of the whole rate build. */
+int kmc_build_rates_exact(kmc_ctx* ctx, int32_t n_chir, kmc_transfer* const* transfers,
+                          double* elapsed_ms);
+/* Chirality classes of the CSR entries: class_start[n_chir^2 + 1] (host) and the device index
+ * list (entries of class k at d_sel[class_start[k] .. class_start[k+1])). */
+int kmc_pair_classes(kmc_ctx* ctx, int32_t n_chir, int64_t* class_start, const int64_t** d_sel);
+/* Per-pair rates of the last pair-rate build (host copy, nnz doubles). */
+int kmc_get_pair_rates(kmc_ctx* ctx, double* rates);
+
 /* Library version string. */
 const char* kmc_version(void);
 

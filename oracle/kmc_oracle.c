@@ -506,7 +506,7 @@ int orc_select_index(const double* cum, int n, double dice) {
 /* ff_time (scatterer.h:74-80) with the cached inverse rate of set_max_rate (scatterer.h:92) */
 static double orc_dwell(const double* row, int64_t n, uint32_t r) {
   double max_rate, inv;
-  if (n == 0) return INFINITY;
+  if (n == 0 || !(row[n - 1] > 0.0)) return INFINITY; /* no neighbour, or 1/Gamma = inf */
   max_rate = row[n - 1];
   inv = 1. / max_rate;
   return -inv * log((double)r / (double)ORC_RAND_MAX);
@@ -654,7 +654,7 @@ static void orc_one(const orc_run_args* a, int64_t e, orc_cand* nb, double* rowb
       rdisp = a->disp + 3 * a->row_ptr[s];
       n = a->row_ptr[s + 1] - a->row_ptr[s];
     }
-    if (h >= h_end || n == 0 || !(t <= a->t_max)) break;
+    if (h >= h_end || n == 0 || !(row[n - 1] > 0.0) || !(t <= a->t_max)) break;
 
     draws(a->seed, (uint32_t)e, h, 0u, w);
     dice = row[n - 1] * (double)(w[0] >> 1) / ORC_RAND_MAX;
@@ -691,7 +691,7 @@ static void orc_one(const orc_run_args* a, int64_t e, orc_cand* nb, double* rowb
           rate = get_rate(w0, w0->chir[j], w0->chir[jj], th, zs, s1, s2);
           gj = (q == 0) ? rate : rate + gj;
         }
-        tj = (nj == 0) ? INFINITY : orc_dwell_time(gj, r);
+        tj = (nj == 0 || !(gj > 0.0)) ? INFINITY : orc_dwell_time(gj, r);
       } else {
         tj = orc_dwell(a->cum + a->row_ptr[j], a->row_ptr[j + 1] - a->row_ptr[j], r);
       }
@@ -783,4 +783,178 @@ int orc_max_threads(void) {
 #else
   return 1;
 #endif
+}
+
+/* ------------------------------------------------------------------------------------------
+ * Exact first-order transfer rate: a literal restatement of exciton_transfer::first_order
+ * (src/exciton_transfer/exciton_transfer.cpp:380-441) with get_relevant_states (:226-270),
+ * match_states (exciton_transfer.h:167-196), calculate_Q (:273-304) and the per-pair double
+ * loop of calculate_J (:307-377).  The exciton model arrays come from the product's CNT model
+ * (they are pinned separately: closed forms and LAPACK eigenvalues).
+ * ---------------------------------------------------------------------------------------- */
+#include <complex.h>
+
+typedef struct orc_tube {
+  int Nu, L, nk, n_mu, nr, nk_cm, nk_c, ik_cm0;
+  const double* energy;   /* [nk_cm][nr] */
+  const double* psi;      /* complex [nk_cm][nr][nk_c] */
+  const int32_t* idx;     /* [nk_cm][nk_c][4] */
+  const double* wf;       /* complex [n_mu][nk][band][atom] */
+  double dk_l[2], aCC[2], t_vec[2], t_vec_3d[3];
+  double Au, radius, length_m;
+  const double* pos_u_2d; /* [Nu][2] */
+  const double* pos_u_3d; /* [Nu][3] */
+} orc_tube;
+
+typedef struct {
+  const orc_tube* t;
+  int ik_cm_idx, n, ik_cm;
+  double energy;
+} orc_state;
+
+static int cmp_state(const void* a, const void* b) {
+  const orc_state* x = (const orc_state*)a;
+  const orc_state* y = (const orc_state*)b;
+  if (x->energy < y->energy) return -1;
+  if (x->energy > y->energy) return 1;
+  /* ties keep collection order (stable) */
+  return (x->n * 100000 + x->ik_cm_idx) - (y->n * 100000 + y->ik_cm_idx);
+}
+
+static int orc_relevant(const orc_tube* t, double min_energy, double T, double kb, orc_state* out) {
+  const double thr = min_energy + fabs(log(1.e-3) * kb * T);
+  int cnt = 0, i_n, ic;
+  for (i_n = 0; i_n < t->nr; i_n++)
+    for (ic = 0; ic < t->nk_cm; ic++) {
+      double e = t->energy[(size_t)ic * t->nr + i_n];
+      if (e <= thr) {
+        out[cnt].t = t;
+        out[cnt].ik_cm_idx = ic;
+        out[cnt].n = i_n;
+        out[cnt].ik_cm = t->ik_cm0 + ic;
+        out[cnt].energy = e;
+        cnt++;
+      }
+    }
+  qsort(out, cnt, sizeof(orc_state), cmp_state);
+  return cnt;
+}
+
+static double complex orc_q_partial(const orc_state* s) {
+  const orc_tube* t = s->t;
+  double kx = s->ik_cm * t->dk_l[0], ky = s->ik_cm * t->dk_l[1];
+  double complex ef0 = cexp(I * (kx * 0.0 + ky * 0.0));
+  double complex ef1 = cexp(I * (kx * t->aCC[0] + ky * t->aCC[1]));
+  double complex q = 0;
+  int k;
+  for (k = 0; k < t->nk_c; k++) {
+    const int32_t* id = t->idx + ((size_t)s->ik_cm_idx * t->nk_c + k) * 4;
+    const double* wc = t->wf + 2 * ((((size_t)id[1] * t->nk + id[0]) * 2 + 1) * 2);
+    const double* wv = t->wf + 2 * ((((size_t)id[3] * t->nk + id[2]) * 2 + 0) * 2);
+    double complex ccA = wc[0] + I * wc[1], ccB = wc[2] + I * wc[3];
+    double complex cvA = wv[0] + I * wv[1], cvB = wv[2] + I * wv[3];
+    const double* ps = t->psi + 2 * (((size_t)s->ik_cm_idx * t->nr + s->n) * t->nk_c + k);
+    double complex acc = ccA * conj(cvA) * ef0 + ccB * conj(cvB) * ef1;
+    q += (ps[0] + I * ps[1]) * acc;
+  }
+  return q;
+}
+
+static void orc_ru_3d(const orc_tube* t, double shift, double z_shift, double angle, double* out) {
+  int n = t->Nu * t->L, i, a;
+  double ymax, ymin, c = cos(angle), s = sin(angle);
+  for (i = 0; i < t->L; i++)
+    for (a = 0; a < t->Nu; a++) {
+      int k = i * t->Nu + a;
+      out[3 * k] = t->pos_u_3d[3 * a] + i * t->t_vec_3d[0];
+      out[3 * k + 1] = t->pos_u_3d[3 * a + 1] + i * t->t_vec_3d[1];
+      out[3 * k + 2] = t->pos_u_3d[3 * a + 2] + i * t->t_vec_3d[2];
+    }
+  ymax = out[1];
+  ymin = out[1];
+  for (i = 0; i < n; i++) {
+    if (out[3 * i + 1] > ymax) ymax = out[3 * i + 1];
+    if (out[3 * i + 1] < ymin) ymin = out[3 * i + 1];
+  }
+  for (i = 0; i < n; i++) {
+    double x, y;
+    out[3 * i + 1] -= (ymax + ymin) / 2.;
+    out[3 * i + 1] += shift;
+    out[3 * i + 2] += z_shift;
+    x = out[3 * i] * c - out[3 * i + 1] * s;
+    y = out[3 * i] * s + out[3 * i + 1] * c;
+    out[3 * i] = x;
+    out[3 * i + 1] = y;
+  }
+}
+
+static void orc_ru_2d(const orc_tube* t, double* out) {
+  int i, a;
+  for (i = 0; i < t->L; i++)
+    for (a = 0; a < t->Nu; a++) {
+      int k = i * t->Nu + a;
+      out[2 * k] = t->pos_u_2d[2 * a] + i * t->t_vec[0];
+      out[2 * k + 1] = t->pos_u_2d[2 * a + 1] + i * t->t_vec[1];
+    }
+}
+
+double orc_first_order(const orc_tube* d, const orc_tube* a, double T, double broad, double z_shift,
+                       double ash1, double ash2, double theta, int* n_pairs_out) {
+  const double pi = 3.141592, inv_pi = 1. / 3.141592;
+  const double eV = 1.6 * pow(10, -19.0);
+  const double hb = 6.5 * pow(10, -16.0) * eV;
+  const double kb = 1.3865 * pow(10, -23.0);
+  const double eps0 = 8.85 * pow(10, -12);
+  const double q0 = 1.6 * pow(10, -19);
+  double min_energy = d->energy[0], Z = 0, rate = 0, l0;
+  size_t i;
+  int nd, na, p, f, n_pairs = 0;
+  orc_state *ds = (orc_state*)malloc(sizeof(orc_state) * d->nk_cm * d->nr);
+  orc_state *as = (orc_state*)malloc(sizeof(orc_state) * a->nk_cm * a->nr);
+  double *i3 = (double*)malloc(sizeof(double) * 3 * d->Nu * d->L);
+  double *f3 = (double*)malloc(sizeof(double) * 3 * a->Nu * a->L);
+  double *i2 = (double*)malloc(sizeof(double) * 2 * d->Nu * d->L);
+  double *f2 = (double*)malloc(sizeof(double) * 2 * a->Nu * a->L);
+  double complex* fexp = (double complex*)malloc(sizeof(double complex) * a->Nu * a->L);
+  for (i = 0; i < (size_t)d->nk_cm * d->nr; i++) min_energy = d->energy[i] < min_energy ? d->energy[i] : min_energy;
+  nd = orc_relevant(d, min_energy, T, kb, ds);
+  na = orc_relevant(a, min_energy, T, kb, as);
+  for (p = 0; p < nd; p++) Z += exp(-ds[p].energy / (kb * T));
+  orc_ru_3d(d, ash1, 0, 0, i3);
+  orc_ru_3d(a, ash2, z_shift, theta, f3);
+  orc_ru_2d(d, i2);
+  orc_ru_2d(a, f2);
+  l0 = inv_pi * broad / (0 * 0 + broad * broad);
+  for (p = 0; p < nd; p++)
+    for (f = 0; f < na; f++) {
+      double de = ds[p].energy - as[f].energy, coeff, M;
+      double complex Q, J = 0;
+      int ii, jj, ni = d->Nu * d->L, nf = a->Nu * a->L;
+      if (!(inv_pi * broad / (de * de + broad * broad) > 1.e-2 * l0)) continue;
+      n_pairs++;
+      coeff = (pow(q0, 2) * d->Au * a->Au) /
+              (16 * pow(pi, 3) * eps0 * d->radius * a->radius * sqrt(d->length_m * a->length_m));
+      Q = coeff * conj(orc_q_partial(&ds[p])) * orc_q_partial(&as[f]);
+      for (jj = 0; jj < nf; jj++) {
+        double kx = as[f].ik_cm * a->dk_l[0], ky = as[f].ik_cm * a->dk_l[1];
+        fexp[jj] = cexp(+I * (kx * f2[2 * jj] + ky * f2[2 * jj + 1]));
+      }
+      for (ii = 0; ii < ni; ii++) {
+        double kx = ds[p].ik_cm * d->dk_l[0], ky = ds[p].ik_cm * d->dk_l[1];
+        double complex iexp = cexp(-I * (kx * i2[2 * ii] + ky * i2[2 * ii + 1]));
+        for (jj = 0; jj < nf; jj++) {
+          double dv[3];
+          dv[0] = i3[3 * ii] - f3[3 * jj];
+          dv[1] = i3[3 * ii + 1] - f3[3 * jj + 1];
+          dv[2] = i3[3 * ii + 2] - f3[3 * jj + 2];
+          J += iexp * fexp[jj] / arma_norm(dv, 3);
+        }
+      }
+      M = cabs(Q * J) / sqrt(d->length_m * a->length_m);
+      rate += (2 * pi / hb) * (exp(-ds[p].energy / (kb * T)) / Z) * pow(M, 2) *
+              (inv_pi * broad / (de * de + broad * broad));
+    }
+  free(ds); free(as); free(i3); free(f3); free(i2); free(f2); free(fexp);
+  if (n_pairs_out) *n_pairs_out = n_pairs;
+  return rate;
 }

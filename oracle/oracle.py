@@ -263,3 +263,62 @@ class Excitons:
         self.lib.orc_reduce_samples(self.E, self.n_bins, n_batches, _p(self.k_rec, _U32),
                                     _p(self.rec_disp, _D), _p(self.rec_drain, _U32), _p(out, _D))
         return out
+
+
+# ---------------------------------------------------------------------------------------------
+# exact transfer rate (literal first_order)
+# ---------------------------------------------------------------------------------------------
+class _Tube(C.Structure):
+    _fields_ = [(k, C.c_int) for k in ("Nu", "L", "nk", "n_mu", "nr", "nk_cm", "nk_c", "ik_cm0")] + \
+               [("energy", _D), ("psi", _D), ("idx", _I32), ("wf", _D),
+                ("dk_l", C.c_double * 2), ("aCC", C.c_double * 2), ("t_vec", C.c_double * 2),
+                ("t_vec_3d", C.c_double * 3), ("Au", C.c_double), ("radius", C.c_double),
+                ("length_m", C.c_double), ("pos_u_2d", _D), ("pos_u_3d", _D)]
+
+
+def tube_from_model(model) -> dict:
+    """Arrays of a product CntModel (cnt_film_monte_carlo_amd.exciton) for the oracle."""
+    i = model.info
+    psi, idx = model.exciton_states()
+    _, wf = model.electrons()
+    pa, _, p3 = model.atoms()
+    return dict(Nu=i.Nu, L=i.length_cells, nk=i.nk, n_mu=i.n_mu, nr=i.nk_relev, nk_cm=i.nk_cm,
+                nk_c=i.nk_c, ik_cm0=i.ik_cm0, energy=np.ascontiguousarray(model.dispersion()),
+                psi=np.ascontiguousarray(psi), idx=np.ascontiguousarray(idx, np.int32),
+                wf=np.ascontiguousarray(wf), dk_l=tuple(i.dk_l), aCC=tuple(i.aCC),
+                t_vec=tuple(i.t_vec), t_vec_3d=(0.0, i.t_vec[1], 0.0), Au=i.Au, radius=i.radius,
+                length_m=i.length_m, pos_u_2d=np.ascontiguousarray(pa),
+                pos_u_3d=np.ascontiguousarray(p3))
+
+
+def _tube_struct(t: dict) -> _Tube:
+    s = _Tube()
+    for k in ("Nu", "L", "nk", "n_mu", "nr", "nk_cm", "nk_c", "ik_cm0"):
+        setattr(s, k, int(t[k]))
+    s.energy = t["energy"].ctypes.data_as(_D)
+    s.psi = t["psi"].ctypes.data_as(_D)
+    s.idx = t["idx"].ctypes.data_as(_I32)
+    s.wf = t["wf"].ctypes.data_as(_D)
+    for k in ("dk_l", "aCC", "t_vec", "t_vec_3d"):
+        arr = getattr(s, k)
+        for q, v in enumerate(t[k]):
+            arr[q] = v
+    s.Au, s.radius, s.length_m = t["Au"], t["radius"], t["length_m"]
+    s.pos_u_2d = t["pos_u_2d"].ctypes.data_as(_D)
+    s.pos_u_3d = t["pos_u_3d"].ctypes.data_as(_D)
+    return s
+
+
+def first_order(donor: dict, acceptor: dict, temperature: float, broadening: float, z_shift: float,
+                axis_shift_1: float, axis_shift_2: float, theta: float):
+    """Literal first_order on the CPU; returns (rate, number of matched pairs)."""
+    lib = library()
+    fn = lib.orc_first_order
+    fn.restype = C.c_double
+    fn.argtypes = [C.POINTER(_Tube), C.POINTER(_Tube), C.c_double, C.c_double, C.c_double,
+                   C.c_double, C.c_double, C.c_double, C.POINTER(C.c_int)]
+    sd, sa = _tube_struct(donor), _tube_struct(acceptor)
+    npairs = C.c_int(0)
+    r = fn(C.byref(sd), C.byref(sa), temperature, broadening, z_shift, axis_shift_1, axis_shift_2,
+           theta, C.byref(npairs))
+    return r, npairs.value

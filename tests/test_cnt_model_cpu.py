@@ -1,0 +1,133 @@
+"""CPU pins of the CNT exciton model (product host code restating src/exciton_transfer/cnt.cpp)
+and of the host half of the exact transfer rate.  The reference cannot run here (Armadillo
+absent), so the model is pinned by closed forms of the tube geometry and graphene bands, by
+LAPACK (numpy) eigenvalues of its own exciton kernels, by invariants, and by the oracle's literal
+restatement of first_order for the matched-state bookkeeping."""
+import math
+
+import numpy as np
+import pytest
+
+EV = 1.6e-19
+
+
+@pytest.fixture(scope="module")
+def models():
+    from cnt_film_monte_carlo_amd import exciton as X
+    return {c: X.CntModel(*c, 10) for c in [(4, 2), (7, 5), (8, 6), (9, 7)]}
+
+
+@pytest.mark.parametrize("n,m", [(4, 2), (7, 5), (8, 6), (9, 7)])
+def test_tube_geometry_closed_forms(models, n, m):
+    i = models[(n, m)].info
+    dR = math.gcd(2 * n + m, n + 2 * m)
+    assert i.Nu == 2 * (n * n + m * m + n * m) // dR
+    assert (i.t1, i.t2) == ((2 * m + n) // dR, -(2 * n + m) // dR)
+    a_cc = 1.42e-10
+    a_l = float(np.float32(math.sqrt(3.0))) * a_cc           # single-precision sqrt(3) (cnt.h:26)
+    ch_len = a_l * math.sqrt(n * n + m * m + n * m)
+    assert i.radius == pytest.approx(ch_len / 2 / 3.141592, rel=1e-13)
+    t_len = math.sqrt(3) * ch_len / dR                        # |T| = sqrt(3)|Ch|/dR
+    assert i.length_m == pytest.approx(10 * t_len, rel=1e-12)
+    # rotated frame: Ch along x, T along y (cnt.cpp:44-58)
+    assert abs(i.ch_vec[1]) < 1e-12 * ch_len and abs(i.t_vec[0]) < 1e-12 * t_len
+    # K2 . T = 2 pi (with the reference's pi), dk_l = K2 / L
+    k2t = i.K2[0] * i.t_vec[0] + i.K2[1] * i.t_vec[1]
+    assert k2t == pytest.approx(2 * 3.141592, rel=1e-12)
+    assert math.hypot(*i.dk_l) == pytest.approx(math.hypot(*i.K2) / 10, rel=1e-14)
+    assert i.Nu % i.Q == 0 and i.nk == i.Nu // i.Q * 10
+
+
+@pytest.mark.parametrize("n,m", [(4, 2), (7, 5)])
+def test_graphene_bands_closed_form(models, n, m):
+    # E_c = -E_v = t0 sqrt(3 + 2cos(k.a1) + 2cos(k.a2) + 2cos(k.(a1-a2))) (textbook tight binding)
+    model = models[(n, m)]
+    i = model.info
+    e, wf = model.electrons()
+    ch, tv = np.array(i.ch_vec), np.array(i.t_vec)
+    dR = math.gcd(2 * n + m, n + 2 * m)
+    t1, t2 = (2 * m + n) // dR, -(2 * n + m) // dR
+    # solve ch = n a1 + m a2, T = t1 a1 + t2 a2 for the rotated lattice vectors
+    A = np.array([[n, m], [t1, t2]], float)
+    a1, a2 = np.linalg.solve(A, np.stack([ch, tv]))
+    ik = np.arange(i.nk)[:, None]
+    mu = np.arange(i.n_mu)[None, :]
+    kx = mu * i.K1[0] + ik * i.dk_l[0]
+    ky = mu * i.K1[1] + ik * i.dk_l[1]
+    kd = lambda v: kx * v[0] + ky * v[1]
+    f2 = 3 + 2 * np.cos(kd(a1)) + 2 * np.cos(kd(a2)) + 2 * np.cos(kd(a1 - a2))
+    t0 = 2.7 * EV
+    np.testing.assert_allclose(e[1], t0 * np.sqrt(f2), rtol=1e-9)
+    np.testing.assert_allclose(e[0], -e[1], rtol=0, atol=0)
+    # band wavefunctions are normalised two-component spinors, conduction orthogonal to valence
+    nrm = np.abs(wf) ** 2
+    np.testing.assert_allclose(nrm.sum(-1), 1.0, rtol=1e-14)
+    ov = (np.conj(wf[:, :, 0, :]) * wf[:, :, 1, :]).sum(-1)
+    assert np.abs(ov).max() < 1e-14
+
+
+@pytest.mark.parametrize("n,m", [(4, 2), (7, 5), (9, 7)])
+def test_exciton_eigenvalues_match_lapack(models, n, m):
+    model = models[(n, m)]
+    E = model.dispersion()
+    psi, _ = model.exciton_states()
+    nr = model.info.nk_relev
+    for ic in range(0, model.info.nk_cm, max(1, model.info.nk_cm // 9)):
+        H = model.singlet_kernel(ic)
+        np.testing.assert_allclose(H, H.conj().T, rtol=0, atol=1e-30)
+        ref = np.linalg.eigvalsh(H)
+        np.testing.assert_allclose(E[ic], ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+        # psi = [v; v] / sqrt(2) with v the unit eigenvectors (cnt.cpp:973-974)
+        v = psi[ic][:, :nr].T * math.sqrt(2)
+        assert np.allclose(np.linalg.norm(v, axis=0), 1.0, rtol=1e-12)
+        assert np.allclose(psi[ic][:, :nr], psi[ic][:, nr:], rtol=0, atol=0)
+        resid = H @ v - v * E[ic]
+        assert np.abs(resid).max() < 1e-12 * np.abs(ref).max()
+
+
+def test_interaction_invariants(models):
+    vq, PI, eps = models[(4, 2)].interaction()
+    i = models[(4, 2)].info
+    assert np.all(PI >= 0)                                   # 2 sum |<v|c>|^2 / (Ec - Ev), Ec > Ev
+    q0 = i.nk - 1
+    mu0 = i.n_mu - 1
+    assert np.all(vq[q0, mu0].real > 0) and np.all(np.abs(vq[q0, mu0].imag) < 1e-30)  # Ohno at q=0
+    np.testing.assert_allclose(eps, 1.0 + vq.real.mean(-1) * PI, rtol=1e-15)
+
+
+def test_relevant_states_and_matching_vs_oracle(models, orc):
+    from cnt_film_monte_carlo_amd import exciton as X
+    for d, a in [((4, 2), (4, 2)), ((7, 5), (7, 5)), ((8, 6), (7, 5))]:
+        t = X.Transfer(models[d], models[a], 300.0)
+        td, ta = orc.tube_from_model(models[d]), orc.tube_from_model(models[a])
+        _, n_pairs = orc.first_order(td, ta, 300.0, X.BROADENING, 5e-9, 0.0, 0.0, 0.5)
+        assert t.info.n_pairs == n_pairs
+        ka, kb, c = t.terms()
+        assert np.all(c > 0)
+        assert t.info.n_terms == len(set(zip(ka.tolist(), kb.tolist())))
+
+
+def test_oracle_first_order_far_field_is_dipolar(models, orc):
+    # Each transition density carries zero net charge, so far from the tubes the Coulomb
+    # coupling is dipole-dipole: J ~ kappa / z^3 with kappa = cos(theta) when the separation
+    # is perpendicular to both axes.  Hence rate * z^6 -> const and rate(theta) / rate(0)
+    # -> cos^2(theta); a wrong rotation sign, a dropped atom sum or a transposed pair fails.
+    from cnt_film_monte_carlo_amd import exciton as X
+    t = orc.tube_from_model(models[(4, 2)])
+
+    def rate(z, th):
+        return orc.first_order(t, t, 300.0, X.BROADENING, z, 0.0, 0.0, th)[0]
+
+    r2, r4 = rate(2e-6, 0.0), rate(4e-6, 0.0)
+    assert r2 / r4 == pytest.approx(64.0, rel=1e-4)
+    for th in (0.6, 1.2, 2.5):
+        assert rate(4e-6, th) / r4 == pytest.approx(math.cos(th) ** 2, rel=1e-5)
+    assert rate(4e-6, math.pi / 2) / r4 < 1e-9
+
+
+def test_temperature_widens_the_window(models):
+    from cnt_film_monte_carlo_amd import exciton as X
+    m = models[(7, 5)]
+    lo = X.Transfer(m, m, 150.0).info.n_donor_states
+    hi = X.Transfer(m, m, 400.0).info.n_donor_states
+    assert lo <= hi and hi >= 7
