@@ -27,8 +27,8 @@ using namespace cntkmc;
 
 namespace {
 
-constexpr int MAXMOM = 16;    // centre-of-mass momenta per tube handled by the kernel
-constexpr int MAXTERMS = 256; // nonzero C_ab terms (one owning thread each)
+constexpr int CHUNK = 16;     // acceptor momenta per kernel pass (W_b held in registers)
+constexpr int MAXTERMS = 256; // nonzero C_ab terms per pass (one owning thread each)
 constexpr int TILE = 128;     // acceptor atoms per LDS tile
 
 struct ex_state {
@@ -95,14 +95,15 @@ void tube_atoms(const cnt_model& c, std::vector<double>& r3, std::vector<vec2>& 
 // ---------------------------------------------------------------------------------------------
 struct exact_args {
   int32_t na, nb;                 // atoms per tube
-  int32_t nmb;                    // acceptor momenta (padded to the kernel's NMB)
-  int32_t n_terms;                // nonzero (a, b) terms
+  int32_t nmb;                    // acceptor momenta of this pass (padded to the kernel's NMB)
+  int32_t n_terms;                // nonzero (a, b) terms of this pass
+  int32_t accumulate;             // 0: out = pass sum; 1: out += pass sum
   const double4* donor;           // [na] centred positions (w unused)
   const double4* acceptor;        // [nb]
   const double2* u;               // [nma][na]  exp(-i k_a . r)
-  const double2* v;               // [nmb][nb]  exp(+i k_b . r)
+  const double2* v;               // [nmb][nb]  exp(+i k_b . r) of this pass's momenta
   const int32_t* term_a;          // [n_terms] donor momentum slot
-  const int32_t* term_b;          // [n_terms] acceptor momentum slot
+  const int32_t* term_b;          // [n_terms] acceptor momentum slot within the pass
   const double* term_c;           // [n_terms] C_ab
   int64_t n_geom;
   const double *theta, *z, *a1, *a2;
@@ -192,15 +193,24 @@ __global__ __launch_bounds__(256) void k_exact_rate(exact_args a) {
     for (int off = 32; off > 0; off >>= 1) part += __shfl_down(part, off, 64);
     if ((tid & 63) == 0) s_red[tid >> 6] = part;
     __syncthreads();
-    if (tid == 0) a.out[e] = (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]);
+    if (tid == 0) {
+      const double sum = (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]);
+      a.out[e] = a.accumulate ? a.out[e] + sum : sum;
+    }
     __syncthreads();
   }
 }
 
 int padded_nmb(int nmb) { return nmb <= 2 ? 2 : nmb <= 4 ? 4 : nmb <= 8 ? 8 : nmb <= 12 ? 12 : 16; }
 
-hipError_t launch_exact(const exact_args& a, hipStream_t st) {
-  if (a.n_geom == 0) return hipSuccess;
+// Geometries whose transfer has no matched state pair: the rate is exactly zero.
+__global__ void k_zero_rates(int64_t n, const int64_t* sel, double* out) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < n) out[sel ? sel[g] : g] = 0.0;
+}
+
+// One pass over a chunk of at most 16 acceptor momenta.
+hipError_t launch_pass(const exact_args& a, hipStream_t st) {
   const unsigned grid = (unsigned)std::min<int64_t>(a.n_geom, 65536);
   switch (a.nmb) {
     case 2: hipLaunchKernelGGL(k_exact_rate<2>, dim3(grid), dim3(256), 0, st, a); break;
@@ -229,7 +239,8 @@ struct kmc_transfer {
   std::vector<int> mom_a, mom_b;            // ik_cm of each momentum slot
   std::vector<int32_t> term_a, term_b;      // nonzero terms
   std::vector<double> term_c;
-  int nmb_pad = 2;
+  struct pass { int b0, nmb_pad, t0, nt; };  // acceptor slots [b0, b0 + 16), terms [t0, t0 + nt)
+  std::vector<pass> passes;
   double clamp_lo[3] = {-1e300, -1e300, -1e300}, clamp_hi[3] = {1e300, 1e300, 1e300};
   std::vector<double> h_donor, h_acc;
   std::vector<vec2> r2d, r2a;
@@ -277,20 +288,38 @@ exact_args args_of(const kmc_transfer* t) {
   exact_args a{};
   a.na = t->info.n_atoms_donor;
   a.nb = t->info.n_atoms_acceptor;
-  a.nmb = t->nmb_pad;
-  a.n_terms = t->info.n_terms;
   a.donor = t->d_donor;
   a.acceptor = t->d_acc;
   a.u = t->d_u;
-  a.v = t->d_v;
-  a.term_a = t->d_ta;
-  a.term_b = t->d_tb;
-  a.term_c = t->d_tc;
   for (int k = 0; k < 3; k++) {
     a.clamp_lo[k] = t->clamp_lo[k];
     a.clamp_hi[k] = t->clamp_hi[k];
   }
   return a;
+}
+
+// rate = sum over passes of sum_{terms of the pass} C_ab |J_ab|^2; the passes run in order on
+// one stream, the first writes and the rest add.
+hipError_t launch_exact(const kmc_transfer* t, exact_args a, hipStream_t st) {
+  if (a.n_geom == 0) return hipSuccess;
+  if (t->passes.empty()) {
+    const unsigned grid = (unsigned)((a.n_geom + 255) / 256);
+    hipLaunchKernelGGL(k_zero_rates, dim3(grid), dim3(256), 0, st, a.n_geom, a.sel, a.out);
+    return hipGetLastError();
+  }
+  hipError_t e = hipSuccess;
+  for (size_t k = 0; k < t->passes.size() && e == hipSuccess; k++) {
+    const auto& p = t->passes[k];
+    a.nmb = p.nmb_pad;
+    a.n_terms = p.nt;
+    a.accumulate = k > 0;
+    a.v = t->d_v + (size_t)p.b0 * a.nb;
+    a.term_a = t->d_ta + p.t0;
+    a.term_b = t->d_tb + p.t0;
+    a.term_c = t->d_tc + p.t0;
+    e = launch_pass(a, st);
+  }
+  return e;
 }
 
 }  // namespace
@@ -331,7 +360,8 @@ int kmc_transfer_create(int32_t device, const kmc_cnt* donor, const kmc_cnt* acc
     v.push_back(k);
     return (int)v.size() - 1;
   };
-  std::vector<double> C((size_t)MAXMOM * MAXMOM, 0.0);
+  struct term { int p, q; double c; };
+  std::vector<term> acc;  // coefficients per momentum pair, in first-match order
   int n_pairs = 0;
   // match_states (exciton_transfer.h:167-196) and the sum of first_order (:411-427)
   for (size_t i = 0; i < ds.size(); i++)
@@ -343,28 +373,32 @@ int kmc_transfer_create(int32_t device, const kmc_cnt* donor, const kmc_cnt* acc
       const double w = (2 * pi / hb) * (std::exp(-ds[i].energy / (kb * temperature)) / Z) *
                        std::norm(Q) / LiLf * lorentzian(de);
       const int p = slot(t->mom_a, ds[i].ik_cm), q = slot(t->mom_b, as[f].ik_cm);
-      if (p >= MAXMOM || q >= MAXMOM) {
-        delete t;
-        return tfail(KMC_ERR_CAPACITY, "kmc_transfer_create: more than 16 centre-of-mass "
-                                       "momenta per tube among the matched states");
-      }
-      C[(size_t)p * MAXMOM + q] += w;
+      auto it = std::find_if(acc.begin(), acc.end(), [&](const term& x) { return x.p == p && x.q == q; });
+      if (it == acc.end()) acc.push_back({p, q, w});
+      else it->c += w;
     }
   if (t->mom_a.empty()) t->mom_a.push_back(0);
   if (t->mom_b.empty()) t->mom_b.push_back(0);
   const int nma = (int)t->mom_a.size(), nmb = (int)t->mom_b.size();
-  for (int p = 0; p < nma; p++)
-    for (int q = 0; q < nmb; q++)
-      if (C[(size_t)p * MAXMOM + q] != 0.0) {
-        t->term_a.push_back(p);
-        t->term_b.push_back(q);
-        t->term_c.push_back(C[(size_t)p * MAXMOM + q]);
-      }
-  if ((int)t->term_c.size() > MAXTERMS) {
-    delete t;
-    return tfail(KMC_ERR_CAPACITY, "kmc_transfer_create: more than 256 momentum terms");
+  // terms ordered by (donor slot, acceptor slot) and grouped into passes of 16 acceptor slots
+  std::sort(acc.begin(), acc.end(), [](const term& x, const term& y) {
+    return x.q / CHUNK != y.q / CHUNK ? x.q / CHUNK < y.q / CHUNK : x.p != y.p ? x.p < y.p : x.q < y.q;
+  });
+  for (const term& x : acc) {
+    if (x.c == 0.0) continue;
+    const int b0 = x.q / CHUNK * CHUNK;
+    if (t->passes.empty() || t->passes.back().b0 != b0)
+      t->passes.push_back({b0, padded_nmb(std::min(CHUNK, nmb - b0)), (int)t->term_c.size(), 0});
+    if (++t->passes.back().nt > MAXTERMS) {
+      delete t;
+      return tfail(KMC_ERR_CAPACITY, "kmc_transfer_create: more than 256 momentum terms "
+                                     "within 16 acceptor momenta");
+    }
+    t->term_a.push_back(x.p);
+    t->term_b.push_back(x.q - b0);
+    t->term_c.push_back(x.c);
   }
-  t->nmb_pad = padded_nmb(nmb);
+  const int v_rows = t->passes.empty() ? 0 : t->passes.back().b0 + t->passes.back().nmb_pad;
   kmc_transfer_info& in = t->info;
   in.n_donor_states = (int)ds.size();
   in.n_acceptor_states = (int)as.size();
@@ -380,7 +414,7 @@ int kmc_transfer_create(int32_t device, const kmc_cnt* donor, const kmc_cnt* acc
   tube_atoms(D, t->h_donor, t->r2d);
   tube_atoms(A, t->h_acc, t->r2a);
   t->h_u.assign((size_t)nma * t->r2d.size(), cplx(0, 0));
-  t->h_v.assign((size_t)t->nmb_pad * t->r2a.size(), cplx(0, 0));
+  t->h_v.assign((size_t)std::max(v_rows, nmb) * t->r2a.size(), cplx(0, 0));
   for (int p = 0; p < nma; p++) {
     const double kx = t->mom_a[p] * D.dk_l.x, ky = t->mom_a[p] * D.dk_l.y;
     for (size_t i = 0; i < t->r2d.size(); i++)
@@ -457,7 +491,7 @@ int kmc_transfer_rates_device(kmc_transfer* t, int64_t n, const double* d_theta,
   a.sel = d_sel;
   a.out = d_rate;
   hipError_t e = hipSetDevice(t->device);
-  if (e == hipSuccess) e = launch_exact(a, t->stream);
+  if (e == hipSuccess) e = launch_exact(t, a, t->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(t->stream);
   if (e != hipSuccess) return tfail(KMC_ERR_HIP, std::string("kmc_transfer_rates: ") + hipGetErrorString(e));
   return KMC_OK;
@@ -494,7 +528,7 @@ int kmc_transfer_rates(kmc_transfer* t, int64_t n, const double* theta, const do
   a.sel = nullptr;
   a.out = t->d_out;
   if (e == hipSuccess) e = hipEventRecord(t->ev0, t->stream);
-  if (e == hipSuccess) e = launch_exact(a, t->stream);
+  if (e == hipSuccess) e = launch_exact(t, a, t->stream);
   if (e == hipSuccess) e = hipEventRecord(t->ev1, t->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(rate, t->d_out, n * sizeof(double), hipMemcpyDeviceToHost, t->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(t->stream);

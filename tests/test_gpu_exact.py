@@ -21,26 +21,37 @@ def say(*a):
 @pytest.fixture(scope="module")
 def models():
     from cnt_film_monte_carlo_amd import exciton as X
-    return {c: X.CntModel(*c, 10) for c in [(4, 2), (7, 5), (8, 6)]}
+    return {c: X.CntModel(*c, 10) for c in [(4, 2), (7, 5), (8, 6), (9, 7)]}
 
 
-@pytest.mark.parametrize("pair,geoms", [
-    (((4, 2), (4, 2)), [(1.5e-9, 0, 0, 0.0), (3e-9, 2e-9, -1e-9, 1.2), (2e-9, -5e-9, 4e-9, 3.141592),
-                        (7e-9, 0, 0, 1.5707963), (1.6e-9, 1e-9, 1e-9, 2.3561944)]),
-    (((7, 5), (7, 5)), [(1.5e-9, 0, 0, 0.0), (4e-9, -3e-9, 2e-9, 0.7)]),
-    (((8, 6), (7, 5)), [(2.5e-9, 1e-9, 0, 2.0)]),
+@pytest.mark.parametrize("pair,T,geoms", [
+    (((4, 2), (4, 2)), 300.0, [(1.5e-9, 0, 0, 0.0), (3e-9, 2e-9, -1e-9, 1.2), (2e-9, -5e-9, 4e-9, 3.141592),
+                               (7e-9, 0, 0, 1.5707963), (1.6e-9, 1e-9, 1e-9, 2.3561944)]),
+    (((7, 5), (7, 5)), 300.0, [(1.5e-9, 0, 0, 0.0), (4e-9, -3e-9, 2e-9, 0.7)]),
+    (((8, 6), (7, 5)), 300.0, [(2.5e-9, 1e-9, 0, 2.0)]),
+    # 19 acceptor momenta: two kernel passes (16 + 3)
+    (((7, 5), (9, 7)), 400.0, [(2e-9, 1e-9, -2e-9, 0.9)]),
 ])
-def test_transfer_matches_literal_first_order(models, orc, pair, geoms):
+def test_transfer_matches_literal_first_order(models, orc, pair, T, geoms):
     from cnt_film_monte_carlo_amd import exciton as X
     d, a = pair
-    say(f"first_order {d}->{a} at {len(geoms)} geometries")
-    t = X.Transfer(models[d], models[a], 300.0)
+    say(f"first_order {d}->{a} T={T} at {len(geoms)} geometries")
+    t = X.Transfer(models[d], models[a], T)
     g = np.array(geoms)
     got = t.rates(g[:, 3], g[:, 0], g[:, 1], g[:, 2])
     td, ta = orc.tube_from_model(models[d]), orc.tube_from_model(models[a])
     for k, (z, s1, s2, th) in enumerate(geoms):
-        ref, _ = orc.first_order(td, ta, 300.0, X.BROADENING, z, s1, s2, th)
+        ref, _ = orc.first_order(td, ta, T, X.BROADENING, z, s1, s2, th)
         assert got[k] == pytest.approx(ref, rel=1e-9, abs=1e-300), (k, got[k], ref)
+
+
+def test_no_matched_pairs_gives_zero(models):
+    # (9,7) donor to (7,5) acceptor at 150 K: no state pair within the Lorentzian window
+    from cnt_film_monte_carlo_amd import exciton as X
+    t = X.Transfer(models[(9, 7)], models[(7, 5)], 150.0)
+    assert t.info.n_pairs == 0 and t.info.n_terms == 0
+    r = t.rates([0.3, 1.0], [2e-9, 5e-9], [0.0, 1e-9], [0.0, -1e-9])
+    assert np.all(r == 0.0)
 
 
 def test_far_field_is_dipolar(models):
