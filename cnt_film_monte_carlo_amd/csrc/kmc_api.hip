@@ -16,6 +16,7 @@
 #include "../../include/cntkmc.h"
 #include "kmc_internal.h"
 #include "kmc_kernels.h"
+#include "kmc_mlp.h"
 #include "philox.h"
 
 using namespace cntkmc;
@@ -78,6 +79,7 @@ struct kmc_ctx {
   dbuf<uint8_t> d_cls;
   dbuf<uint32_t> d_cls_count;
   dbuf<int64_t> d_cls_start, d_sel;
+  dbuf<uint8_t> d_mlp_flag;  // surrogate flags of the last surrogate build
 
   // excitons
   int64_t n_exc = 0;
@@ -305,7 +307,7 @@ int kmc_destroy(kmc_ctx* c) {
   for (int q = 0; q < 4; ++q) dfree(c->d_geo[q]);
   dfree(c->d_units); dfree(c->d_src_roff); dfree(c->d_roff); dfree(c->d_blocks);
   dfree(c->d_counts64); dfree(c->d_cls); dfree(c->d_cls_count); dfree(c->d_cls_start);
-  dfree(c->d_sel);
+  dfree(c->d_sel); dfree(c->d_mlp_flag);
   dfree(c->d_err); dfree(c->x_site); dfree(c->x_t); dfree(c->x_dx); dfree(c->x_dy);
   dfree(c->x_dz); dfree(c->x_hops); dfree(c->x_hash); dfree(c->x_drains); dfree(c->x_krec);
   dfree(c->x_heading); dfree(c->d_site_count); dfree(c->r_dx); dfree(c->r_dy); dfree(c->r_dz);
@@ -625,7 +627,7 @@ int kmc_pair_classes(kmc_ctx* c, int32_t n_chir, int64_t* class_start, const int
   for (int k = 0; k < nc; ++k) class_start[k + 1] = class_start[k] + cnt[k];
   KMC_TRY(copy_in(c, c->d_cls_start.p, class_start, (nc + 1) * sizeof(int64_t)));
   KMC_HIP(c, hipMemsetAsync(c->d_cls_count.p, 0, nc * sizeof(uint32_t), c->stream));
-  KMC_HIP(c, launch_pair_scatter(c->nnz, c->d_cls.p, c->d_cls_start.p, c->d_cls_count.p,
+  KMC_HIP(c, launch_pair_scatter(c->nnz, c->d_cls.p, nullptr, c->d_cls_start.p, c->d_cls_count.p,
                                  c->d_sel.p, c->stream));
   KMC_HIP(c, hipStreamSynchronize(c->stream));
   if (d_sel) *d_sel = c->d_sel.p;
@@ -653,6 +655,61 @@ int kmc_build_rates_exact(kmc_ctx* c, int32_t n_chir, kmc_transfer* const* trans
     if (s != KMC_OK) return fail(c, s, std::string("kmc_build_rates_exact: ") + kmc_transfer_last_error());
   }
   KMC_TRY(kmc_set_pair_rates_device(c, c->d_pair_rates.p));
+  if (elapsed_ms)
+    *elapsed_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return KMC_OK;
+}
+
+int kmc_build_rates_mlp(kmc_ctx* c, kmc_mlp* mlp, int32_t n_chir, const int32_t* chir_class,
+                        double temperature, kmc_transfer* const* transfers, int64_t* n_fallback,
+                        double* elapsed_ms) {
+  if (!c || !mlp || !chir_class || !transfers || !(temperature > 0))
+    return fail(c, KMC_ERR_ARG, "kmc_build_rates_mlp: bad argument");
+  if (c->csr_state != 1 || !c->d_geo[0].p)
+    return fail(c, KMC_ERR_STATE, "kmc_build_rates_mlp: build with KMC_RATE_PAIR first");
+  if (cntkmc::mlp_device(mlp) != c->cfg.device)
+    return fail(c, KMC_ERR_ARG, "kmc_build_rates_mlp: surrogate lives on another device");
+  const auto t0 = std::chrono::steady_clock::now();
+  const int nc = n_chir * n_chir;
+  std::vector<int64_t> start(nc + 1);
+  KMC_TRY(kmc_pair_classes(c, n_chir, start.data(), nullptr));
+  {
+    const int s = cntkmc::mlp_prepare(mlp);
+    if (s != KMC_OK) return fail(c, s, std::string("kmc_build_rates_mlp: ") + kmc_mlp_last_error());
+  }
+  const int64_t nnz = std::max<int64_t>(c->nnz, 1);
+  KMC_TRY(dalloc(c, c->d_pair_rates, nnz));
+  KMC_TRY(dalloc(c, c->d_mlp_flag, nnz));
+  KMC_HIP(c, hipSetDevice(c->cfg.device));
+  KMC_HIP(c, cntkmc::mlp_launch_pairs(mlp, c->nnz, c->d_geo[0].p, c->d_geo[1].p, c->d_geo[2].p,
+                                      c->d_geo[3].p, c->d_cls.p, n_chir, chir_class, temperature,
+                                      c->d_pair_rates.p, c->d_mlp_flag.p, c->stream));
+  // flagged entries grouped by chirality class, then the exact routine on each group
+  KMC_HIP(c, hipMemsetAsync(c->d_cls_count.p, 0, nc * sizeof(uint32_t), c->stream));
+  KMC_HIP(c, launch_mask_count(c->nnz, c->d_cls.p, c->d_mlp_flag.p, c->d_cls_count.p, c->stream));
+  std::vector<uint32_t> cnt(nc);
+  KMC_TRY(copy_out(c, cnt.data(), c->d_cls_count.p, nc * sizeof(uint32_t)));
+  start[0] = 0;
+  for (int k = 0; k < nc; ++k) start[k + 1] = start[k] + cnt[k];
+  for (int k = 0; k < nc; ++k)
+    if (cnt[k] && !transfers[k])
+      return fail(c, KMC_ERR_ARG, "kmc_build_rates_mlp: flagged pairs of a class without a transfer");
+  if (start[nc] > 0) {
+    KMC_TRY(copy_in(c, c->d_cls_start.p, start.data(), (nc + 1) * sizeof(int64_t)));
+    KMC_HIP(c, hipMemsetAsync(c->d_cls_count.p, 0, nc * sizeof(uint32_t), c->stream));
+    KMC_HIP(c, launch_pair_scatter(c->nnz, c->d_cls.p, c->d_mlp_flag.p, c->d_cls_start.p,
+                                   c->d_cls_count.p, c->d_sel.p, c->stream));
+    KMC_HIP(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < nc; ++k) {
+      if (!cnt[k]) continue;
+      const int s = kmc_transfer_rates_device(transfers[k], cnt[k], c->d_geo[0].p, c->d_geo[1].p,
+                                              c->d_geo[2].p, c->d_geo[3].p, c->d_sel.p + start[k],
+                                              c->d_pair_rates.p);
+      if (s != KMC_OK) return fail(c, s, std::string("kmc_build_rates_mlp: ") + kmc_transfer_last_error());
+    }
+  }
+  KMC_TRY(kmc_set_pair_rates_device(c, c->d_pair_rates.p));
+  if (n_fallback) *n_fallback = start[nc];
   if (elapsed_ms)
     *elapsed_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return KMC_OK;

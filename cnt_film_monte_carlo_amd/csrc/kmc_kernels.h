@@ -101,8 +101,11 @@ hipError_t launch_sum_counts(int64_t n, int64_t stride, const uint32_t* copies, 
 hipError_t launch_pair_classes(int64_t n, int64_t nnz, const site_hdr* hdr, const edge_rec* edge,
                                const uint8_t* chir, int32_t n_chir, uint8_t* cls, uint32_t* counts,
                                hipStream_t st);
-hipError_t launch_pair_scatter(int64_t nnz, const uint8_t* cls, const int64_t* starts, uint32_t* fill,
-                               int64_t* sel, hipStream_t st);
+hipError_t launch_pair_scatter(int64_t nnz, const uint8_t* cls, const uint8_t* mask,
+                               const int64_t* starts, uint32_t* fill, int64_t* sel, hipStream_t st);
+// counts[cls[p]] += 1 for every entry with mask[p] != 0
+hipError_t launch_mask_count(int64_t nnz, const uint8_t* cls, const uint8_t* mask, uint32_t* counts,
+                             hipStream_t st);
 hipError_t launch_hop(const hop_args& a, bool count, bool sample, bool trace, hipStream_t st);
 hipError_t launch_reduce(const reduce_args& a, hipStream_t st);
 

@@ -710,12 +710,14 @@ __global__ __launch_bounds__(256) void k_pair_class(int64_t n, const site_hdr* _
   }
 }
 
+// entries with mask[p] == 0 (mask optional) are left out
 __global__ __launch_bounds__(256) void k_pair_scatter(int64_t nnz, const uint8_t* __restrict__ cls,
+                                                      const uint8_t* __restrict__ mask,
                                                       const int64_t* __restrict__ starts,
                                                       uint32_t* __restrict__ fill,
                                                       int64_t* __restrict__ sel) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= nnz) return;
+  if (p >= nnz || (mask && !mask[p])) return;
   const int c = cls[p];
   const uint32_t slot = atomicAdd(&fill[c], 1u);
   sel[starts[c] + slot] = p;
@@ -731,10 +733,24 @@ hipError_t launch_pair_classes(int64_t n, int64_t nnz, const site_hdr* hdr, cons
   return hipGetLastError();
 }
 
-hipError_t launch_pair_scatter(int64_t nnz, const uint8_t* cls, const int64_t* starts, uint32_t* fill,
-                               int64_t* sel, hipStream_t st) {
+hipError_t launch_pair_scatter(int64_t nnz, const uint8_t* cls, const uint8_t* mask,
+                               const int64_t* starts, uint32_t* fill, int64_t* sel, hipStream_t st) {
   if (nnz == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_pair_scatter, grid_for(nnz), dim3(256), 0, st, nnz, cls, starts, fill, sel);
+  hipLaunchKernelGGL(k_pair_scatter, grid_for(nnz), dim3(256), 0, st, nnz, cls, mask, starts, fill, sel);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_mask_count(int64_t nnz, const uint8_t* __restrict__ cls,
+                                                    const uint8_t* __restrict__ mask,
+                                                    uint32_t* __restrict__ counts) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < nnz && mask[p]) atomicAdd(&counts[cls[p]], 1u);
+}
+
+hipError_t launch_mask_count(int64_t nnz, const uint8_t* cls, const uint8_t* mask, uint32_t* counts,
+                             hipStream_t st) {
+  if (nnz == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_mask_count, grid_for(nnz), dim3(256), 0, st, nnz, cls, mask, counts);
   return hipGetLastError();
 }
 

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -541,4 +542,32 @@ int kmc_transfer_rates(kmc_transfer* t, int64_t n, const double* theta, const do
   return KMC_OK;
 }
 
+int kmc_transfer_table(kmc_transfer* t, int32_t nt, const double* th, int32_t nz, const double* z,
+                       int32_t n1, const double* a1, int32_t n2, const double* a2, double* rates,
+                       double* elapsed_ms) {
+  if (!t || nt < 1 || nz < 1 || n1 < 1 || n2 < 1 || !th || !z || !a1 || !a2 || !rates)
+    return tfail(KMC_ERR_ARG, "kmc_transfer_table: bad argument");
+  const auto t0 = std::chrono::steady_clock::now();
+  // nodes in the order of create_davoody_scatt_table's loops (monte_carlo.cpp:110-122)
+  const int64_t n = (int64_t)nt * nz * n1 * n2;
+  std::vector<double> g(4 * n);
+  int64_t e = 0;
+  for (int i = 0; i < nt; ++i)
+    for (int j = 0; j < nz; ++j)
+      for (int k = 0; k < n1; ++k)
+        for (int q = 0; q < n2; ++q, ++e) {
+          g[e] = th[i];
+          g[n + e] = z[j];
+          g[2 * n + e] = a1[k];
+          g[3 * n + e] = a2[q];
+        }
+  const int s = kmc_transfer_rates(t, n, g.data(), g.data() + n, g.data() + 2 * n, g.data() + 3 * n,
+                                   rates, nullptr);
+  if (s != KMC_OK) return s;
+  if (elapsed_ms)
+    *elapsed_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return KMC_OK;
+}
+
 }  // extern "C"
+

@@ -54,6 +54,8 @@ _SIGS = {
     "kmc_transfer_set_clamp": (C.c_int, [_P, _D, _D]),
     "kmc_transfer_rates": (C.c_int, [_P, C.c_int64, _D, _D, _D, _D, _D, _D]),
     "kmc_transfer_rates_device": (C.c_int, [_P, C.c_int64, _P, _P, _P, _P, _P, _P]),
+    "kmc_transfer_table": (C.c_int, [_P, C.c_int32, _D, C.c_int32, _D, C.c_int32, _D, C.c_int32, _D,
+                                     _D, _D]),
 }
 _bound = False
 
@@ -217,6 +219,19 @@ class Transfer:
         out = np.zeros(n)
         ms = C.c_double(0.0)
         rc = self.lib.kmc_transfer_rates(self.h, n, *[_p(v) for v in g], _p(out), C.byref(ms))
+        if rc < 0:
+            raise kmc.KmcError(rc, self.lib.kmc_transfer_last_error().decode())
+        self.last_ms = ms.value
+        return out
+
+    def table(self, theta, z, a1, a2):
+        """Exact 4-D table [n_theta][n_z][n_a1][n_a2] on the grid (create_davoody_scatt_table).
+        Sets self.last_ms to the wall time of the build."""
+        g = [np.ascontiguousarray(v, np.float64).ravel() for v in (theta, z, a1, a2)]
+        out = np.zeros([v.size for v in g])
+        ms = C.c_double(0.0)
+        rc = self.lib.kmc_transfer_table(self.h, g[0].size, _p(g[0]), g[1].size, _p(g[1]), g[2].size,
+                                         _p(g[2]), g[3].size, _p(g[3]), _p(out), C.byref(ms))
         if rc < 0:
             raise kmc.KmcError(rc, self.lib.kmc_transfer_last_error().decode())
         self.last_ms = ms.value

@@ -101,6 +101,7 @@ _SIGS = {
                                     C.c_int32, _D, _D]),
     "kmc_linspace": (C.c_int, [C.c_double, C.c_double, C.c_int32, _D]),
     "kmc_build_rates_exact": (C.c_int, [_P, C.c_int32, C.POINTER(_P), _D]),
+    "kmc_build_rates_mlp": (C.c_int, [_P, _P, C.c_int32, _I32, C.c_double, C.POINTER(_P), _I64, _D]),
     "kmc_pair_classes": (C.c_int, [_P, C.c_int32, _I64, C.POINTER(_P)]),
     "kmc_get_pair_rates": (C.c_int, [_P, _D]),
 }
@@ -275,6 +276,23 @@ class Context:
         ms = C.c_double(0.0)
         self._ck(self.lib.kmc_build_rates_exact(self.h, n_chir, arr, C.byref(ms)))
         return ms.value
+
+    def build_rates_mlp(self, surrogate, chir_class, temperature: float, transfers):
+        """Per-pair surrogate rates with exact fallback for flagged pairs, then the CSR.
+
+        chir_class[i] = surrogate class of the context's chirality id i (-1: unknown);
+        transfers[ci * n_chir + cj] recomputes flagged pairs of that class (None allowed where
+        no pair is flagged).  Returns (ms, pairs recomputed exactly)."""
+        cc = np.ascontiguousarray(chir_class, np.int32)
+        n_chir = cc.size
+        if len(transfers) != n_chir * n_chir:
+            raise ValueError("transfers must have n_chir^2 entries")
+        arr = (_P * len(transfers))(*[None if t is None else t.h for t in transfers])
+        ms = C.c_double(0.0)
+        nfb = C.c_int64(0)
+        self._ck(self.lib.kmc_build_rates_mlp(self.h, surrogate.h, n_chir, _ptr(cc, _I32),
+                                              float(temperature), arr, C.byref(nfb), C.byref(ms)))
+        return ms.value, nfb.value
 
     def pair_classes(self, n_chir: int):
         start = np.zeros(n_chir * n_chir + 1, np.int64)

@@ -272,6 +272,13 @@ int kmc_transfer_rates(kmc_transfer* t, int64_t n, const double* theta, const do
 int kmc_transfer_rates_device(kmc_transfer* t, int64_t n, const double* d_theta,
                               const double* d_z, const double* d_a1, const double* d_a2,
                               const int64_t* d_sel, double* d_rate);
+/* Exact 4-D rate table on the grid given: create_davoody_scatt_table
+ * (src/monte_carlo/monte_carlo.cpp:63-150) for one donor/acceptor pair.  rates (host) is
+ * [n_theta][n_z][n_a1][n_a2] row-major, as kmc_set_rate_table takes it.  elapsed_ms = wall time
+ * of the build (upload, kernel, download). */
+int kmc_transfer_table(kmc_transfer* t, int32_t n_theta, const double* theta, int32_t n_z,
+                       const double* z_shift, int32_t n_a1, const double* axis_shift_1,
+                       int32_t n_a2, const double* axis_shift_2, double* rates, double* elapsed_ms);
 
 /* Per-pair exact rates for a KMC_RATE_PAIR build: CSR entries are grouped by chirality class
  * (donor * n_chir + acceptor) on the device and class k is evaluated with transfers[k] (NULL:
@@ -287,6 +294,91 @@ int kmc_build_rates_exact(kmc_ctx* ctx, int32_t n_chir, kmc_transfer* const* tra
 int kmc_pair_classes(kmc_ctx* ctx, int32_t n_chir, int64_t* class_start, const int64_t** d_sel);
 /* Per-pair rates of the last pair-rate build (host copy, nnz doubles). */
 int kmc_get_pair_rates(kmc_ctx* ctx, double* rates);
+
+/* ---- Learned rate surrogate (no reference counterpart) --------------------------------------
+ * An MLP of KMC_MLP_HIDDEN hidden layers of width KMC_MLP_WIDTH with SiLU maps the features of a
+ * pair geometry to a normalised log10 transfer rate.  It stands beside
+ * exciton_transfer::first_order (src/exciton_transfer/exciton_transfer.cpp:380-441), which stays
+ * the fallback for pairs outside the trained domain or in regions whose held-out residual
+ * exceeded the tolerance.  Matrix operands are BF16 (weights, features, activations, gradients
+ * of pre-activations); accumulation, biases, the output layer and the master weights are FP32.
+ *
+ * Features (KMC_MLP_IN slots, 12 used), from theta, the clamped (z, a1, a2) and T:
+ *   0 cos(theta)  1 sin(theta)  2..4 z, a1, a2 mapped linearly from the clamp box to [-1, 1]
+ *   5..7 donor chirality one-hot  8..10 acceptor chirality one-hot
+ *   11 ln(T / sqrt(t_lo t_hi)) / ln(sqrt(t_hi / t_lo))  (maps [t_lo, t_hi] to [-1, 1])
+ * Output y: log10(rate / s^-1) = y_shift + y_scale * y.
+ * Parameters (KMC_MLP_PARAMS floats, FP32): W0 [16][128], b0, W1 [128][128], b1, W2, b2, W3, b3,
+ * w4 [128], b4 [1]; W_l is [in][out] row-major. */
+#define KMC_MLP_IN 16
+#define KMC_MLP_WIDTH 128
+#define KMC_MLP_HIDDEN 4
+#define KMC_MLP_CLASSES 3
+#define KMC_MLP_PARAMS 51841
+
+#define KMC_MLP_FLAG_DOMAIN 1   /* chirality or temperature outside the trained domain */
+#define KMC_MLP_FLAG_RESIDUAL 2 /* bin whose held-out residual exceeded the tolerance   */
+
+typedef struct kmc_mlp kmc_mlp;
+typedef struct kmc_mlp_domain {
+  double box_lo[3], box_hi[3]; /* z, a1, a2 clamp box (m) */
+  double t_lo, t_hi;           /* trained temperature range (K) */
+  double y_shift, y_scale;     /* output normalisation */
+} kmc_mlp_domain;
+
+int kmc_mlp_create(int32_t device, const kmc_mlp_domain* domain, kmc_mlp** out);
+int kmc_mlp_destroy(kmc_mlp* mlp);
+const char* kmc_mlp_last_error(void);
+int kmc_mlp_set_params(kmc_mlp* mlp, const float* params);
+int kmc_mlp_get_params(kmc_mlp* mlp, float* params);
+/* Residual map: flags[dims[0]][dims[1]][dims[2]][dims[3]][dims[4]] over (donor class * 3 +
+ * acceptor class, T bin, z bin, theta bin, max(|a1|, |a2|) bin), bins uniform over the domain;
+ * nonzero = send the pair to the exact routine.  dims[0] must be 9.  NULL clears the map. */
+int kmc_mlp_set_residual_map(kmc_mlp* mlp, const int32_t dims[5], const uint8_t* flags);
+/* Feature rows [n][KMC_MLP_IN] (host); class < 0 or >= 3 gives an all-zero one-hot. */
+int kmc_mlp_features(const kmc_mlp* mlp, int64_t n, const double* theta, const double* z_shift,
+                     const double* axis_shift_1, const double* axis_shift_2, const int32_t* cls_d,
+                     const int32_t* cls_a, const double* temperature, float* features);
+/* Network output y for n feature rows (host arrays), through the training kernels. */
+int kmc_mlp_forward(kmc_mlp* mlp, int64_t n, const float* features, float* y);
+/* Mean squared error of y against targets and its gradient w.r.t. every parameter (host). */
+int kmc_mlp_loss_grad(kmc_mlp* mlp, int64_t n, const float* features, const float* targets,
+                      double* loss, float* grad);
+/* One Adam step (step >= 1, bias-corrected) with the gradient given (host). */
+int kmc_mlp_adam_step(kmc_mlp* mlp, const float* grad, double lr, double beta1, double beta2,
+                      double eps, int64_t step);
+/* Train on n rows for `epochs` epochs of shuffled mini-batches (batch a multiple of 128; the
+ * last partial batch of an epoch is dropped), Adam with a cosine learning rate from lr to
+ * lr_min.  epoch_loss[epochs] (may be NULL) gets the mean batch loss of each epoch. */
+int kmc_mlp_train(kmc_mlp* mlp, int64_t n, const float* features, const float* targets,
+                  int32_t epochs, int32_t batch, double lr, double lr_min, uint64_t seed,
+                  double* epoch_loss, double* elapsed_ms);
+/* Fused inference kernel on device arrays: rates of n geometries of one chirality pair (cls_d,
+ * cls_a) at temperature T.  d_flag (may be NULL) gets 0 or a KMC_MLP_FLAG_* value; flagged
+ * entries still receive the network's rate. */
+int kmc_mlp_rates_device(kmc_mlp* mlp, int64_t n, const double* d_theta, const double* d_z,
+                         const double* d_a1, const double* d_a2, int32_t cls_d, int32_t cls_a,
+                         double temperature, double* d_rate, uint8_t* d_flag);
+/* Same on host arrays; kernel_ms (may be NULL) = device time of the inference kernel. */
+int kmc_mlp_rates(kmc_mlp* mlp, int64_t n, const double* theta, const double* z_shift,
+                  const double* axis_shift_1, const double* axis_shift_2, int32_t cls_d,
+                  int32_t cls_a, double temperature, double* rate, uint8_t* flag,
+                  double* kernel_ms);
+/* Surrogate-built 4-D table (layout of kmc_transfer_table).  Flagged nodes are recomputed by
+ * `fallback` (the exact transfer of the same pair and temperature; NULL leaves the network's
+ * value).  n_fallback (may be NULL) = nodes recomputed; elapsed_ms = wall time of the build. */
+int kmc_mlp_table(kmc_mlp* mlp, int32_t cls_d, int32_t cls_a, double temperature,
+                  kmc_transfer* fallback, int32_t n_theta, const double* theta, int32_t n_z,
+                  const double* z_shift, int32_t n_a1, const double* axis_shift_1, int32_t n_a2,
+                  const double* axis_shift_2, double* rates, int64_t* n_fallback,
+                  double* elapsed_ms);
+/* Per-pair surrogate rates for a KMC_RATE_PAIR build, then the CSR.  chir_class[n_chir] maps
+ * the context's chirality ids to surrogate classes (-1: unknown, always exact); flagged pairs
+ * are recomputed by transfers[ci * n_chir + cj] (entries may be NULL only for classes with no
+ * flagged pair).  n_fallback (may be NULL) = pairs recomputed exactly. */
+int kmc_build_rates_mlp(kmc_ctx* ctx, kmc_mlp* mlp, int32_t n_chir, const int32_t* chir_class,
+                        double temperature, kmc_transfer* const* transfers, int64_t* n_fallback,
+                        double* elapsed_ms);
 
 /* Library version string. */
 const char* kmc_version(void);

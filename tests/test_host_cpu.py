@@ -18,8 +18,9 @@ def test_library_exports_every_declared_symbol(kmc):
     lib = kmc.library()
     for name in sorted(declared):
         assert hasattr(lib, name), name
-    from cnt_film_monte_carlo_amd import exciton
-    assert declared == set(kmc.exported_symbols()) | set(exciton.exported_symbols())
+    from cnt_film_monte_carlo_amd import exciton, surrogate
+    bound = set(kmc.exported_symbols()) | set(exciton.exported_symbols()) | set(surrogate.exported_symbols())
+    assert declared == bound
     assert lib.kmc_version().decode().startswith("cntkmc")
 
 
