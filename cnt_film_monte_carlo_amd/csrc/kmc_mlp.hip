@@ -76,8 +76,21 @@ __host__ __device__ inline double box_coord(double v, const domain_consts& d, in
   const double c = fmin(fmax(v, d.lo[k]), d.hi[k]);
   return (2.0 * c - (d.lo[k] + d.hi[k])) / (d.hi[k] - d.lo[k]);
 }
+// Features 2..4: the box coordinates of z and of (a1, a2) taken with the sign that makes a2 > 0
+// (a1 > 0 when a2 == 0); the a1/a2 box is symmetric, so clamping commutes with the flip.
+__host__ __device__ inline void shift_features(double z, double a1, double a2, const domain_consts& d,
+                                               float* f) {
+  const double c1 = fmin(fmax(a1, d.lo[1]), d.hi[1]), c2 = fmin(fmax(a2, d.lo[2]), d.hi[2]);
+  const bool flip = c2 < 0.0 || (c2 == 0.0 && c1 < 0.0);
+  f[0] = (float)box_coord(z, d, 0);
+  f[1] = (float)box_coord(flip ? -c1 : c1, d, 1);
+  f[2] = (float)box_coord(flip ? -c2 : c2, d, 2);
+}
 __host__ __device__ inline double temp_feature(double T, const domain_consts& d) {
   return log(T / sqrt(d.t_lo * d.t_hi)) / log(sqrt(d.t_hi / d.t_lo));
+}
+__host__ __device__ inline double inv_temp_feature(double T, const domain_consts& d) {
+  return (2.0 / T - (1.0 / d.t_lo + 1.0 / d.t_hi)) / (1.0 / d.t_lo - 1.0 / d.t_hi);
 }
 __host__ __device__ inline int bin_of(double u, int n) {  // u in [0, 1]
   const int b = (int)(u * n);
@@ -305,7 +318,7 @@ struct infer_args {
   int32_t cls_d, cls_a;     // used without pair_cls
   int8_t cmap[16];          // context chirality id -> surrogate class
   double T;
-  float tfeat;
+  float tfeat, tinv;
   bool t_ok;
   const bf16* pk;
   const float* prm;
@@ -386,15 +399,14 @@ __global__ __launch_bounds__(64 * INFER_WAVES) void k_mlp_infer(infer_args a) {
         sincos(a.theta[e], &st, &ct);
         f[0] = (float)ct;
         f[1] = (float)st;
-        f[2] = (float)box_coord(a.z[e], a.dom, 0);
-        f[3] = (float)box_coord(a.a1[e], a.dom, 1);
-        f[4] = (float)box_coord(a.a2[e], a.dom, 2);
+        shift_features(a.z[e], a.a1[e], a.a2[e], a.dom, f + 2);
 #pragma unroll
         for (int j = 0; j < NCLS; ++j) f[5 + j] = cd == j ? 1.0f : 0.0f;
       } else {
 #pragma unroll
         for (int j = 0; j < NCLS; ++j) f[j] = ca == j ? 1.0f : 0.0f;
         f[3] = a.tfeat;
+        f[4] = a.tinv;
       }
     }
     bf16x8 x[8];
@@ -637,6 +649,7 @@ infer_args base_infer(const kmc_mlp* m, double T) {
   a.T = T;
   a.t_ok = T >= m->dom.t_lo && T <= m->dom.t_hi;
   a.tfeat = (float)temp_feature(T, m->dom);
+  a.tinv = (float)inv_temp_feature(T, m->dom);
   a.pk = m->d_pk;
   a.prm = m->d_prm;
   a.dom = m->dom;
@@ -730,6 +743,8 @@ int kmc_mlp_create(int32_t device, const kmc_mlp_domain* dom, kmc_mlp** out) {
   *out = nullptr;
   for (int k = 0; k < 3; ++k)
     if (!(dom->box_lo[k] < dom->box_hi[k])) return mfail(KMC_ERR_ARG, "kmc_mlp_create: empty box");
+  if (dom->box_lo[1] != -dom->box_hi[1] || dom->box_lo[2] != -dom->box_hi[2])
+    return mfail(KMC_ERR_ARG, "kmc_mlp_create: the axis-shift box must be symmetric about 0");
   if (!(dom->t_lo > 0 && dom->t_lo < dom->t_hi) || !(dom->y_scale != 0))
     return mfail(KMC_ERR_ARG, "kmc_mlp_create: bad temperature range or output scale");
   int ndev = 0;
@@ -848,14 +863,13 @@ int kmc_mlp_features(const kmc_mlp* m, int64_t n, const double* th, const double
     std::fill(r, r + IN, 0.0f);
     r[0] = (float)std::cos(th[i]);
     r[1] = (float)std::sin(th[i]);
-    r[2] = (float)box_coord(z[i], m->dom, 0);
-    r[3] = (float)box_coord(a1[i], m->dom, 1);
-    r[4] = (float)box_coord(a2[i], m->dom, 2);
+    shift_features(z[i], a1[i], a2[i], m->dom, r + 2);
     for (int j = 0; j < NCLS; ++j) {
       r[5 + j] = cd[i] == j ? 1.0f : 0.0f;
       r[8 + j] = ca[i] == j ? 1.0f : 0.0f;
     }
     r[11] = (float)temp_feature(T[i], m->dom);
+    r[12] = (float)inv_temp_feature(T[i], m->dom);
   }
   return KMC_OK;
 }

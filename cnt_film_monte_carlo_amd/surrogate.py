@@ -13,8 +13,9 @@ Pipeline (train_surrogate):
      LOG10_FLOOR (rates below 1 /s are ~12 orders below any site's total rate);
   3. training on the GPU by the library's BF16 MFMA forward/backward kernels with Adam;
   4. a held-out set gives the error against the FP64 rates and the residual map: bins of
-     (pair class, T, z, theta, max|a|) whose held-out relative error exceeded the tolerance (or
-     that held no held-out pair) send their pairs to the exact routine.
+     (pair class, T, z, theta, max|a|) in which more than 10% of the held-out pairs exceeded
+     the tolerance (held-out p90 residual > tol), or that held fewer than 4 held-out pairs,
+     send their pairs to the exact routine.
 """
 from __future__ import annotations
 
@@ -31,6 +32,7 @@ from . import exciton, kmc
 
 CHIRALITIES = [(7, 5), (8, 6), (9, 7)]   # surrogate classes 0, 1, 2
 IN, WIDTH, HIDDEN, N_PARAMS = 16, 128, 4, 51841
+N_FEATURES = 13   # feature slots in use (the rest are zero)
 FLAG_DOMAIN, FLAG_RESIDUAL = 1, 2
 LOG10_FLOOR = 0.0
 DEFAULT_MODEL = Path(__file__).resolve().parent / "data" / "rate_surrogate.npz"
@@ -334,7 +336,7 @@ def make_dataset(models, n, seed, domain: Domain, temps_per_class=96, log=print)
         cols["cd"].append(np.full(per, cd, np.int32))
         cols["ca"].append(np.full(per, ca, np.int32))
         cols["T"].append(np.full(per, T))
-        if log and (gi + 1) % 96 == 0:
+        if log and (gi + 1) % 384 == 0:
             log(f"labels: {gi + 1}/{len(groups)} groups, {time.time() - t0:.1f} s")
     out = {k: np.concatenate(v)[:n] for k, v in cols.items()}
     out["seconds"] = time.time() - t0
@@ -349,11 +351,11 @@ def init_params(seed):
     off = 0
     for l in range(HIDDEN):
         fan = IN if l == 0 else WIDTH
-        used = 12 if l == 0 else WIDTH
+        used = N_FEATURES if l == 0 else WIDTH
         lim = math.sqrt(6.0 / used)
         w = rng.uniform(-lim, lim, (fan, WIDTH)).astype(np.float32)
         if l == 0:
-            w[12:] = 0.0
+            w[N_FEATURES:] = 0.0
         p[off:off + fan * WIDTH] = w.ravel()
         off += fan * WIDTH + WIDTH
     p[off:off + WIDTH] = rng.uniform(-1, 1, WIDTH) * math.sqrt(3.0 / WIDTH)
@@ -386,13 +388,14 @@ def bin_indices(dims, cd, ca, T, theta, z, a1, a2, domain: Domain):
     return (((pc * dims[1] + iT) * dims[2] + iz) * dims[3] + ith) * dims[4] + ia
 
 
-def residual_map(dims, bins, rel_err, tol=0.05):
-    """Flag bins whose held-out error exceeded tol, and bins that held no held-out pair."""
+def residual_map(dims, bins, rel_err, tol=0.05, quantile=0.9, min_count=4):
+    """Flag the bins whose held-out relative error at `quantile` exceeds tol (more than
+    1 - quantile of their held-out pairs beyond tol), and bins with fewer than min_count
+    held-out pairs, whose error is unknown."""
     size = int(np.prod(dims))
-    worst = np.zeros(size)
-    np.maximum.at(worst, bins, rel_err)
-    seen = np.bincount(bins, minlength=size) > 0
-    return ((worst > tol) | ~seen).astype(np.uint8)
+    cnt = np.bincount(bins, minlength=size)
+    over = np.bincount(bins, weights=(np.asarray(rel_err) > tol).astype(np.float64), minlength=size)
+    return ((over > (1.0 - quantile) * cnt) | (cnt < min_count)).astype(np.uint8)
 
 
 @dataclass
@@ -409,17 +412,37 @@ class TrainReport:
     val_p99_rel: float
     val_frac_over_tol: float
     flagged_bins: float
+    val_frac_flagged: float
+    val_frac_over_tol_unflagged: float
     per_class: dict = field(default_factory=dict)
 
 
-def train_surrogate(n_train=2_000_000, n_val=200_000, epochs=300, batch=8192, lr=2e-3,
-                    lr_min=1e-5, seed=1001, tol=0.05, rmap_dims=(9, 6, 8, 8, 4), device=0,
+def train_chunked(s, feat, target, epochs, batch, lr, lr_min, seed, chunk=100, log=print):
+    """Train in chunks of `chunk` epochs: each chunk runs a half cosine between the values the
+    global cosine schedule (lr -> lr_min over all epochs) takes at its ends, so the schedule
+    follows the global one piecewise while progress is reported between chunks.  Adam's
+    moments and step count carry over from chunk to chunk.  Returns (losses, ms)."""
+    def sched(e):
+        return lr_min + 0.5 * (lr - lr_min) * (1.0 + math.cos(math.pi * e / epochs))
+    losses, ms = [], 0.0
+    for k, e0 in enumerate(range(0, epochs, chunk)):
+        e1 = min(epochs, e0 + chunk)
+        l, m = s.train(feat, target, e1 - e0, batch, sched(e0), sched(e1), seed + k)
+        losses.extend(l.tolist())
+        ms += m
+        if log:
+            log(f"epochs {e1}/{epochs}: loss {l[-1]:.3e} ({ms / 1e3:.1f} s)")
+    return np.array(losses), ms
+
+
+def train_surrogate(n_train=2_000_000, n_val=1_000_000, epochs=1200, batch=8192, lr=3e-3,
+                    lr_min=1e-6, seed=1001, tol=0.05, rmap_dims=(9, 10, 8, 8, 4), device=0,
                     models=None, log=print):
     """Label, train, validate, build the residual map.  Returns (RateSurrogate, TrainReport)."""
     dom = Domain()
     models = models or {c: exciton.CntModel(*c, 10) for c in CHIRALITIES}
-    tr = make_dataset(models, n_train, seed, dom, log=log)
-    va = make_dataset(models, n_val, seed + 7919, dom, temps_per_class=24, log=log)
+    tr = make_dataset(models, n_train, seed, dom, temps_per_class=384, log=log)
+    va = make_dataset(models, n_val, seed + 7919, dom, temps_per_class=96, log=log)
     lab = tr["log10_rate"]
     dom.y_shift = float(np.mean(lab))
     dom.y_scale = float(np.std(lab))
@@ -427,7 +450,7 @@ def train_surrogate(n_train=2_000_000, n_val=200_000, epochs=300, batch=8192, lr
     s.set_params(init_params(seed))
     f_tr = s.features(tr["theta"], tr["z"], tr["a1"], tr["a2"], tr["cd"], tr["ca"], tr["T"])
     y_tr = ((lab - dom.y_shift) / dom.y_scale).astype(np.float32)
-    losses, ms = s.train(f_tr, y_tr, epochs, batch, lr, lr_min, seed)
+    losses, ms = train_chunked(s, f_tr, y_tr, epochs, batch, lr, lr_min, seed, log=log)
     log(f"trained {epochs} epochs in {ms / 1e3:.1f} s, loss {losses[0]:.3e} -> {losses[-1]:.3e}")
     f_va = s.features(va["theta"], va["z"], va["a1"], va["a2"], va["cd"], va["ca"], va["T"])
     pred = s.log10_of(s.forward(f_va))
@@ -442,10 +465,11 @@ def train_surrogate(n_train=2_000_000, n_val=200_000, epochs=300, batch=8192, lr
             m = (va["cd"] == cd) & (va["ca"] == ca)
             per[f"{CHIRALITIES[cd]}->{CHIRALITIES[ca]}"] = {
                 "median": float(np.median(err[m])), "p99": float(np.quantile(err[m], 0.99))}
+    vf = flags[bins] != 0
     rep = TrainReport(n_train, n_val, tr["seconds"], tr["exact_kernel_seconds"], ms / 1e3, epochs,
                       float(losses[-1]), float(np.median(err)), float(np.quantile(err, 0.95)),
                       float(np.quantile(err, 0.99)), float(np.mean(err > tol)),
-                      float(flags.mean()), per)
+                      float(flags.mean()), float(np.mean(vf)), float(np.mean((err > tol) & ~vf)), per)
     s.meta = {"train": rep.__dict__, "seed": seed, "tol": tol,
               "chiralities": CHIRALITIES, "loss_curve": [float(v) for v in losses]}
     return s, rep

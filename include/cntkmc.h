@@ -303,10 +303,15 @@ int kmc_get_pair_rates(kmc_ctx* ctx, double* rates);
  * exceeded the tolerance.  Matrix operands are BF16 (weights, features, activations, gradients
  * of pre-activations); accumulation, biases, the output layer and the master weights are FP32.
  *
- * Features (KMC_MLP_IN slots, 12 used), from theta, the clamped (z, a1, a2) and T:
- *   0 cos(theta)  1 sin(theta)  2..4 z, a1, a2 mapped linearly from the clamp box to [-1, 1]
+ * Features (KMC_MLP_IN slots, 13 used), from theta, the clamped (z, a1, a2) and T:
+ *   0 cos(theta)  1 sin(theta)  2..4 z, a1, a2 mapped linearly from the clamp box to [-1, 1],
+ *   with (a1, a2) replaced by (-a1, -a2) when a2 < 0 (or a2 == 0 and a1 < 0): reversing both
+ *   tubes (the C2 axes of a nanotube) leaves the exact rate unchanged to about 1e-4, so both
+ *   hop directions along a tube get one prediction.  The a1/a2 box must be symmetric.
  *   5..7 donor chirality one-hot  8..10 acceptor chirality one-hot
  *   11 ln(T / sqrt(t_lo t_hi)) / ln(sqrt(t_hi / t_lo))  (maps [t_lo, t_hi] to [-1, 1])
+ *   12 1/T mapped linearly from [1/t_hi, 1/t_lo] to [-1, 1]  (Boltzmann factors are linear
+ *      in 1/T in the log rate)
  * Output y: log10(rate / s^-1) = y_shift + y_scale * y.
  * Parameters (KMC_MLP_PARAMS floats, FP32): W0 [16][128], b0, W1 [128][128], b1, W2, b2, W3, b3,
  * w4 [128], b4 [1]; W_l is [in][out] row-major. */

@@ -75,11 +75,11 @@ def init_params(seed=0):
     rng = np.random.default_rng(seed)
     d = {}
     for l in range(NH):
-        fan_in = 12 if l == 0 else WD
+        fan_in = 13 if l == 0 else WD
         lim = math.sqrt(6.0 / fan_in)
         w = rng.uniform(-lim, lim, (IN if l == 0 else WD, WD))
         if l == 0:
-            w[12:] = 0.0
+            w[13:] = 0.0
         d[f"W{l}"] = w
         d[f"b{l}"] = np.zeros(WD)
     d["w4"] = rng.uniform(-1, 1, WD) * math.sqrt(3.0 / WD)
@@ -136,11 +136,16 @@ def features(theta, z, a1, a2, cd, ca, T, box_lo, box_hi, t_lo, t_hi):
     f = np.zeros((n, IN), np.float64)
     f[:, 0] = np.cos(theta)
     f[:, 1] = np.sin(theta)
-    for k, v in enumerate((z, a1, a2)):
-        c = np.minimum(np.maximum(v, box_lo[k]), box_hi[k])
+    cl = [np.minimum(np.maximum(v, box_lo[k]), box_hi[k]) for k, v in enumerate((z, a1, a2))]
+    # reversing both tubes maps (a1, a2) to (-a1, -a2): the sign with a2 > 0 (a1 > 0 on a2 == 0)
+    flip = (cl[2] < 0) | ((cl[2] == 0) & (cl[1] < 0))
+    cl[1] = np.where(flip, -cl[1], cl[1])
+    cl[2] = np.where(flip, -cl[2], cl[2])
+    for k, c in enumerate(cl):
         f[:, 2 + k] = (2 * c - (box_lo[k] + box_hi[k])) / (box_hi[k] - box_lo[k])
     for j in range(CLASSES):
         f[:, 5 + j] = (cd == j)
         f[:, 8 + j] = (ca == j)
     f[:, 11] = np.log(T / math.sqrt(t_lo * t_hi)) / math.log(math.sqrt(t_hi / t_lo))
+    f[:, 12] = (2.0 / T - (1.0 / t_lo + 1.0 / t_hi)) / (1.0 / t_lo - 1.0 / t_hi)
     return f.astype(np.float32)

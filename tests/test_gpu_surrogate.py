@@ -62,7 +62,8 @@ def test_features_match_oracle(S, MO, net):
     d = s.domain
     g = MO.features(th, z, a1, a2, cd, ca, T, d.box_lo, d.box_hi, d.t_lo, d.t_hi)
     np.testing.assert_allclose(f, g, rtol=0, atol=2e-7)
-    assert np.all(f[:, 12:] == 0) and np.all(np.abs(f[:, 2:5]) <= 1.0)
+    assert np.all(f[:, 13:] == 0) and np.all(np.abs(f[:, 2:5]) <= 1.0)
+    assert np.all(np.abs(f[:, 11:13]) <= 1.0 + 1e-6)
 
 
 def test_forward_matches_oracle(S, MO, net):
@@ -109,9 +110,9 @@ def test_loss_grad_matches_oracle(S, MO, net):
         for name, off, shape in MO.LAYOUT:
             k = int(np.prod(shape))
             a, b = g[off:off + k].astype(np.float64), go[off:off + k]
-            if name == "W0":  # feature slots 12..15 are always zero: their rows get no gradient
-                a, b = a.reshape(shape)[:12], b.reshape(shape)[:12]
-                assert np.all(g[off:off + k].reshape(shape)[12:] == 0)
+            if name == "W0":  # feature slots 13..15 are always zero: their rows get no gradient
+                a, b = a.reshape(shape)[:13], b.reshape(shape)[:13]
+                assert np.all(g[off:off + k].reshape(shape)[13:] == 0)
             rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
             say(f"n={n} grad {name}: rel L2 {rel:.2e}")
             assert rel < 2e-3, name
@@ -173,3 +174,16 @@ def test_flags_domain_and_residual(S, net):
         assert np.all(fl == S.FLAG_DOMAIN)
     finally:
         s.set_residual_map(None, None)
+
+
+def test_reversing_both_tubes_gives_the_same_rate(S, net):
+    # (a1, a2) -> (-a1, -a2) is reversing both tubes; the features are canonicalised, so the
+    # two geometries must get bitwise the same rate (both hop directions along a tube included)
+    s, _ = net
+    rng = np.random.default_rng(9)
+    th, z, a1, a2 = geometries(rng, 4000, S)
+    a1[:100] = 0.0
+    a2[:100] = rng.choice([-10e-9, 10e-9], 100)
+    r0, _, _ = s.rates(th, z, a1, a2, 0, 0, 300.0)
+    r1, _, _ = s.rates(th, z, -a1, -a2, 0, 0, 300.0)
+    np.testing.assert_array_equal(r0, r1)
