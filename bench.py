@@ -3,10 +3,16 @@
 
 Workload (film_625k_sites): 5,000 straight CNTs of length U[0.5, 2] um in a 5x5x0.2 um box,
 10 nm segment sites (624,739 sites), chiralities (7,5)/(8,6)/(9,7), cutoff 20 nm, y periodic,
-injection at x <= 0 and drain at x >= 5 um; 1M excitons.  A step is one pass of the hop
-kernel over every exciton for STEP_HOPS = 10^4 hops (10^10 hops), with MSD sampling and
-per-site hop counts on; the default 10 timed steps are the full 10^5 hops per exciton.
-The rate-table (CSR) build time is reported next to the hop throughput.
+injection at x <= 0 and drain at x >= 5 um; 1M excitons at 300 K.
+
+Rates: the nine donor/acceptor 4-D tables on the input.json grid (21 x 11 x 11 x 11 nodes),
+built twice -- by the exact first-order rate on the GPU (create_davoody_scatt_table) and by
+the BF16 surrogate with exact fallback for flagged nodes; both build times are reported.  The
+hop run uses the surrogate-built tables.  A step is one pass of the hop kernel over every
+exciton for STEP_HOPS = 10^4 hops (10^10 hops), with MSD sampling and per-site hop counts on;
+the default 10 timed steps are the full 10^5 hops per exciton.  A second run on the
+exact-built tables with the same seeds gives the diffusion coefficient the surrogate run is
+compared against.
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
@@ -26,6 +32,9 @@ sys.path.insert(0, str(ROOT))
 METRIC = "exciton hops/sec (1 GPU) at D within 2%; rate-table build s (MLP vs exact)"
 STEP_HOPS = 10_000
 HBM_PEAK = 8.0e12
+BF16_PEAK = 2.5e15          # dense MFMA, MI355X
+MLP_FLOP_PER_PAIR = 2 * (13 * 128 + 3 * 128 * 128 + 128)   # 13 used inputs, 4 x 128, 1 output
+TEMPERATURE = 300.0
 
 
 def log(*a):
@@ -77,6 +86,104 @@ def cpu_baseline(net, w, tables, csr, seconds=12.0):
     }
 
 
+def rate_tables(w):
+    """Nine exact and nine surrogate-built tables at 300 K on the input.json grid, timed."""
+    import numpy as np
+
+    from cnt_film_monte_carlo_amd import exciton as X
+    from cnt_film_monte_carlo_amd import kmc
+    from cnt_film_monte_carlo_amd import surrogate as S
+
+    grid = kmc.default_grids()
+    sur = S.RateSurrogate.load()
+    cls = [S.CHIRALITIES.index(c) for c in w.chiralities]
+    models = {c: X.CntModel(*c, 10) for c in w.chiralities}
+    trs = []
+    for ci in w.chiralities:
+        for cj in w.chiralities:
+            t = X.Transfer(models[ci], models[cj], TEMPERATURE)
+            t.set_clamp(sur.domain.box_lo, sur.domain.box_hi)
+            t.rates([0.1], [2e-9], [0.0], [0.0])     # device upload outside the timings
+            trs.append(t)
+    sur.rates([0.1], [2e-9], [0.0], [0.0], 0, 0, TEMPERATURE)
+    exact, mlp = [], []
+    exact_s = mlp_s = 0.0
+    fallback = 0
+    k = 0
+    for i in range(len(cls)):
+        for j in range(len(cls)):
+            ex = trs[k].table(*grid)
+            exact_s += trs[k].last_ms / 1e3
+            tab, nfb, ms = sur.table(cls[i], cls[j], TEMPERATURE, trs[k], *grid)
+            mlp_s += ms / 1e3
+            fallback += nfb
+            exact.append(ex)
+            mlp.append(tab)
+            k += 1
+    exact = np.stack(exact)
+    mlp = np.stack(mlp)
+    err = S.relative_error(np.log10(np.maximum(mlp, 1e-300)), np.log10(np.maximum(exact, 1e-300)))
+    log(f"tables: exact {exact_s:.3f} s, surrogate {mlp_s * 1e3:.1f} ms ({exact_s / mlp_s:.0f}x), "
+        f"{fallback}/{exact.size} nodes exact, error median {np.median(err):.4f} p99 "
+        f"{np.quantile(err, 0.99):.4f}")
+    return dict(grid=grid, exact=exact, mlp=mlp, exact_s=exact_s, mlp_s=mlp_s, fallback=fallback,
+                nodes=int(exact.size), err_median=float(np.median(err)),
+                err_p99=float(np.quantile(err, 0.99)), err_over=float(np.mean(err > 0.05)),
+                surrogate=sur, transfers=trs, models=models)
+
+
+def surrogate_roofline(sur, n=1 << 22, reps=5):
+    """Fused BF16 inference kernel on n pair geometries: algorithmic FLOP / kernel time."""
+    import numpy as np
+
+    from cnt_film_monte_carlo_amd import surrogate as S
+
+    rng = np.random.default_rng(3)
+    g = S.sample_geometries(rng, n, sur.domain)
+    ms = []
+    for _ in range(reps):
+        _, _, k = sur.rates(*g, 0, 0, TEMPERATURE)
+        ms.append(k)
+    avg = float(np.mean(ms[1:]))
+    achieved = n * MLP_FLOP_PER_PAIR / (avg * 1e-3)
+    log(f"surrogate inference: {n} pairs in {avg:.3f} ms = {achieved / 1e12:.1f} TFLOP/s")
+    return {"kernel": "k_mlp_infer", "bound": "mfma", "achieved": achieved / 1e12,
+            "peak": BF16_PEAK / 1e12, "unit": "TFLOP/s", "frac": achieved / BF16_PEAK,
+            "traffic": None, "flop_per_pair": MLP_FLOP_PER_PAIR, "pairs_per_launch": n,
+            "avg_launch_ms": avg, "pairs_per_s": n / (avg * 1e-3)}
+
+
+def cpu_table_baseline(tabs, per_class=1):
+    """The oracle's literal first_order (exciton_transfer.cpp:380-441, one core per node) on a
+    sample of table nodes, scaled to the nine tables."""
+    import numpy as np
+
+    from oracle import oracle as orc
+
+    models = tabs["models"]
+    chir = list(models)
+    th, z, a1, a2 = tabs["grid"]
+    rng = np.random.default_rng(4)
+    secs = []
+    for ci in chir:
+        td = orc.tube_from_model(models[ci])
+        for cj in chir:
+            ta = orc.tube_from_model(models[cj])
+            for _ in range(per_class):
+                t0 = time.perf_counter()
+                orc.first_order(td, ta, TEMPERATURE, 4.0e-3 * 1.6e-19, float(rng.choice(z)),
+                                float(rng.choice(a1)), float(rng.choice(a2)), float(rng.choice(th)))
+                secs.append((ci, cj, time.perf_counter() - t0))
+    per_pair = {}
+    for ci, cj, s in secs:
+        per_pair.setdefault((ci, cj), []).append(s)
+    nodes = tabs["nodes"] // (len(chir) ** 2)
+    total = sum(np.mean(v) * nodes for v in per_pair.values())
+    log(f"cpu table build (oracle first_order, 1 core): {total:.0f} s estimated")
+    return {"value": total, "unit": "s (nine tables)", "cores": 1, "kind": "port",
+            "sample": f"{len(secs)} nodes timed, mean per chirality pair x {nodes} nodes"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,10 +219,11 @@ def main():
     t0 = time.perf_counter()
     net = nw.build_network(w)
     log(f"network: {net['n']} sites in {time.perf_counter() - t0:.2f} s")
-    tables = nw.forster_tables(3)
-    th, z, a1, a2, rates = tables
+    tabs = rate_tables(w)
+    th, z, a1, a2 = tabs["grid"]
+    tables = (th, z, a1, a2, tabs["mlp"])
     ctx = nw.make_context(w, net, seed=2024)
-    ctx.set_rate_table(th, z, a1, a2, rates)
+    ctx.set_rate_table(th, z, a1, a2, tabs["mlp"])
     ctx.sync()
     t0 = time.perf_counter()
     ctx.build_cells()
@@ -167,6 +275,29 @@ def main():
     except kmc.KmcError:
         D, ci = None, None
     log(f"hops {hops_total:.3e} in {wall:.2f} s; D = {D}")
+    ctx.close()
+
+    # the same run on the exact-built tables: the diffusion coefficient to compare against
+    ctx_x = nw.make_context(w, net, seed=2024)
+    ctx_x.set_rate_table(th, z, a1, a2, tabs["exact"])
+    ctx_x.build_cells()
+    ctx_x.build_rates(kmc.RATE_TABLE)
+    ctx_x.inject(E)
+    ctx_x.set_samples(args.bins, t_sample_max)
+    for i in range(args.steps):
+        ctx_x.run(STEP_HOPS, count_sites=False, timed=False)
+    red_x, _ = ctx_x.reduce_samples(32)
+    ctx_x.close()
+    try:
+        Dx, cix = kmc.estimate_diffusion(red_x, t_sample_max, args.bins // 4, args.bins)
+        Dx, cix = Dx.tolist(), cix.tolist()
+    except kmc.KmcError:
+        Dx, cix = None, None
+    d_rel = None
+    if D is not None and Dx is not None:
+        d_rel = [(D[k] - Dx[k]) / Dx[k] if Dx[k] else None for k in range(3)]
+    log(f"exact-table run: D = {Dx}; surrogate vs exact {d_rel}")
+    sur_roof = surrogate_roofline(tabs["surrogate"])
 
     traffic = None
     tf = ROOT / "profiles" / "hop_kernel_traffic.json"
@@ -179,6 +310,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(net, w, tables, csr)
+        cpu["table_build"] = cpu_table_baseline(tabs)
 
     value = hops_total / wall
     out = {
@@ -203,12 +335,29 @@ def main():
             "hops_per_step": STEP_HOPS,
             "cutoff_nm": w.cutoff * 1e9,
             "chiralities": [list(c) for c in w.chiralities],
-            "rates": "wong closed-form table on the input.json grid (rate type 'wong')",
+            "rates": ("first-order transfer rates at 300 K, 9 donor/acceptor tables on the "
+                      "input.json grid, built by the BF16 surrogate with exact fallback"),
             "msd_bins": args.bins,
         },
-        "rate_table_build_s": build_s,
+        "rate_table_build_s": {
+            "mlp": tabs["mlp_s"],
+            "exact": tabs["exact_s"],
+            "speedup": tabs["exact_s"] / tabs["mlp_s"],
+            "nodes": tabs["nodes"],
+            "nodes_exact_fallback": tabs["fallback"],
+            "rel_err_median": tabs["err_median"],
+            "rel_err_p99": tabs["err_p99"],
+            "frac_over_5pct": tabs["err_over"],
+            "note": "exact = first-order rate on the GPU for every node; mlp = fused BF16 "
+                    "inference plus exact recomputation of flagged nodes; wall time incl. "
+                    "host<->device copies",
+        },
+        "csr_build_s": build_s,
         "diffusion_m2_per_s": D,
         "diffusion_ci95": ci,
+        "diffusion_exact_tables_m2_per_s": Dx,
+        "diffusion_exact_tables_ci95": cix,
+        "diffusion_rel_diff_vs_exact": d_rel,
         "reduce_kernel_ms": red_ms,
         "roofline": {
             "kernel": "k_hop<count,sample>",
@@ -222,9 +371,9 @@ def main():
             "visit_weighted_degree": d_v,
             "avg_launch_ms": avg_ms,
         },
+        "roofline_surrogate": sur_roof,
         "cpu_baseline": cpu,
     }
-    ctx.close()
     print(json.dumps(out), flush=True)
 
 

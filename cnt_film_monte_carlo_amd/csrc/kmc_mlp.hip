@@ -105,7 +105,8 @@ __host__ __device__ inline int64_t bin_index(const int32_t* dims, int cd, int ca
   const int iT = bin_of((T - d.t_lo) / (d.t_hi - d.t_lo), dims[1]);
   const int iz = bin_of((zc - d.lo[0]) / (d.hi[0] - d.lo[0]), dims[2]);
   const int ith = bin_of(th / M_PI, dims[3]);
-  const int ia = bin_of(fmax(c1, c2) / amax, dims[4]);
+  const double ua = fmax(c1, c2) / amax;
+  const int ia = bin_of(ua * ua, dims[4]);  // equal-area rings of the (a1, a2) square
   return ((((int64_t)(cd * NCLS + ca) * dims[1] + iT) * dims[2] + iz) * dims[3] + ith) * dims[4] + ia;
 }
 

@@ -383,7 +383,7 @@ def bin_indices(dims, cd, ca, T, theta, z, a1, a2, domain: Domain):
     iT = b((np.asarray(T) - domain.t_lo) / (domain.t_hi - domain.t_lo), dims[1])
     iz = b((zc - lo[0]) / (hi[0] - lo[0]), dims[2])
     ith = b(np.asarray(theta) / math.pi, dims[3])
-    ia = b(np.maximum(c1, c2) / amax, dims[4])
+    ia = b((np.maximum(c1, c2) / amax) ** 2, dims[4])
     pc = np.asarray(cd) * 3 + np.asarray(ca)
     return (((pc * dims[1] + iT) * dims[2] + iz) * dims[3] + ith) * dims[4] + ia
 
@@ -396,6 +396,32 @@ def residual_map(dims, bins, rel_err, tol=0.05, quantile=0.9, min_count=4):
     cnt = np.bincount(bins, minlength=size)
     over = np.bincount(bins, weights=(np.asarray(rel_err) > tol).astype(np.float64), minlength=size)
     return ((over > (1.0 - quantile) * cnt) | (cnt < min_count)).astype(np.uint8)
+
+
+def validate(s, models, n_val, seed, tol=0.05, rmap_dims=(9, 10, 8, 8, 4), temps_per_class=128,
+             log=print):
+    """Fresh exactly-labelled held-out pairs: the error of the surrogate against the FP64
+    rates and the residual map built from it (installed on s).  Returns a stats dict."""
+    dom = s.domain
+    va = make_dataset(models, n_val, seed, dom, temps_per_class=temps_per_class, log=log)
+    f_va = s.features(va["theta"], va["z"], va["a1"], va["a2"], va["cd"], va["ca"], va["T"])
+    err = relative_error(s.log10_of(s.forward(f_va)), va["log10_rate"])
+    bins = bin_indices(rmap_dims, va["cd"], va["ca"], va["T"], va["theta"], va["z"], va["a1"],
+                       va["a2"], dom)
+    flags = residual_map(rmap_dims, bins, err, tol)
+    s.set_residual_map(rmap_dims, flags)
+    vf = flags[bins] != 0
+    per = {}
+    for cd in range(3):
+        for ca in range(3):
+            m = (va["cd"] == cd) & (va["ca"] == ca)
+            per[f"{CHIRALITIES[cd]}->{CHIRALITIES[ca]}"] = {
+                "median": float(np.median(err[m])), "p99": float(np.quantile(err[m], 0.99))}
+    return {"n_val": int(n_val), "val_median_rel": float(np.median(err)),
+            "val_p95_rel": float(np.quantile(err, 0.95)), "val_p99_rel": float(np.quantile(err, 0.99)),
+            "val_frac_over_tol": float(np.mean(err > tol)), "flagged_bins": float(flags.mean()),
+            "val_frac_flagged": float(np.mean(vf)),
+            "val_frac_over_tol_unflagged": float(np.mean((err > tol) & ~vf)), "per_class": per}
 
 
 @dataclass
@@ -435,14 +461,13 @@ def train_chunked(s, feat, target, epochs, batch, lr, lr_min, seed, chunk=100, l
     return np.array(losses), ms
 
 
-def train_surrogate(n_train=2_000_000, n_val=1_000_000, epochs=1200, batch=8192, lr=3e-3,
+def train_surrogate(n_train=2_000_000, n_val=2_000_000, epochs=1200, batch=8192, lr=3e-3,
                     lr_min=1e-6, seed=1001, tol=0.05, rmap_dims=(9, 10, 8, 8, 4), device=0,
                     models=None, log=print):
     """Label, train, validate, build the residual map.  Returns (RateSurrogate, TrainReport)."""
     dom = Domain()
     models = models or {c: exciton.CntModel(*c, 10) for c in CHIRALITIES}
     tr = make_dataset(models, n_train, seed, dom, temps_per_class=384, log=log)
-    va = make_dataset(models, n_val, seed + 7919, dom, temps_per_class=96, log=log)
     lab = tr["log10_rate"]
     dom.y_shift = float(np.mean(lab))
     dom.y_scale = float(np.std(lab))
@@ -452,24 +477,11 @@ def train_surrogate(n_train=2_000_000, n_val=1_000_000, epochs=1200, batch=8192,
     y_tr = ((lab - dom.y_shift) / dom.y_scale).astype(np.float32)
     losses, ms = train_chunked(s, f_tr, y_tr, epochs, batch, lr, lr_min, seed, log=log)
     log(f"trained {epochs} epochs in {ms / 1e3:.1f} s, loss {losses[0]:.3e} -> {losses[-1]:.3e}")
-    f_va = s.features(va["theta"], va["z"], va["a1"], va["a2"], va["cd"], va["ca"], va["T"])
-    pred = s.log10_of(s.forward(f_va))
-    err = relative_error(pred, va["log10_rate"])
-    bins = bin_indices(rmap_dims, va["cd"], va["ca"], va["T"], va["theta"], va["z"], va["a1"],
-                       va["a2"], dom)
-    flags = residual_map(rmap_dims, bins, err, tol)
-    s.set_residual_map(rmap_dims, flags)
-    per = {}
-    for cd in range(3):
-        for ca in range(3):
-            m = (va["cd"] == cd) & (va["ca"] == ca)
-            per[f"{CHIRALITIES[cd]}->{CHIRALITIES[ca]}"] = {
-                "median": float(np.median(err[m])), "p99": float(np.quantile(err[m], 0.99))}
-    vf = flags[bins] != 0
+    st = validate(s, models, n_val, seed + 7919, tol, rmap_dims, log=log)
     rep = TrainReport(n_train, n_val, tr["seconds"], tr["exact_kernel_seconds"], ms / 1e3, epochs,
-                      float(losses[-1]), float(np.median(err)), float(np.quantile(err, 0.95)),
-                      float(np.quantile(err, 0.99)), float(np.mean(err > tol)),
-                      float(flags.mean()), float(np.mean(vf)), float(np.mean((err > tol) & ~vf)), per)
+                      float(losses[-1]), st["val_median_rel"], st["val_p95_rel"], st["val_p99_rel"],
+                      st["val_frac_over_tol"], st["flagged_bins"], st["val_frac_flagged"],
+                      st["val_frac_over_tol_unflagged"], st["per_class"])
     s.meta = {"train": rep.__dict__, "seed": seed, "tol": tol,
               "chiralities": CHIRALITIES, "loss_curve": [float(v) for v in losses]}
     return s, rep

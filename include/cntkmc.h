@@ -337,7 +337,8 @@ const char* kmc_mlp_last_error(void);
 int kmc_mlp_set_params(kmc_mlp* mlp, const float* params);
 int kmc_mlp_get_params(kmc_mlp* mlp, float* params);
 /* Residual map: flags[dims[0]][dims[1]][dims[2]][dims[3]][dims[4]] over (donor class * 3 +
- * acceptor class, T bin, z bin, theta bin, max(|a1|, |a2|) bin), bins uniform over the domain;
+ * acceptor class, T bin, z bin, theta bin, a bin), bins uniform in T, clamped z and theta over
+ * the domain and in (max(|a1|, |a2|) / max |a|)^2 (rings of equal area in the (a1, a2) square);
  * nonzero = send the pair to the exact routine.  dims[0] must be 9.  NULL clears the map. */
 int kmc_mlp_set_residual_map(kmc_mlp* mlp, const int32_t dims[5], const uint8_t* flags);
 /* Feature rows [n][KMC_MLP_IN] (host); class < 0 or >= 3 gives an all-zero one-hot. */
