@@ -86,7 +86,7 @@ def cpu_baseline(net, w, tables, csr, seconds=12.0):
     }
 
 
-def rate_tables(w):
+def rate_tables(w, exact_tables=True):
     """Nine exact and nine surrogate-built tables at 300 K on the input.json grid, timed."""
     import numpy as np
 
@@ -112,9 +112,12 @@ def rate_tables(w):
     k = 0
     for i in range(len(cls)):
         for j in range(len(cls)):
-            ex = trs[k].table(*grid)
-            exact_s += trs[k].last_ms / 1e3
             tab, nfb, ms = sur.table(cls[i], cls[j], TEMPERATURE, trs[k], *grid)
+            if exact_tables:
+                ex = trs[k].table(*grid)
+                exact_s += trs[k].last_ms / 1e3
+            else:
+                ex = tab
             mlp_s += ms / 1e3
             fallback += nfb
             exact.append(ex)
@@ -122,6 +125,11 @@ def rate_tables(w):
             k += 1
     exact = np.stack(exact)
     mlp = np.stack(mlp)
+    if not exact_tables:
+        log(f"tables: surrogate {mlp_s * 1e3:.1f} ms, {fallback}/{mlp.size} nodes exact")
+        return dict(grid=grid, exact=None, mlp=mlp, exact_s=None, mlp_s=mlp_s, fallback=fallback,
+                    nodes=int(mlp.size), err_median=None, err_p99=None, err_over=None,
+                    surrogate=sur, transfers=trs, models=models)
     err = S.relative_error(np.log10(np.maximum(mlp, 1e-300)), np.log10(np.maximum(exact, 1e-300)))
     log(f"tables: exact {exact_s:.3f} s, surrogate {mlp_s * 1e3:.1f} ms ({exact_s / mlp_s:.0f}x), "
         f"{fallback}/{exact.size} nodes exact, error median {np.median(err):.4f} p99 "
@@ -130,6 +138,35 @@ def rate_tables(w):
                 nodes=int(exact.size), err_median=float(np.median(err)),
                 err_p99=float(np.quantile(err, 0.99)), err_over=float(np.mean(err > 0.05)),
                 surrogate=sur, transfers=trs, models=models)
+
+
+def exact_table_run(w, net, tabs, E, args, t_sample_max, D):
+    """The timed run's hop sequence on the exact-built tables (same seeds): its diffusion
+    coefficient and the surrogate run's relative difference from it."""
+    from cnt_film_monte_carlo_amd import kmc
+    from cnt_film_monte_carlo_amd import networks as nw
+
+    th, z, a1, a2 = tabs["grid"]
+    ctx_x = nw.make_context(w, net, seed=2024)
+    ctx_x.set_rate_table(th, z, a1, a2, tabs["exact"])
+    ctx_x.build_cells()
+    ctx_x.build_rates(kmc.RATE_TABLE)
+    ctx_x.inject(E)
+    ctx_x.set_samples(args.bins, t_sample_max)
+    for _ in range(args.steps):
+        ctx_x.run(STEP_HOPS, count_sites=False, timed=False)
+    red_x, _ = ctx_x.reduce_samples(32)
+    ctx_x.close()
+    try:
+        Dx, cix = kmc.estimate_diffusion(red_x, t_sample_max, args.bins // 4, args.bins)
+        Dx, cix = Dx.tolist(), cix.tolist()
+    except kmc.KmcError:
+        return None, None, None
+    d_rel = None
+    if D is not None:
+        d_rel = [(D[k] - Dx[k]) / Dx[k] if Dx[k] else None for k in range(3)]
+    log(f"exact-table run: D = {Dx}; surrogate vs exact {d_rel}")
+    return Dx, cix, d_rel
 
 
 def surrogate_roofline(sur, n=1 << 22, reps=5):
@@ -191,6 +228,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--excitons", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-exact", action="store_true",
+                    help="skip the exact table build and the exact-table comparison run")
     ap.add_argument("--bins", type=int, default=64)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes (counter calibration)")
@@ -219,7 +258,7 @@ def main():
     t0 = time.perf_counter()
     net = nw.build_network(w)
     log(f"network: {net['n']} sites in {time.perf_counter() - t0:.2f} s")
-    tabs = rate_tables(w)
+    tabs = rate_tables(w, exact_tables=not args.no_exact)
     th, z, a1, a2 = tabs["grid"]
     tables = (th, z, a1, a2, tabs["mlp"])
     ctx = nw.make_context(w, net, seed=2024)
@@ -278,26 +317,11 @@ def main():
     ctx.close()
 
     # the same run on the exact-built tables: the diffusion coefficient to compare against
-    ctx_x = nw.make_context(w, net, seed=2024)
-    ctx_x.set_rate_table(th, z, a1, a2, tabs["exact"])
-    ctx_x.build_cells()
-    ctx_x.build_rates(kmc.RATE_TABLE)
-    ctx_x.inject(E)
-    ctx_x.set_samples(args.bins, t_sample_max)
-    for i in range(args.steps):
-        ctx_x.run(STEP_HOPS, count_sites=False, timed=False)
-    red_x, _ = ctx_x.reduce_samples(32)
-    ctx_x.close()
-    try:
-        Dx, cix = kmc.estimate_diffusion(red_x, t_sample_max, args.bins // 4, args.bins)
-        Dx, cix = Dx.tolist(), cix.tolist()
-    except kmc.KmcError:
-        Dx, cix = None, None
-    d_rel = None
-    if D is not None and Dx is not None:
-        d_rel = [(D[k] - Dx[k]) / Dx[k] if Dx[k] else None for k in range(3)]
-    log(f"exact-table run: D = {Dx}; surrogate vs exact {d_rel}")
+    Dx = cix = d_rel = None
+    if tabs["exact"] is not None:
+        Dx, cix, d_rel = exact_table_run(w, net, tabs, E, args, t_sample_max, D)
     sur_roof = surrogate_roofline(tabs["surrogate"])
+
 
     traffic = None
     tf = ROOT / "profiles" / "hop_kernel_traffic.json"
@@ -342,7 +366,7 @@ def main():
         "rate_table_build_s": {
             "mlp": tabs["mlp_s"],
             "exact": tabs["exact_s"],
-            "speedup": tabs["exact_s"] / tabs["mlp_s"],
+            "speedup": tabs["exact_s"] / tabs["mlp_s"] if tabs["exact_s"] else None,
             "nodes": tabs["nodes"],
             "nodes_exact_fallback": tabs["fallback"],
             "rel_err_median": tabs["err_median"],

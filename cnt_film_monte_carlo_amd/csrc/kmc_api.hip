@@ -17,6 +17,8 @@
 #include "kmc_internal.h"
 #include "kmc_kernels.h"
 #include "kmc_mlp.h"
+
+#include <hipcub/device/device_radix_sort.hpp>
 #include "philox.h"
 
 using namespace cntkmc;
@@ -75,6 +77,11 @@ struct kmc_ctx {
   dbuf<uint32_t> d_units, d_src_roff;
   dbuf<int64_t> d_roff;
   dbuf<uint4> d_blocks;
+  dbuf<int64_t> d_first;         // block offsets in placement order (scan output)
+  dbuf<uint32_t> d_keys, d_ids;  // exciton sort: 2 x n_exc each (input and output halves)
+  dbuf<uint8_t> d_sort_tmp;
+  size_t sort_tmp_bytes = 0;
+  int64_t resort_hops = 625;     // hops per sub-launch between exciton sorts (0: one launch)
   // per-chirality-class pair lists for externally evaluated rates
   dbuf<uint8_t> d_cls;
   dbuf<uint32_t> d_cls_count;
@@ -213,23 +220,33 @@ int copy_out(kmc_ctx* c, void* dst, const void* src, size_t bytes) {
 }
 
 // Row blocks of the hop kernel from the finished CSR (hdr, cum, edge), plus the block offsets of
-// the source sites.
+// the source sites.  Blocks are laid out in cell-list order when the cell list is current, so
+// sites that are close in space are close in memory.
 int pack_rows(kmc_ctx* c) {
   const int64_t n = c->n_sites;
   KMC_TRY(dalloc(c, c->d_units, n));
   KMC_TRY(dalloc(c, c->d_roff, n + 1));
-  KMC_HIP(c, launch_block_sizes(n, c->d_hdr.p, c->d_units.p, c->stream));
-  KMC_HIP(c, exclusive_scan(c->d_units.p, c->d_roff.p, n, c->d_scan_tmp.p, c->stream));
-  int64_t units = 0;
-  KMC_TRY(copy_out(c, &units, c->d_roff.p + n, sizeof(int64_t)));
-  if (units >= (int64_t)EDGE_DRAIN_BIT)
+  KMC_TRY(dalloc(c, c->d_first, n + 1));
+  KMC_TRY(dalloc(c, c->d_scan_tmp, scan_tmp_elems(n)));
+  const int32_t* order = c->cells_ok ? c->d_cell_sites.p : nullptr;
+  if (order) {
+    KMC_HIP(c, launch_block_sizes(n, c->d_hdr.p, order, c->d_units.p, c->stream));
+    KMC_HIP(c, exclusive_scan(c->d_units.p, c->d_first.p, n, c->d_scan_tmp.p, c->stream));
+    KMC_HIP(c, launch_place_blocks(n, order, c->d_first.p, c->d_roff.p, c->stream));
+  } else {
+    KMC_HIP(c, launch_block_sizes(n, c->d_hdr.p, nullptr, c->d_units.p, c->stream));
+    KMC_HIP(c, exclusive_scan(c->d_units.p, c->d_roff.p, n, c->d_scan_tmp.p, c->stream));
+  }
+  int64_t lines = 0;
+  KMC_TRY(copy_out(c, &lines, c->d_roff.p + n, sizeof(int64_t)));
+  if (lines >= (int64_t)EDGE_DRAIN_BIT)
     return fail(c, KMC_ERR_CAPACITY, "pack_rows: CSR too large for 31-bit block offsets");
-  c->n_units = units;
-  // 8 spare units: the hop kernel reads eight cumulative rates past short rows
-  KMC_TRY(dalloc(c, c->d_blocks, units + 8));
-  KMC_HIP(c, hipMemsetAsync(c->d_blocks.p, 0, (units + 8) * sizeof(uint4), c->stream));
+  c->n_units = lines * ROW_UNITS;
+  KMC_TRY(dalloc(c, c->d_blocks, c->n_units));
+  KMC_HIP(c, hipMemsetAsync(c->d_blocks.p, 0, c->n_units * sizeof(uint4), c->stream));
   KMC_HIP(c, launch_fill_blocks(n, c->d_hdr.p, c->d_cum.p, c->d_edge.p, c->d_roff.p,
-                                c->d_blocks.p, c->stream));
+                                c->d_pos[0].p, c->d_pos[1].p, c->d_pos[2].p, c->d_blocks.p,
+                                c->stream));
   KMC_TRY(dalloc(c, c->d_src_roff, c->h_src.size()));
   KMC_HIP(c, launch_gather_offsets((int64_t)c->h_src.size(), c->d_src.p, c->d_roff.p,
                                    c->d_src_roff.p, c->stream));
@@ -306,6 +323,7 @@ int kmc_destroy(kmc_ctx* c) {
   dfree(c->d_edge);
   for (int q = 0; q < 4; ++q) dfree(c->d_geo[q]);
   dfree(c->d_units); dfree(c->d_src_roff); dfree(c->d_roff); dfree(c->d_blocks);
+  dfree(c->d_first); dfree(c->d_keys); dfree(c->d_ids); dfree(c->d_sort_tmp);
   dfree(c->d_counts64); dfree(c->d_cls); dfree(c->d_cls_count); dfree(c->d_cls_start);
   dfree(c->d_sel); dfree(c->d_mlp_flag);
   dfree(c->d_err); dfree(c->x_site); dfree(c->x_t); dfree(c->x_dx); dfree(c->x_dy);
@@ -883,6 +901,12 @@ int kmc_set_trace(kmc_ctx* c, int64_t n_trace, int64_t max_hops) {
   return KMC_OK;
 }
 
+int kmc_set_hop_schedule(kmc_ctx* c, int64_t resort_hops) {
+  if (!c || resort_hops < 0) return fail(c, KMC_ERR_ARG, "kmc_set_hop_schedule: bad argument");
+  c->resort_hops = resort_hops;
+  return KMC_OK;
+}
+
 int kmc_run(kmc_ctx* c, int64_t hops_per_exciton, double t_max, int32_t count_sites,
             double* kernel_ms) {
   if (!c) return fail(nullptr, KMC_ERR_ARG, "kmc_run: null context");
@@ -897,7 +921,6 @@ int kmc_run(kmc_ctx* c, int64_t hops_per_exciton, double t_max, int32_t count_si
   a.src_roff = c->d_src_roff.p;
   a.n_src = (uint32_t)c->h_src.size();
   a.seed = c->cfg.seed;
-  a.hop_budget = (uint64_t)hops_per_exciton;
   a.t_max = t_max;
   a.n_bins = c->n_bins;
   a.dt_sample = c->n_bins > 0 ? c->t_sample_max / c->n_bins : 0.0;
@@ -912,9 +935,42 @@ int kmc_run(kmc_ctx* c, int64_t hops_per_exciton, double t_max, int32_t count_si
   a.tr_site = c->tr_site.p;
   a.tr_time = c->tr_time.p;
   a.tr_n = c->tr_n.p;
+  for (int k = 0; k < 3; ++k)
+    a.per_len[k] = (c->cfg.periodic_mask & (1 << k)) ? c->cfg.period_len[k] : 0.0;
+  // The budget runs as sub-launches of at most resort_hops hops; before each, the excitons are
+  // sorted by the block of their current site (radix sort of 32-bit keys) so that neighbouring
+  // threads and each XCD's L2 share rows.  Draws are keyed by exciton and hop, so the split and
+  // the order leave every trajectory unchanged.
+  const int64_t E = c->n_exc;
+  KMC_TRY(dalloc(c, c->d_keys, 2 * E));
+  KMC_TRY(dalloc(c, c->d_ids, 2 * E));
+  {
+    size_t need = 0;
+    KMC_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, need, c->d_keys.p, c->d_keys.p + E,
+                                                  c->d_ids.p, c->d_ids.p + E, (int)E, 0, 32,
+                                                  c->stream));
+    if (need > c->sort_tmp_bytes) {
+      KMC_TRY(dalloc(c, c->d_sort_tmp, need));
+      c->sort_tmp_bytes = need;
+    }
+  }
+  a.perm = c->d_ids.p + E;
+  const int64_t chunk = c->resort_hops > 0 ? c->resort_hops : std::max<int64_t>(hops_per_exciton, 1);
   KMC_HIP(c, hipEventRecord(c->ev0, c->stream));
-  KMC_HIP(c, launch_hop(a, count_sites != 0, c->n_bins > 0, c->n_trace > 0 && c->tr_max > 0,
-                        c->stream));
+  int64_t done = 0;
+  do {
+    const int64_t sub = std::min(chunk, hops_per_exciton - done);
+    KMC_HIP(c, launch_exciton_keys(E, c->x_site.p, c->d_roff.p, c->d_keys.p, c->d_ids.p,
+                                   c->stream));
+    size_t have = c->sort_tmp_bytes;
+    KMC_HIP(c, hipcub::DeviceRadixSort::SortPairs(c->d_sort_tmp.p, have, c->d_keys.p,
+                                                  c->d_keys.p + E, c->d_ids.p, c->d_ids.p + E,
+                                                  (int)E, 0, 32, c->stream));
+    a.hop_budget = (uint64_t)sub;
+    KMC_HIP(c, launch_hop(a, count_sites != 0, c->n_bins > 0, c->n_trace > 0 && c->tr_max > 0,
+                          c->stream));
+    done += sub;
+  } while (done < hops_per_exciton);
   KMC_HIP(c, hipEventRecord(c->ev1, c->stream));
   if (kernel_ms) {
     KMC_HIP(c, hipEventSynchronize(c->ev1));

@@ -39,9 +39,9 @@ struct exciton_arrays {
 
 struct hop_args {
   exciton_arrays x;
-  const uint4* blocks;      // row blocks (16-byte units)
-  const int64_t* roff;      // block offset of every site
-  const uint32_t* src_roff; // block offsets of the source sites
+  const uint4* blocks;      // row blocks: 128-byte lines of ROW_UNITS 16-byte units
+  const int64_t* roff;      // first line of every site's block
+  const uint32_t* src_roff; // first lines of the source sites' blocks
   uint32_t n_src;
   uint64_t seed;
   uint64_t hop_budget;
@@ -59,6 +59,8 @@ struct hop_args {
   int32_t* tr_site;
   double* tr_time;
   int64_t* tr_n;
+  double per_len[3];        // period per axis (0 = open) for the minimum-image displacement
+  const uint32_t* perm;     // exciton processed at each position (NULL = identity)
 };
 
 struct reduce_args {
@@ -90,10 +92,15 @@ hipError_t launch_finish_pair_rates(int64_t n, const double* rates, const csr_ar
                                     int32_t* err, hipStream_t st);
 hipError_t launch_inject(const exciton_arrays& x, const uint4* blocks, const uint32_t* src_roff,
                          uint32_t n_src, uint64_t seed, hipStream_t st);
-hipError_t launch_block_sizes(int64_t n, const site_hdr* hdr, uint32_t* units, hipStream_t st);
-hipError_t launch_fill_blocks(int64_t n, const site_hdr* hdr, const double* cum,
-                              const edge_rec* edge, const int64_t* roff, uint4* blocks,
+hipError_t launch_block_sizes(int64_t n, const site_hdr* hdr, const int32_t* order, uint32_t* lines,
                               hipStream_t st);
+hipError_t launch_place_blocks(int64_t n, const int32_t* order, const int64_t* first, int64_t* roff,
+                               hipStream_t st);
+hipError_t launch_exciton_keys(int64_t n, const int32_t* site, const int64_t* roff, uint32_t* keys,
+                               uint32_t* ids, hipStream_t st);
+hipError_t launch_fill_blocks(int64_t n, const site_hdr* hdr, const double* cum,
+                              const edge_rec* edge, const int64_t* roff, const double* px,
+                              const double* py, const double* pz, uint4* blocks, hipStream_t st);
 hipError_t launch_gather_offsets(int64_t n, const int32_t* idx, const int64_t* roff, uint32_t* out,
                                  hipStream_t st);
 hipError_t launch_sum_counts(int64_t n, int64_t stride, const uint32_t* copies, uint64_t* out,

@@ -300,49 +300,91 @@ __device__ __forceinline__ double dwell_time(uint32_t n, double gamma, uint32_t 
 }
 
 // ---------------------------------------------------------------------------------------------
-// Row blocks: the hop kernel's copy of the CSR.  Each site owns one contiguous block of 16-byte
-// units: [n | site | Gamma] [cum[0..n) (+8 B pad when n is odd)] [payload[0..n)], where a
-// payload entry is {target block offset | drain bit, float dx, dy, dz}.  One hop touches one
-// block: the header and the first eight cumulative rates arrive with five independent 16-byte
-// loads, and the chosen payload sits a few lines further in the same block.
+// Row blocks: the hop kernel's copy of the CSR.  Each site owns whole 128-byte lines (the L2
+// line), so a hop with at most ROW_INLINE neighbours touches exactly one line.  First line, in
+// 16-byte units:
+//   u0      n | site | Gamma (FP64)
+//   u1..u4  cum[0..8)   FP64 prefix sums (unused slots past n are zero)
+//   u5, u6  tgt[0..8)   line index of the target's block | EDGE_DRAIN_BIT for a drain target
+//   u7      x, y, z     FP32 position of the site | 0
+// A site with n > ROW_INLINE continues on the next lines: cum[8..n) then tgt[8..n).  The
+// displacement of a hop is the difference of the two sites' FP32 positions, so the block needs
+// no per-pair payload.
 // ---------------------------------------------------------------------------------------------
+constexpr uint32_t ROW_INLINE = 8;
+
+__host__ __device__ __forceinline__ uint32_t block_lines(uint32_t n) {
+  const uint32_t m = n > ROW_INLINE ? n - ROW_INLINE : 0u;
+  return 1u + (12u * m + 127u) / 128u;
+}
+
+// Lines of the q-th block in placement order (order[q] = its site; identity when NULL).
 __global__ __launch_bounds__(256) void k_block_sizes(int64_t n, const site_hdr* __restrict__ hdr,
-                                                     uint32_t* __restrict__ units) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t d = hdr[i].n;
-  units[i] = 1u + (d + 1u) / 2u + d;
+                                                     const int32_t* __restrict__ order,
+                                                     uint32_t* __restrict__ lines) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  lines[q] = block_lines(hdr[order ? order[q] : q].n);
+}
+
+// roff[site] = first line of its block, from the scanned line counts in placement order.
+__global__ __launch_bounds__(256) void k_place_blocks(int64_t n, const int32_t* __restrict__ order,
+                                                      const int64_t* __restrict__ first,
+                                                      int64_t* __restrict__ roff) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q > n) return;
+  if (q == n) roff[n] = first[n];
+  else roff[order[q]] = first[q];
+}
+
+// Sort keys of the excitons: the first line of their current site's block.
+__global__ __launch_bounds__(256) void k_exciton_keys(int64_t n, const int32_t* __restrict__ site,
+                                                      const int64_t* __restrict__ roff,
+                                                      uint32_t* __restrict__ keys,
+                                                      uint32_t* __restrict__ ids) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  keys[e] = (uint32_t)roff[site[e]];
+  ids[e] = (uint32_t)e;
 }
 
 __global__ __launch_bounds__(256) void k_pack_rows(int64_t n, const site_hdr* __restrict__ hdr,
                                                    const double* __restrict__ cum,
                                                    const edge_rec* __restrict__ edge,
                                                    const int64_t* __restrict__ roff,
+                                                   const double* __restrict__ px,
+                                                   const double* __restrict__ py,
+                                                   const double* __restrict__ pz,
                                                    uint4* __restrict__ blocks) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const site_hdr h = hdr[i];
-  const uint32_t R = (uint32_t)roff[i];
+  uint4* b = blocks + (size_t)ROW_UNITS * (size_t)roff[i];
   uint4 head;
   head.x = h.n;
   head.y = (uint32_t)i;
   head.z = (uint32_t)__double2loint(h.gamma);
   head.w = (uint32_t)__double2hiint(h.gamma);
-  blocks[R] = head;
-  double* c = reinterpret_cast<double*>(blocks + R + 1);
-  for (uint32_t k = 0; k < h.n; ++k) c[k] = cum[h.rs + k];
-  if (h.n & 1u) c[h.n] = 0.0;
-  uint4* pay = blocks + R + 1 + (h.n + 1u) / 2u;
+  b[0] = head;
+  double* c = reinterpret_cast<double*>(b + 1);
+  uint32_t* t = reinterpret_cast<uint32_t*>(b + 5);
+  const uint32_t m = h.n > ROW_INLINE ? h.n - ROW_INLINE : 0u;
+  double* ce = reinterpret_cast<double*>(b + ROW_UNITS);
+  uint32_t* te = reinterpret_cast<uint32_t*>(ce + m);
   for (uint32_t k = 0; k < h.n; ++k) {
     const edge_rec e = edge[h.rs + k];
     const uint32_t j = e.col & EDGE_COL_MASK;
-    uint4 p;
-    p.x = (uint32_t)roff[j] | (e.col & EDGE_DRAIN_BIT);
-    p.y = __float_as_uint(e.dx);
-    p.z = __float_as_uint(e.dy);
-    p.w = __float_as_uint(e.dz);
-    pay[k] = p;
+    const uint32_t tg = (uint32_t)roff[j] | (e.col & EDGE_DRAIN_BIT);
+    if (k < ROW_INLINE) {
+      c[k] = cum[h.rs + k];
+      t[k] = tg;
+    } else {
+      ce[k - ROW_INLINE] = cum[h.rs + k];
+      te[k - ROW_INLINE] = tg;
+    }
   }
+  b[7] = make_uint4(__float_as_uint((float)px[i]), __float_as_uint((float)py[i]),
+                    __float_as_uint((float)pz[i]), 0u);
 }
 
 __global__ __launch_bounds__(256) void k_gather_offsets(int64_t n, const int32_t* __restrict__ idx,
@@ -357,12 +399,19 @@ struct row_regs {
   uint32_t n, site;
   double gamma;
   double c[8];
+  uint4 t0, t1;   // tgt[0..4), tgt[4..8)
+  float x, y, z;
 };
 
+// The eight 16-byte loads of one block's first line, issued together.
 __device__ __forceinline__ row_regs load_row(const uint4* __restrict__ blocks, uint32_t R) {
-  const uint4 h = blocks[R];
-  const uint4 q0 = blocks[R + 1], q1 = blocks[R + 2], q2 = blocks[R + 3], q3 = blocks[R + 4];
+  const uint4* b = blocks + (size_t)ROW_UNITS * R;
+  const uint4 h = b[0];
+  const uint4 q0 = b[1], q1 = b[2], q2 = b[3], q3 = b[4];
   row_regs r;
+  r.t0 = b[5];
+  r.t1 = b[6];
+  const uint4 p = b[7];
   r.n = h.x;
   r.site = h.y;
   r.gamma = __hiloint2double((int)h.w, (int)h.z);
@@ -374,6 +423,9 @@ __device__ __forceinline__ row_regs load_row(const uint4* __restrict__ blocks, u
   r.c[5] = __hiloint2double((int)q2.w, (int)q2.z);
   r.c[6] = __hiloint2double((int)q3.y, (int)q3.x);
   r.c[7] = __hiloint2double((int)q3.w, (int)q3.z);
+  r.x = __uint_as_float(p.x);
+  r.y = __uint_as_float(p.y);
+  r.z = __uint_as_float(p.z);
   return r;
 }
 
@@ -388,12 +440,36 @@ __device__ __forceinline__ uint32_t select_entry(const uint4* __restrict__ block
 #pragma unroll
   for (uint32_t q = 0; q < 8; ++q) idx += (q < lim && r.c[q] <= dice) ? 1u : 0u;
   if (idx == 8u && lim > 8u) {
-    const double* cg = reinterpret_cast<const double*>(blocks + R + 1);
+    const double* ce = reinterpret_cast<const double*>(blocks + (size_t)ROW_UNITS * R + ROW_UNITS);
     uint32_t k = 8;
-    while (k < lim && cg[k] <= dice) ++k;
+    while (k < lim && ce[k - 8u] <= dice) ++k;
     idx = k;
   }
   return idx;
+}
+
+// Target word of entry idx: from the registers for idx < 8, else from the continuation lines.
+__device__ __forceinline__ uint32_t target_of(const uint4* __restrict__ blocks, uint32_t R,
+                                              const row_regs& r, uint32_t idx) {
+  if (idx >= ROW_INLINE) {
+    const uint32_t m = r.n - ROW_INLINE;
+    const double* ce = reinterpret_cast<const double*>(blocks + (size_t)ROW_UNITS * R + ROW_UNITS);
+    return reinterpret_cast<const uint32_t*>(ce + m)[idx - ROW_INLINE];
+  }
+  const uint4 q = idx < 4u ? r.t0 : r.t1;
+  const uint32_t k = idx & 3u;
+  return k == 0u ? q.x : k == 1u ? q.y : k == 2u ? q.z : q.w;
+}
+
+// One coordinate of a hop's displacement: FP64 difference of the two FP32 positions (exact for
+// sites this close), shifted by one period on a periodic axis when it exceeds half of it.
+__device__ __forceinline__ double min_image(double d, double period) {
+#pragma clang fp contract(off)
+  if (period > 0.0) {
+    if (d > 0.5 * period) d -= period;
+    else if (d < -0.5 * period) d += period;
+  }
+  return d;
 }
 
 // Per-thread write-back cache of departure counts: four (site, count) entries in LRU order,
@@ -440,7 +516,7 @@ __global__ __launch_bounds__(256) void k_inject(exciton_arrays x, const uint4* _
   const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= x.n) return;
   const u32x4 w = draw4(seed, e, 0, STREAM_INIT);
-  const uint4 h = blocks[src_roff[r31(w.v[0]) % n_src]];
+  const uint4 h = blocks[(size_t)ROW_UNITS * src_roff[r31(w.v[0]) % n_src]];
   const uint32_t r = nonzero_draw(w.v[1], w.v[3], seed, e, 0, STREAM_INIT_REDRAW);
   x.site[e] = (int32_t)h.y;
   x.t_next[e] = 0.0 + dwell_time(h.x, __hiloint2double((int)h.w, (int)h.z), r);
@@ -456,11 +532,18 @@ __global__ __launch_bounds__(256) void k_inject(exciton_arrays x, const uint4* _
 
 // One thread per exciton; each thread runs its exciton through its whole hop budget
 // (event-driven form of particle::step, src/monte_carlo/particle.cpp:57-80, with v = 0).
+// One thread per exciton.  Workgroups are dealt to the XCDs in turn, so workgroup b takes the
+// b%8-th eighth of the (sorted) exciton order: each XCD's L2 serves excitons that sit on
+// neighbouring blocks.  The exciton itself is perm[position] (its id keys every draw, so the
+// order changes nothing but locality).
 template <bool COUNT, bool SAMPLE, bool TRACE>
-__global__ __launch_bounds__(256) void k_hop(hop_args a) {
+__global__ __launch_bounds__(256, 4) void k_hop(hop_args a) {
 #pragma clang fp contract(off)
-  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= a.x.n) return;
+  const uint32_t per_xcd = gridDim.x / KMC_COUNT_COPIES;
+  const uint32_t chunk = (blockIdx.x % KMC_COUNT_COPIES) * per_xcd + blockIdx.x / KMC_COUNT_COPIES;
+  const uint32_t pos = chunk * blockDim.x + threadIdx.x;
+  if (pos >= a.x.n) return;
+  const uint32_t e = a.perm ? a.perm[pos] : pos;
   double t = a.x.t_next[e];
   uint64_t h = a.x.hops[e];
   double dx = a.x.dx[e], dy = a.x.dy[e], dz = a.x.dz[e];
@@ -495,22 +578,23 @@ __global__ __launch_bounds__(256) void k_hop(hop_args a) {
     const u32x4 w = draw4(a.seed, e, h, STREAM_HOP);
     const double dice = (row.gamma * (double)r31(w.v[0])) / RAND_MAX_D;
     const uint32_t idx = select_entry(a.blocks, R, row, dice);
-    const uint4 p = a.blocks[R + 1u + (row.n + 1u) / 2u + idx];
+    const uint32_t tg = target_of(a.blocks, R, row, idx);
     if (COUNT) cc.add(row.site, cbase);
-    dx = dx + (double)__uint_as_float(p.y);
-    dy = dy + (double)__uint_as_float(p.z);
-    dz = dz + (double)__uint_as_float(p.w);
+    const float sx = row.x, sy = row.y, sz = row.z;
     const double t_hop = t;
 
     // dwell at the new site (particle.cpp:183 -> ff_time)
-    R = p.x & EDGE_COL_MASK;
+    R = tg & EDGE_COL_MASK;
     row = load_row(a.blocks, R);
+    dx = dx + min_image((double)row.x - (double)sx, a.per_len[0]);
+    dy = dy + min_image((double)row.y - (double)sy, a.per_len[1]);
+    dz = dz + min_image((double)row.z - (double)sz, a.per_len[2]);
     hs = (hs ^ (uint64_t)row.site) * FNV_PRIME;
     const uint32_t r = nonzero_draw(w.v[1], w.v[3], a.seed, e, h, STREAM_HOP_REDRAW);
     t = t + dwell_time(row.n, row.gamma, r);
     // drain: count current and re-inject, keeping displacement and dwell (kubo_step,
     // src/monte_carlo/monte_carlo.cpp:330-336)
-    if (p.x & EDGE_DRAIN_BIT) {
+    if (tg & EDGE_DRAIN_BIT) {
       ++nd;
       R = a.src_roff[r31(w.v[2]) % a.n_src];
       row = load_row(a.blocks, R);
@@ -538,7 +622,10 @@ __global__ __launch_bounds__(256) void k_hop(hop_args a) {
 }
 
 hipError_t launch_hop(const hop_args& a, bool count, bool sample, bool trace, hipStream_t st) {
-  const dim3 grid((unsigned)((a.x.n + 255) / 256)), block(256);
+  // a multiple of KMC_COUNT_COPIES workgroups, so the per-XCD chunks tile the excitons exactly
+  const uint64_t groups = (a.x.n + 255) / 256;
+  const dim3 grid((unsigned)((groups + KMC_COUNT_COPIES - 1) / KMC_COUNT_COPIES * KMC_COUNT_COPIES)),
+      block(256);
 #define KMC_HOP_CASE(C, S, T)                                        \
   if (count == C && sample == S && trace == T) {                     \
     hipLaunchKernelGGL((k_hop<C, S, T>), grid, block, 0, st, a);     \
@@ -678,17 +765,32 @@ hipError_t launch_inject(const exciton_arrays& x, const uint4* blocks, const uin
   return hipGetLastError();
 }
 
-hipError_t launch_block_sizes(int64_t n, const site_hdr* hdr, uint32_t* units, hipStream_t st) {
+hipError_t launch_block_sizes(int64_t n, const site_hdr* hdr, const int32_t* order, uint32_t* lines,
+                              hipStream_t st) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_block_sizes, grid_for(n), dim3(256), 0, st, n, hdr, units);
+  hipLaunchKernelGGL(k_block_sizes, grid_for(n), dim3(256), 0, st, n, hdr, order, lines);
+  return hipGetLastError();
+}
+
+hipError_t launch_place_blocks(int64_t n, const int32_t* order, const int64_t* first, int64_t* roff,
+                               hipStream_t st) {
+  hipLaunchKernelGGL(k_place_blocks, grid_for(n + 1), dim3(256), 0, st, n, order, first, roff);
+  return hipGetLastError();
+}
+
+hipError_t launch_exciton_keys(int64_t n, const int32_t* site, const int64_t* roff, uint32_t* keys,
+                               uint32_t* ids, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_exciton_keys, grid_for(n), dim3(256), 0, st, n, site, roff, keys, ids);
   return hipGetLastError();
 }
 
 hipError_t launch_fill_blocks(int64_t n, const site_hdr* hdr, const double* cum,
-                              const edge_rec* edge, const int64_t* roff, uint4* blocks,
-                              hipStream_t st) {
+                              const edge_rec* edge, const int64_t* roff, const double* px,
+                              const double* py, const double* pz, uint4* blocks, hipStream_t st) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_pack_rows, grid_for(n), dim3(256), 0, st, n, hdr, cum, edge, roff, blocks);
+  hipLaunchKernelGGL(k_pack_rows, grid_for(n), dim3(256), 0, st, n, hdr, cum, edge, roff, px, py,
+                     pz, blocks);
   return hipGetLastError();
 }
 

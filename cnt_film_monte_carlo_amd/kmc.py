@@ -101,6 +101,7 @@ _SIGS = {
                                     C.c_int32, _D, _D]),
     "kmc_linspace": (C.c_int, [C.c_double, C.c_double, C.c_int32, _D]),
     "kmc_build_rates_exact": (C.c_int, [_P, C.c_int32, C.POINTER(_P), _D]),
+    "kmc_set_hop_schedule": (C.c_int, [_P, C.c_int64]),
     "kmc_build_rates_mlp": (C.c_int, [_P, _P, C.c_int32, _I32, C.c_double, C.POINTER(_P), _I64, _D]),
     "kmc_pair_classes": (C.c_int, [_P, C.c_int32, _I64, C.POINTER(_P)]),
     "kmc_get_pair_rates": (C.c_int, [_P, _D]),
@@ -218,6 +219,8 @@ class Context:
         self._ck(self.lib.kmc_set_sites(self.h, n, *[_ptr(a, _D) for a in arrs], _ptr(left, _I32),
                                         _ptr(right, _I32), _ptr(chir, _U8), _ptr(fl, _U8)))
         self.n_sites = n
+        # FP32 positions: the hop kernel's displacements are differences of these
+        self.site_pos = np.stack(arrs[:3], 1).astype(np.float32)
 
     def build_cells(self):
         self._ck(self.lib.kmc_build_cells(self.h))
@@ -314,7 +317,10 @@ class Context:
         gam = np.zeros(self.n_sites, np.float64)
         self._ck(self.lib.kmc_get_csr(self.h, _ptr(row, _I64), _ptr(col, _I32), _ptr(cum, _D),
                                       _ptr(disp, _F), _ptr(gam, _D)))
-        return dict(row_ptr=row, col=col, cum=cum, disp=disp, gamma=gam)
+        per = np.array([self.cfg.period_len[a] if self.cfg.periodic_mask & (1 << a) else 0.0
+                        for a in range(3)])
+        return dict(row_ptr=row, col=col, cum=cum, disp=disp, gamma=gam, period=per,
+                    site_pos=getattr(self, "site_pos", None))
 
     def set_csr(self, row_ptr, col, cum, disp):
         row = _c(row_ptr, np.int64)
@@ -344,6 +350,10 @@ class Context:
         self._ck(self.lib.kmc_run(self.h, int(hops_per_exciton), float(t_max), int(count_sites),
                                   C.byref(ms) if timed else None))
         return ms.value
+
+    def set_hop_schedule(self, resort_hops: int):
+        """Hops per sub-launch of run() between exciton sorts (0: one launch)."""
+        self._ck(self.lib.kmc_set_hop_schedule(self.h, int(resort_hops)))
 
     def sync(self):
         self._ck(self.lib.kmc_sync(self.h))

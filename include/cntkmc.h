@@ -136,9 +136,14 @@ int kmc_set_trace(kmc_ctx* ctx, int64_t n_trace, int64_t max_hops);
  * whose time exceeds t_max (pass INFINITY for a pure hop budget).  Re-injection on reaching a
  * drain site keeps the displacement and the pending dwell time (kubo_step, monte_carlo.cpp:330-336).
  * count_sites != 0 adds per-site departure counts.  kernel_ms (may be NULL) receives the device
- * time of the hop kernel, measured with HIP events on the context stream. */
+ * time of the whole pass (all sub-launches and the exciton sorts between them), measured with
+ * HIP events on the context stream. */
 int kmc_run(kmc_ctx* ctx, int64_t hops_per_exciton, double t_max, int32_t count_sites,
             double* kernel_ms);
+/* Launch schedule of kmc_run (no reference counterpart; results do not depend on it): the hop
+ * budget runs as sub-launches of at most resort_hops hops, each preceded by a sort of the
+ * excitons by the memory block of their current site; 0 = one launch (default 625). */
+int kmc_set_hop_schedule(kmc_ctx* ctx, int64_t resort_hops);
 
 /* Per-exciton state read-back: current site, time of the next hop, hops done, displacement
  * [E][3], drain (current) count, FNV-1a hash of the visited-site sequence.  NULL = skip. */

@@ -551,7 +551,7 @@ typedef struct orc_run_args {
   const int64_t* row_ptr;
   const int32_t* col;
   const double* cum;
-  const float* disp;
+  const float* pos; /* [n_sites][3] FP32 site positions: displacement = pos[target] - pos[source] */
   const uint8_t* flags;
   int64_t n_src;
   const int32_t* src;
@@ -583,13 +583,24 @@ typedef struct orc_run_args {
   /* faithful mode: recompute neighbours per hop from this network (NULL = use the CSR) */
   const orc_net* net;
   int n_threads;
+  double per_len[3]; /* period per axis (0 = open): displacements take the minimum image */
 } orc_run_args;
+
+/* One coordinate of a hop's displacement: the difference of two FP32 positions in FP64 (exact
+ * for sites this close), shifted by one period on a periodic axis when it exceeds half of it. */
+static double orc_min_image(double d, double period) {
+  if (period > 0.0) {
+    if (d > 0.5 * period) d -= period;
+    else if (d < -0.5 * period) d += period;
+  }
+  return d;
+}
 
 /* One exciton through its budget: particle::step (particle.cpp:57-80) in event form,
  * update_state (scatterer.cpp:9-31), ff_time (scatterer.h:74-80), re-injection on a drain
  * site keeping delta_pos and ff_time (monte_carlo.cpp:330-336). */
 static void orc_one(const orc_run_args* a, int64_t e, orc_cand* nb, double* rowbuf,
-                    float* dispbuf, int32_t* colbuf, int64_t cap) {
+                    int32_t* colbuf, int64_t cap) {
   int32_t s = a->site[e];
   double t = a->t_next[e];
   uint64_t h = a->hops[e];
@@ -604,7 +615,6 @@ static void orc_one(const orc_run_args* a, int64_t e, orc_cand* nb, double* rowb
   for (;;) {
     const double* row;
     const int32_t* rcol;
-    const float* rdisp;
     int64_t n;
     uint32_t w[4], r;
     unsigned idx;
@@ -643,15 +653,12 @@ static void orc_one(const orc_run_args* a, int64_t e, orc_cand* nb, double* rowb
         rate = get_rate(w0, w0->chir[s], w0->chir[jj], th, zs, s1, s2);
         rowbuf[q] = (q == 0) ? rate : rate + rowbuf[q - 1];
         colbuf[q] = jj;
-        for (kk = 0; kk < 3; ++kk) dispbuf[3 * q + kk] = (float)(nb[q].pj[kk] - pi[kk]);
       }
       row = rowbuf;
       rcol = colbuf;
-      rdisp = dispbuf;
     } else {
       row = a->cum + a->row_ptr[s];
       rcol = a->col + a->row_ptr[s];
-      rdisp = a->disp + 3 * a->row_ptr[s];
       n = a->row_ptr[s + 1] - a->row_ptr[s];
     }
     if (h >= h_end || n == 0 || !(row[n - 1] > 0.0) || !(t <= a->t_max)) break;
@@ -664,9 +671,9 @@ static void orc_one(const orc_run_args* a, int64_t e, orc_cand* nb, double* rowb
 #pragma omp atomic
       a->site_count[s] += 1;
     }
-    dx = dx + (double)rdisp[3 * idx];
-    dy = dy + (double)rdisp[3 * idx + 1];
-    dz = dz + (double)rdisp[3 * idx + 2];
+    dx = dx + orc_min_image((double)a->pos[3 * j] - (double)a->pos[3 * s], a->per_len[0]);
+    dy = dy + orc_min_image((double)a->pos[3 * j + 1] - (double)a->pos[3 * s + 1], a->per_len[1]);
+    dz = dz + orc_min_image((double)a->pos[3 * j + 2] - (double)a->pos[3 * s + 2], a->per_len[2]);
     hs = (hs ^ (uint64_t)(uint32_t)j) * FNV_PR;
     r = dwell_draw(a->seed, (uint32_t)e, h, w[1], w[3], 2u);
     t_hop = t;
@@ -740,13 +747,11 @@ int orc_run(const orc_run_args* a) {
   {
     orc_cand* nb = (orc_cand*)malloc(cap * sizeof(orc_cand));
     double* rowbuf = (double*)malloc(cap * sizeof(double));
-    float* dispbuf = (float*)malloc(3 * cap * sizeof(float));
     int32_t* colbuf = (int32_t*)malloc(cap * sizeof(int32_t));
 #pragma omp for schedule(dynamic, 16)
-    for (e = 0; e < a->E; ++e) orc_one(a, e, nb, rowbuf, dispbuf, colbuf, cap);
+    for (e = 0; e < a->E; ++e) orc_one(a, e, nb, rowbuf, colbuf, cap);
     free(nb);
     free(rowbuf);
-    free(dispbuf);
     free(colbuf);
   }
   return 0;

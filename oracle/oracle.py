@@ -27,7 +27,7 @@ _U64 = C.POINTER(C.c_uint64)
 
 class RunArgs(C.Structure):
     _fields_ = [
-        ("n_sites", C.c_int64), ("row_ptr", _I64), ("col", _I32), ("cum", _D), ("disp", _F),
+        ("n_sites", C.c_int64), ("row_ptr", _I64), ("col", _I32), ("cum", _D), ("pos", _F),
         ("flags", _U8), ("n_src", C.c_int64), ("src", _I32), ("seed", C.c_uint64),
         ("E", C.c_int64), ("site", _I32), ("t_next", _D), ("hops", _U64), ("dsp", _D),
         ("drains", _U32), ("hash", _U64), ("k_rec", _U32),
@@ -35,7 +35,7 @@ class RunArgs(C.Structure):
         ("n_bins", C.c_int32), ("t_sample_max", C.c_double), ("rec_disp", _D),
         ("rec_drain", _U32), ("site_count", _U64),
         ("n_trace", C.c_int64), ("tr_max", C.c_int64), ("tr_site", _I32), ("tr_time", _D),
-        ("tr_n", _I64), ("net", _P), ("n_threads", C.c_int),
+        ("tr_n", _I64), ("net", _P), ("n_threads", C.c_int), ("per_len", C.c_double * 3),
     ]
 
 
@@ -135,6 +135,7 @@ class Net:
         self.flags = _c(net.get("flags", np.zeros(n)), np.uint8)
         plo = _c(period_lo, np.float64)
         pln = _c(period_len, np.float64)
+        self.period = np.array([pln[a] if periodic_mask & (1 << a) else 0.0 for a in range(3)])
         self.h = self.lib.orc_net_create(
             n, *[_p(self.arr[k], _D) for k in ("x", "y", "z", "ox", "oy", "oz")],
             _p(self.chir, _U8), _p(self.flags, _U8), cutoff, min_distance, periodic_mask,
@@ -177,7 +178,8 @@ class Net:
         self.lib.orc_net_fill(self.h, mode, _p(row, _I64), _p(pr, _D), _p(col, _I32),
                               _p(cum, _D) if (mode == 0 or pr is not None) else None,
                               _p(disp, _F), *[_p(g, _D) for g in geo])
-        out = dict(row_ptr=row, col=col, cum=cum, disp=disp)
+        out = dict(row_ptr=row, col=col, cum=cum, disp=disp, period=self.period.copy(),
+                   site_pos=np.stack([self.arr[k] for k in ("x", "y", "z")], 1).astype(np.float32))
         if geometry:
             out.update(theta=geo[0], z=geo[1], a1=geo[2], a2=geo[3])
         return out
@@ -187,11 +189,13 @@ class Excitons:
     """Oracle exciton ensemble over a CSR (or recomputing neighbours per hop)."""
 
     def __init__(self, csr: dict, flags: np.ndarray, seed: int, n_excitons: int):
+        """csr: row_ptr, col, cum, site_pos (FP32 [n][3]) and period (per axis, 0 = open)."""
         self.lib = library()
         self.row = _c(csr["row_ptr"], np.int64)
         self.col = _c(csr["col"], np.int32)
         self.cum = _c(csr["cum"], np.float64)
-        self.disp = _c(csr["disp"], np.float32)
+        self.pos = _c(csr["site_pos"], np.float32)
+        self.per_len = np.asarray(csr.get("period", (0.0, 0.0, 0.0)), np.float64)
         self.flags = _c(flags, np.uint8)
         self.src = np.nonzero(self.flags & 1)[0].astype(np.int32)
         self.seed = seed
@@ -238,7 +242,9 @@ class Excitons:
         a = RunArgs()
         a.n_sites = self.row.size - 1
         a.row_ptr, a.col, a.cum = _p(self.row, _I64), _p(self.col, _I32), _p(self.cum, _D)
-        a.disp, a.flags = _p(self.disp, _F), _p(self.flags, _U8)
+        a.pos, a.flags = _p(self.pos, _F), _p(self.flags, _U8)
+        for k in range(3):
+            a.per_len[k] = float(self.per_len[k])
         a.n_src, a.src, a.seed = self.src.size, _p(self.src, _I32), self.seed
         a.E = self.E
         a.site, a.t_next, a.hops = _p(self.site, _I32), _p(self.t_next, _D), _p(self.hops, _U64)
