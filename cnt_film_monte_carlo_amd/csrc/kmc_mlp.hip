@@ -1073,8 +1073,9 @@ int kmc_mlp_table(kmc_mlp* m, int32_t cls_d, int32_t cls_a, double T, kmc_transf
   if (s != KMC_OK) return s;
   std::vector<double> sub;
   std::vector<int64_t> idx;
+  // coaxial nodes (theta = 0, a1 = 0) serve every same-tube hop: always exact
   for (int64_t e = 0; e < n; ++e)
-    if (fl[e]) idx.push_back(e);
+    if (fl[e] || (g[e] == 0.0 && g[2 * n + e] == 0.0)) idx.push_back(e);
   if (fb && !idx.empty()) {
     const int64_t k = (int64_t)idx.size();
     sub.resize(4 * k);

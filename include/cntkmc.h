@@ -375,7 +375,8 @@ int kmc_mlp_rates(kmc_mlp* mlp, int64_t n, const double* theta, const double* z_
                   const double* axis_shift_1, const double* axis_shift_2, int32_t cls_d,
                   int32_t cls_a, double temperature, double* rate, uint8_t* flag,
                   double* kernel_ms);
-/* Surrogate-built 4-D table (layout of kmc_transfer_table).  Flagged nodes are recomputed by
+/* Surrogate-built 4-D table (layout of kmc_transfer_table).  Flagged nodes, and the coaxial
+ * nodes (theta = 0 and axis shift 1 = 0, the geometry of every same-tube hop), are recomputed by
  * `fallback` (the exact transfer of the same pair and temperature; NULL leaves the network's
  * value).  n_fallback (may be NULL) = nodes recomputed; elapsed_ms = wall time of the build. */
 int kmc_mlp_table(kmc_mlp* mlp, int32_t cls_d, int32_t cls_a, double temperature,
