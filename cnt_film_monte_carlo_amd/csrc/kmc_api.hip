@@ -197,6 +197,28 @@ exciton_arrays excitons_of(kmc_ctx* c) {
   return x;
 }
 
+// The hop loop's view of one context (its excitons, blocks, samples and counts).
+net_view view_of(kmc_ctx* c) {
+  net_view v{};
+  v.x = excitons_of(c);
+  v.blocks = c->d_blocks.p;
+  v.roff = c->d_roff.p;
+  v.src_roff = c->d_src_roff.p;
+  v.n_src = (uint32_t)c->h_src.size();
+  v.seed = c->cfg.seed;
+  for (int k = 0; k < 3; ++k)
+    v.per_len[k] = (c->cfg.periodic_mask & (1 << k)) ? c->cfg.period_len[k] : 0.0;
+  v.n_bins = c->n_bins;
+  v.dt_sample = c->n_bins > 0 ? c->t_sample_max / c->n_bins : 0.0;
+  v.rec_dx = c->r_dx.p;
+  v.rec_dy = c->r_dy.p;
+  v.rec_dz = c->r_dz.p;
+  v.rec_drain = c->r_drain.p;
+  v.site_count = c->d_site_count.p;
+  v.count_stride = c->count_stride;
+  return v;
+}
+
 void reset_downstream(kmc_ctx* c, int level) {
   // level 0: sites changed; 1: cells/tables changed; 2: CSR changed
   if (level <= 0) c->cells_ok = false;
@@ -901,6 +923,41 @@ int kmc_set_trace(kmc_ctx* c, int64_t n_trace, int64_t max_hops) {
   return KMC_OK;
 }
 
+// The hop budget runs as sub-launches of at most `chunk` hops (0: one launch); before each, the
+// excitons are sorted by the block of their current site (radix sort of 32-bit keys) so that
+// neighbouring threads and each XCD's L2 share rows.  Draws are keyed by exciton and hop, so the
+// split and the order leave every trajectory unchanged.
+static int run_hops(kmc_ctx* c, hop_args a, int64_t hops, int64_t chunk, bool count, bool sample,
+                    bool trace) {
+  const int64_t E = c->n_exc;
+  KMC_TRY(dalloc(c, c->d_keys, 2 * E));
+  KMC_TRY(dalloc(c, c->d_ids, 2 * E));
+  size_t need = 0;
+  KMC_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, need, c->d_keys.p, c->d_keys.p + E,
+                                                c->d_ids.p, c->d_ids.p + E, (int)E, 0, 32,
+                                                c->stream));
+  if (need > c->sort_tmp_bytes) {
+    KMC_TRY(dalloc(c, c->d_sort_tmp, need));
+    c->sort_tmp_bytes = need;
+  }
+  a.perm = c->d_ids.p + E;
+  const int64_t step = chunk > 0 ? chunk : std::max<int64_t>(hops, 1);
+  int64_t done = 0;
+  do {
+    const int64_t sub = std::min(step, hops - done);
+    KMC_HIP(c, launch_exciton_keys(E, c->x_site.p, c->d_roff.p, c->d_keys.p, c->d_ids.p,
+                                   c->stream));
+    size_t have = c->sort_tmp_bytes;
+    KMC_HIP(c, hipcub::DeviceRadixSort::SortPairs(c->d_sort_tmp.p, have, c->d_keys.p,
+                                                  c->d_keys.p + E, c->d_ids.p, c->d_ids.p + E,
+                                                  (int)E, 0, 32, c->stream));
+    a.hop_budget = (uint64_t)sub;
+    KMC_HIP(c, launch_hop(a, count, sample, trace, c->stream));
+    done += sub;
+  } while (done < hops);
+  return KMC_OK;
+}
+
 int kmc_set_hop_schedule(kmc_ctx* c, int64_t resort_hops) {
   if (!c || resort_hops < 0) return fail(c, KMC_ERR_ARG, "kmc_set_hop_schedule: bad argument");
   c->resort_hops = resort_hops;
@@ -915,62 +972,16 @@ int kmc_run(kmc_ctx* c, int64_t hops_per_exciton, double t_max, int32_t count_si
     return fail(c, KMC_ERR_ARG, "kmc_run: need hops_per_exciton >= 0 and t_max not NaN");
   KMC_HIP(c, hipSetDevice(c->cfg.device));
   hop_args a{};
-  a.x = excitons_of(c);
-  a.blocks = c->d_blocks.p;
-  a.roff = c->d_roff.p;
-  a.src_roff = c->d_src_roff.p;
-  a.n_src = (uint32_t)c->h_src.size();
-  a.seed = c->cfg.seed;
+  a.v = view_of(c);
   a.t_max = t_max;
-  a.n_bins = c->n_bins;
-  a.dt_sample = c->n_bins > 0 ? c->t_sample_max / c->n_bins : 0.0;
-  a.rec_dx = c->r_dx.p;
-  a.rec_dy = c->r_dy.p;
-  a.rec_dz = c->r_dz.p;
-  a.rec_drain = c->r_drain.p;
-  a.site_count = c->d_site_count.p;
-  a.count_stride = c->count_stride;
-  a.n_trace = c->n_trace;
-  a.tr_max = c->tr_max;
-  a.tr_site = c->tr_site.p;
-  a.tr_time = c->tr_time.p;
-  a.tr_n = c->tr_n.p;
-  for (int k = 0; k < 3; ++k)
-    a.per_len[k] = (c->cfg.periodic_mask & (1 << k)) ? c->cfg.period_len[k] : 0.0;
-  // The budget runs as sub-launches of at most resort_hops hops; before each, the excitons are
-  // sorted by the block of their current site (radix sort of 32-bit keys) so that neighbouring
-  // threads and each XCD's L2 share rows.  Draws are keyed by exciton and hop, so the split and
-  // the order leave every trajectory unchanged.
-  const int64_t E = c->n_exc;
-  KMC_TRY(dalloc(c, c->d_keys, 2 * E));
-  KMC_TRY(dalloc(c, c->d_ids, 2 * E));
-  {
-    size_t need = 0;
-    KMC_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, need, c->d_keys.p, c->d_keys.p + E,
-                                                  c->d_ids.p, c->d_ids.p + E, (int)E, 0, 32,
-                                                  c->stream));
-    if (need > c->sort_tmp_bytes) {
-      KMC_TRY(dalloc(c, c->d_sort_tmp, need));
-      c->sort_tmp_bytes = need;
-    }
-  }
-  a.perm = c->d_ids.p + E;
-  const int64_t chunk = c->resort_hops > 0 ? c->resort_hops : std::max<int64_t>(hops_per_exciton, 1);
+  a.tv.n_trace = c->n_trace;
+  a.tv.tr_max = c->tr_max;
+  a.tv.tr_site = c->tr_site.p;
+  a.tv.tr_time = c->tr_time.p;
+  a.tv.tr_n = c->tr_n.p;
   KMC_HIP(c, hipEventRecord(c->ev0, c->stream));
-  int64_t done = 0;
-  do {
-    const int64_t sub = std::min(chunk, hops_per_exciton - done);
-    KMC_HIP(c, launch_exciton_keys(E, c->x_site.p, c->d_roff.p, c->d_keys.p, c->d_ids.p,
-                                   c->stream));
-    size_t have = c->sort_tmp_bytes;
-    KMC_HIP(c, hipcub::DeviceRadixSort::SortPairs(c->d_sort_tmp.p, have, c->d_keys.p,
-                                                  c->d_keys.p + E, c->d_ids.p, c->d_ids.p + E,
-                                                  (int)E, 0, 32, c->stream));
-    a.hop_budget = (uint64_t)sub;
-    KMC_HIP(c, launch_hop(a, count_sites != 0, c->n_bins > 0, c->n_trace > 0 && c->tr_max > 0,
-                          c->stream));
-    done += sub;
-  } while (done < hops_per_exciton);
+  KMC_TRY(run_hops(c, a, hops_per_exciton, c->resort_hops, count_sites != 0, c->n_bins > 0,
+                   c->n_trace > 0 && c->tr_max > 0));
   KMC_HIP(c, hipEventRecord(c->ev1, c->stream));
   if (kernel_ms) {
     KMC_HIP(c, hipEventSynchronize(c->ev1));
@@ -1068,6 +1079,345 @@ int kmc_reduce_samples(kmc_ctx* c, int32_t n_batches, double* out, double* kerne
     KMC_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     *kernel_ms = ms;
   }
+  return KMC_OK;
+}
+
+}  // extern "C"
+
+// =============================================================================================
+// Launch-schedule autotuning and ensembles (no reference counterpart)
+// =============================================================================================
+namespace {
+
+// Copy of one context's exciton state, so that trial runs can be undone.
+struct state_copy {
+  uint8_t* buf = nullptr;
+  int64_t n = 0;
+  static constexpr size_t per = 4 + 8 + 8 + 24 + 4 + 8 + 4 + 1;  // bytes of state per exciton
+};
+
+int save_state(kmc_ctx* c, state_copy& s, bool restore) {
+  const int64_t E = c->n_exc;
+  if (!restore) {
+    KMC_HIP(c, hipMalloc((void**)&s.buf, (size_t)E * state_copy::per));
+    s.n = E;
+  }
+  struct part { void* p; size_t b; };
+  const part parts[] = {{c->x_site.p, 4}, {c->x_t.p, 8},      {c->x_hops.p, 8}, {c->x_dx.p, 8},
+                        {c->x_dy.p, 8},   {c->x_dz.p, 8},     {c->x_drains.p, 4}, {c->x_hash.p, 8},
+                        {c->x_krec.p, 4}, {c->x_heading.p, 1}};
+  size_t off = 0;
+  for (const part& q : parts) {
+    void* dev = q.p;
+    void* bak = s.buf + off;
+    KMC_HIP(c, hipMemcpyAsync(restore ? dev : bak, restore ? bak : dev, (size_t)E * q.b,
+                              hipMemcpyDeviceToDevice, c->stream));
+    off += (size_t)E * q.b;
+  }
+  return KMC_OK;
+}
+
+}  // namespace
+
+struct kmc_ensemble {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<kmc_ctx*> ctxs;
+  std::vector<int64_t> n_exc;    // per network, fixed at creation
+  std::vector<net_view> views;   // uploaded before every run
+  int64_t n_total = 0, n_positions = 0;
+  net_view* d_nets = nullptr;
+  uint16_t *d_wave_net = nullptr, *d_exc_net = nullptr;
+  int64_t *d_pbase = nullptr, *d_ebase = nullptr;
+  uint64_t* d_keys = nullptr;    // 2 n_total: sort input and output
+  uint32_t* d_ids = nullptr;     // 2 n_total
+  void* d_sort_tmp = nullptr;
+  size_t sort_bytes = 0;
+  int end_bit = 33;
+  int64_t resort_hops = 625;
+};
+
+namespace {
+
+thread_local std::string g_ens_err;
+int efail(int code, const std::string& m) {
+  g_ens_err = m;
+  return code;
+}
+#define ENS_HIP(call)                                                                            \
+  do {                                                                                           \
+    const hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess) return efail(KMC_ERR_HIP, std::string(__func__) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// Members unchanged since creation, their pending work finished, their current views uploaded
+// (modify may adjust each view, e.g. for trial runs).
+template <typename F>
+int ens_upload(kmc_ensemble* m, F modify) {
+  for (size_t k = 0; k < m->ctxs.size(); ++k) {
+    kmc_ctx* c = m->ctxs[k];
+    if (c->n_exc != m->n_exc[k] || c->csr_state != 2)
+      return efail(KMC_ERR_STATE, "kmc_ensemble: a member was re-injected or rebuilt since "
+                                  "kmc_ensemble_create");
+    if (c->n_trace > 0) return efail(KMC_ERR_STATE, "kmc_ensemble: members may not record traces");
+    ENS_HIP(hipStreamSynchronize(c->stream));
+    m->views[k] = view_of(c);
+    modify(k, m->views[k]);
+  }
+  ENS_HIP(hipMemcpyAsync(m->d_nets, m->views.data(), m->views.size() * sizeof(net_view),
+                         hipMemcpyHostToDevice, m->stream));
+  return KMC_OK;
+}
+
+int ens_run_hops(kmc_ensemble* m, int64_t hops, int64_t chunk, bool count, bool sample,
+                 double t_max) {
+  ens_args a{};
+  a.nets = m->d_nets;
+  a.wave_net = m->d_wave_net;
+  a.pbase = m->d_pbase;
+  a.ebase = m->d_ebase;
+  a.perm = m->d_ids + m->n_total;
+  a.n_positions = (uint64_t)m->n_positions;
+  a.t_max = t_max;
+  const int64_t step = chunk > 0 ? chunk : std::max<int64_t>(hops, 1);
+  int64_t done = 0;
+  do {
+    const int64_t sub = std::min(step, hops - done);
+    ENS_HIP(launch_ens_keys(m->n_total, m->d_nets, m->d_exc_net, m->d_ebase, m->d_keys, m->d_ids,
+                            m->stream));
+    size_t have = m->sort_bytes;
+    ENS_HIP(hipcub::DeviceRadixSort::SortPairs(m->d_sort_tmp, have, m->d_keys,
+                                               m->d_keys + m->n_total, m->d_ids,
+                                               m->d_ids + m->n_total, (int)m->n_total, 0,
+                                               m->end_bit, m->stream));
+    a.hop_budget = (uint64_t)sub;
+    ENS_HIP(launch_hop_ens(a, count, sample, m->stream));
+    done += sub;
+  } while (done < hops);
+  return KMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kmc_autotune_schedule(kmc_ctx* c, int64_t trial_hops, int32_t n_cand, const int64_t* cand,
+                          int64_t* best, double* hops_per_s) {
+  if (!c || trial_hops < 1 || n_cand < 1 || !cand)
+    return fail(c, KMC_ERR_ARG, "kmc_autotune_schedule: bad argument");
+  if (c->n_exc == 0) return fail(c, KMC_ERR_STATE, "kmc_autotune_schedule: call kmc_inject first");
+  for (int i = 0; i < n_cand; ++i)
+    if (cand[i] < 0) return fail(c, KMC_ERR_ARG, "kmc_autotune_schedule: negative candidate");
+  KMC_HIP(c, hipSetDevice(c->cfg.device));
+  state_copy bak;
+  KMC_TRY(save_state(c, bak, false));
+  uint32_t* counts = nullptr;
+  hipError_t he = hipMalloc((void**)&counts, (size_t)KMC_COUNT_COPIES * c->count_stride * 4);
+  if (he != hipSuccess) {
+    (void)hipFree(bak.buf);
+    return hip_fail(c, he, "kmc_autotune_schedule");
+  }
+  hop_args a{};
+  a.v = view_of(c);
+  a.v.n_bins = 0;           // trials record no samples
+  a.v.site_count = counts;  // and count into scratch
+  a.t_max = INFINITY;
+  int64_t pick = cand[0];
+  double best_rate = -1.0;
+  int st = KMC_OK;
+  for (int i = 0; i < n_cand && st == KMC_OK; ++i) {
+    st = save_state(c, bak, true);
+    if (st == KMC_OK && hipEventRecord(c->ev0, c->stream) != hipSuccess) st = KMC_ERR_HIP;
+    if (st == KMC_OK) st = run_hops(c, a, trial_hops, cand[i], true, false, false);
+    if (st == KMC_OK && hipEventRecord(c->ev1, c->stream) != hipSuccess) st = KMC_ERR_HIP;
+    float ms = 0.f;
+    if (st == KMC_OK && (hipEventSynchronize(c->ev1) != hipSuccess ||
+                         hipEventElapsedTime(&ms, c->ev0, c->ev1) != hipSuccess))
+      st = KMC_ERR_HIP;
+    const double rate = (double)c->n_exc * (double)trial_hops / (ms * 1e-3);
+    if (hops_per_s) hops_per_s[i] = rate;
+    if (st == KMC_OK && rate > best_rate) {
+      best_rate = rate;
+      pick = cand[i];
+    }
+  }
+  if (st == KMC_OK) st = save_state(c, bak, true);
+  if (st == KMC_OK && hipStreamSynchronize(c->stream) != hipSuccess) st = KMC_ERR_HIP;
+  (void)hipFree(counts);
+  (void)hipFree(bak.buf);
+  if (st != KMC_OK) return fail(c, st, "kmc_autotune_schedule: a trial failed");
+  c->resort_hops = pick;
+  if (best) *best = pick;
+  return KMC_OK;
+}
+
+const char* kmc_ensemble_last_error(void) { return g_ens_err.c_str(); }
+
+int kmc_ensemble_destroy(kmc_ensemble* m) {
+  if (!m) return KMC_OK;
+  (void)hipSetDevice(m->device);
+  if (m->stream) (void)hipStreamSynchronize(m->stream);
+  void* bufs[] = {m->d_nets, m->d_wave_net, m->d_exc_net, m->d_pbase, m->d_ebase,
+                  m->d_keys, m->d_ids,      m->d_sort_tmp};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  if (m->ev0) (void)hipEventDestroy(m->ev0);
+  if (m->ev1) (void)hipEventDestroy(m->ev1);
+  if (m->stream) (void)hipStreamDestroy(m->stream);
+  delete m;
+  return KMC_OK;
+}
+
+int kmc_ensemble_create(int32_t n_networks, kmc_ctx* const* networks, kmc_ensemble** out) {
+  if (!networks || !out || n_networks < 1 || n_networks > 65535)
+    return efail(KMC_ERR_ARG, "kmc_ensemble_create: need 1 <= n_networks <= 65535");
+  *out = nullptr;
+  const int dev = networks[0] ? networks[0]->cfg.device : 0;
+  std::vector<int64_t> pbase(n_networks), ebase(n_networks);
+  int64_t np = 0, ne = 0;
+  for (int k = 0; k < n_networks; ++k) {
+    kmc_ctx* c = networks[k];
+    if (!c) return efail(KMC_ERR_ARG, "kmc_ensemble_create: null member");
+    if (c->cfg.device != dev) return efail(KMC_ERR_ARG, "kmc_ensemble_create: members on different devices");
+    if (c->csr_state != 2 || c->n_exc == 0)
+      return efail(KMC_ERR_STATE, "kmc_ensemble_create: every member needs its CSR and excitons");
+    pbase[k] = np;
+    ebase[k] = ne;
+    np += (c->n_exc + 63) / 64 * 64;
+    ne += c->n_exc;
+  }
+  if (ne >= ((int64_t)1 << 31) || np >= ((int64_t)1 << 32))
+    return efail(KMC_ERR_CAPACITY, "kmc_ensemble_create: more than 2^31 excitons");
+  std::vector<uint16_t> wave_net(np / 64), exc_net(ne);
+  for (int k = 0; k < n_networks; ++k) {
+    const int64_t E = networks[k]->n_exc;
+    for (int64_t w = pbase[k] / 64; w < (pbase[k] + (E + 63) / 64 * 64) / 64; ++w) wave_net[w] = (uint16_t)k;
+    for (int64_t e = 0; e < E; ++e) exc_net[ebase[k] + e] = (uint16_t)k;
+  }
+  kmc_ensemble* m = new kmc_ensemble();
+  m->device = dev;
+  m->ctxs.assign(networks, networks + n_networks);
+  for (kmc_ctx* c : m->ctxs) m->n_exc.push_back(c->n_exc);
+  m->views.resize(n_networks);
+  m->n_total = ne;
+  m->n_positions = np;
+  int bits = 0;
+  while ((1 << bits) < n_networks) ++bits;
+  m->end_bit = 32 + std::max(bits, 1);
+  hipError_t e = hipSetDevice(dev);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&m->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&m->ev1);
+  if (e == hipSuccess) e = hipMalloc((void**)&m->d_nets, n_networks * sizeof(net_view));
+  if (e == hipSuccess) e = hipMalloc((void**)&m->d_wave_net, wave_net.size() * sizeof(uint16_t));
+  if (e == hipSuccess) e = hipMalloc((void**)&m->d_exc_net, exc_net.size() * sizeof(uint16_t));
+  if (e == hipSuccess) e = hipMalloc((void**)&m->d_pbase, n_networks * sizeof(int64_t));
+  if (e == hipSuccess) e = hipMalloc((void**)&m->d_ebase, n_networks * sizeof(int64_t));
+  if (e == hipSuccess) e = hipMalloc((void**)&m->d_keys, 2 * ne * sizeof(uint64_t));
+  if (e == hipSuccess) e = hipMalloc((void**)&m->d_ids, 2 * ne * sizeof(uint32_t));
+  if (e == hipSuccess)
+    e = hipcub::DeviceRadixSort::SortPairs(nullptr, m->sort_bytes, m->d_keys, m->d_keys + ne,
+                                           m->d_ids, m->d_ids + ne, (int)ne, 0, m->end_bit,
+                                           m->stream);
+  if (e == hipSuccess) e = hipMalloc(&m->d_sort_tmp, std::max<size_t>(m->sort_bytes, 1));
+  if (e == hipSuccess) e = hipMemcpyAsync(m->d_wave_net, wave_net.data(), wave_net.size() * 2, hipMemcpyHostToDevice, m->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(m->d_exc_net, exc_net.data(), exc_net.size() * 2, hipMemcpyHostToDevice, m->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(m->d_pbase, pbase.data(), n_networks * 8, hipMemcpyHostToDevice, m->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(m->d_ebase, ebase.data(), n_networks * 8, hipMemcpyHostToDevice, m->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+  if (e != hipSuccess) {
+    kmc_ensemble_destroy(m);
+    return efail(KMC_ERR_HIP, std::string("kmc_ensemble_create: ") + hipGetErrorString(e));
+  }
+  *out = m;
+  return KMC_OK;
+}
+
+int kmc_ensemble_set_hop_schedule(kmc_ensemble* m, int64_t resort_hops) {
+  if (!m || resort_hops < 0) return efail(KMC_ERR_ARG, "kmc_ensemble_set_hop_schedule: bad argument");
+  m->resort_hops = resort_hops;
+  return KMC_OK;
+}
+
+int kmc_ensemble_run(kmc_ensemble* m, int64_t hops_per_exciton, double t_max, int32_t count_sites,
+                     double* kernel_ms) {
+  if (!m || hops_per_exciton < 0 || std::isnan(t_max))
+    return efail(KMC_ERR_ARG, "kmc_ensemble_run: bad argument");
+  ENS_HIP(hipSetDevice(m->device));
+  bool sample = false;
+  for (kmc_ctx* c : m->ctxs) sample = sample || c->n_bins > 0;
+  int s = ens_upload(m, [](size_t, net_view&) {});
+  if (s != KMC_OK) return s;
+  ENS_HIP(hipEventRecord(m->ev0, m->stream));
+  s = ens_run_hops(m, hops_per_exciton, m->resort_hops, count_sites != 0, sample, t_max);
+  if (s != KMC_OK) return s;
+  ENS_HIP(hipEventRecord(m->ev1, m->stream));
+  ENS_HIP(hipStreamSynchronize(m->stream));
+  if (kernel_ms) {
+    float ms = 0.f;
+    ENS_HIP(hipEventElapsedTime(&ms, m->ev0, m->ev1));
+    *kernel_ms = ms;
+  }
+  return KMC_OK;
+}
+
+int kmc_ensemble_autotune(kmc_ensemble* m, int64_t trial_hops, int32_t n_cand, const int64_t* cand,
+                          int64_t* best, double* hops_per_s) {
+  if (!m || trial_hops < 1 || n_cand < 1 || !cand)
+    return efail(KMC_ERR_ARG, "kmc_ensemble_autotune: bad argument");
+  for (int i = 0; i < n_cand; ++i)
+    if (cand[i] < 0) return efail(KMC_ERR_ARG, "kmc_ensemble_autotune: negative candidate");
+  ENS_HIP(hipSetDevice(m->device));
+  const size_t n = m->ctxs.size();
+  std::vector<state_copy> bak(n);
+  std::vector<uint32_t*> counts(n, nullptr);
+  auto release = [&]() {
+    for (size_t k = 0; k < n; ++k) {
+      if (bak[k].buf) (void)hipFree(bak[k].buf);
+      if (counts[k]) (void)hipFree(counts[k]);
+    }
+  };
+  int st = KMC_OK;
+  for (size_t k = 0; k < n && st == KMC_OK; ++k) {
+    kmc_ctx* c = m->ctxs[k];
+    st = save_state(c, bak[k], false);
+    if (st == KMC_OK && hipMalloc((void**)&counts[k], (size_t)KMC_COUNT_COPIES * c->count_stride * 4) != hipSuccess)
+      st = KMC_ERR_ALLOC;
+    if (st == KMC_OK && hipStreamSynchronize(c->stream) != hipSuccess) st = KMC_ERR_HIP;
+  }
+  if (st == KMC_OK)
+    st = ens_upload(m, [&](size_t k, net_view& v) {
+      v.n_bins = 0;
+      v.site_count = counts[k];
+    });
+  int64_t pick = cand[0];
+  double best_rate = -1.0;
+  for (int i = 0; i < n_cand && st == KMC_OK; ++i) {
+    for (size_t k = 0; k < n && st == KMC_OK; ++k) st = save_state(m->ctxs[k], bak[k], true);
+    for (size_t k = 0; k < n && st == KMC_OK; ++k)
+      if (hipStreamSynchronize(m->ctxs[k]->stream) != hipSuccess) st = KMC_ERR_HIP;
+    float ms = 0.f;
+    if (st == KMC_OK && hipEventRecord(m->ev0, m->stream) != hipSuccess) st = KMC_ERR_HIP;
+    if (st == KMC_OK) st = ens_run_hops(m, trial_hops, cand[i], true, false, INFINITY);
+    if (st == KMC_OK && (hipEventRecord(m->ev1, m->stream) != hipSuccess ||
+                         hipEventSynchronize(m->ev1) != hipSuccess ||
+                         hipEventElapsedTime(&ms, m->ev0, m->ev1) != hipSuccess))
+      st = KMC_ERR_HIP;
+    const double rate = (double)m->n_total * (double)trial_hops / (ms * 1e-3);
+    if (hops_per_s) hops_per_s[i] = rate;
+    if (st == KMC_OK && rate > best_rate) {
+      best_rate = rate;
+      pick = cand[i];
+    }
+  }
+  for (size_t k = 0; k < n && st == KMC_OK; ++k) st = save_state(m->ctxs[k], bak[k], true);
+  for (size_t k = 0; k < n && st == KMC_OK; ++k)
+    if (hipStreamSynchronize(m->ctxs[k]->stream) != hipSuccess) st = KMC_ERR_HIP;
+  release();
+  if (st != KMC_OK) return efail(st, "kmc_ensemble_autotune: a trial failed");
+  m->resort_hops = pick;
+  if (best) *best = pick;
   return KMC_OK;
 }
 

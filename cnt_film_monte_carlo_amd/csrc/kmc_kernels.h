@@ -37,16 +37,16 @@ struct exciton_arrays {
   uint8_t* heading;
 };
 
-struct hop_args {
-  exciton_arrays x;
+// Everything the hop loop needs of one network (one context).
+struct net_view {
+  exciton_arrays x;         // the network's excitons, indexed by exciton id
   const uint4* blocks;      // row blocks: 128-byte lines of ROW_UNITS 16-byte units
   const int64_t* roff;      // first line of every site's block
   const uint32_t* src_roff; // first lines of the source sites' blocks
   uint32_t n_src;
   uint64_t seed;
-  uint64_t hop_budget;
-  double t_max;
-  // MSD records [n_bins][E]
+  double per_len[3];        // period per axis (0 = open) for the minimum-image displacement
+  // MSD records [n_bins][x.n]
   int32_t n_bins;
   double dt_sample;
   double *rec_dx, *rec_dy, *rec_dz;
@@ -54,13 +54,32 @@ struct hop_args {
   // per-site departure counts: KMC_COUNT_COPIES arrays, count_stride apart
   uint32_t* site_count;
   int64_t count_stride;
-  // trace [n_trace][tr_max]
-  int64_t n_trace, tr_max;
-  int32_t* tr_site;
-  double* tr_time;
-  int64_t* tr_n;
-  double per_len[3];        // period per axis (0 = open) for the minimum-image displacement
+};
+
+struct trace_view {         // trace [n_trace][tr_max] (single network only)
+  int64_t n_trace = 0, tr_max = 0;
+  int32_t* tr_site = nullptr;
+  double* tr_time = nullptr;
+  int64_t* tr_n = nullptr;
+};
+
+struct hop_args {
+  net_view v;
+  uint64_t hop_budget;
+  double t_max;
   const uint32_t* perm;     // exciton processed at each position (NULL = identity)
+  trace_view tv;
+};
+
+struct ens_args {
+  const net_view* nets;     // [n_networks]
+  const uint16_t* wave_net; // network of every 64-position wave
+  const int64_t* pbase;     // first position of each network (a multiple of 64)
+  const int64_t* ebase;     // first ensemble-wide exciton number of each network
+  const uint32_t* perm;     // ensemble-wide exciton numbers in sorted order
+  uint64_t n_positions;
+  uint64_t hop_budget;
+  double t_max;
 };
 
 struct reduce_args {
@@ -114,6 +133,9 @@ hipError_t launch_pair_scatter(int64_t nnz, const uint8_t* cls, const uint8_t* m
 hipError_t launch_mask_count(int64_t nnz, const uint8_t* cls, const uint8_t* mask, uint32_t* counts,
                              hipStream_t st);
 hipError_t launch_hop(const hop_args& a, bool count, bool sample, bool trace, hipStream_t st);
+hipError_t launch_hop_ens(const ens_args& a, bool count, bool sample, hipStream_t st);
+hipError_t launch_ens_keys(int64_t n, const net_view* nets, const uint16_t* exc_net,
+                           const int64_t* ebase, uint64_t* keys, uint32_t* ids, hipStream_t st);
 hipError_t launch_reduce(const reduce_args& a, hipStream_t st);
 
 }  // namespace cntkmc

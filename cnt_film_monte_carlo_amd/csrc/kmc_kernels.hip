@@ -530,100 +530,164 @@ __global__ __launch_bounds__(256) void k_inject(exciton_arrays x, const uint4* _
   x.heading[e] = (uint8_t)(r31(w.v[2]) & 1u);
 }
 
-// One thread per exciton; each thread runs its exciton through its whole hop budget
-// (event-driven form of particle::step, src/monte_carlo/particle.cpp:57-80, with v = 0).
-// One thread per exciton.  Workgroups are dealt to the XCDs in turn, so workgroup b takes the
-// b%8-th eighth of the (sorted) exciton order: each XCD's L2 serves excitons that sit on
-// neighbouring blocks.  The exciton itself is perm[position] (its id keys every draw, so the
-// order changes nothing but locality).
+// One exciton through its hop budget (event-driven form of particle::step,
+// src/monte_carlo/particle.cpp:57-80, with v = 0).  v holds everything of the exciton's network;
+// it is uniform across the wave, so its fields live in scalar registers.
 template <bool COUNT, bool SAMPLE, bool TRACE>
-__global__ __launch_bounds__(256, 4) void k_hop(hop_args a) {
+__device__ __forceinline__ void hop_exciton(const net_view& v, uint32_t e, uint64_t hop_budget,
+                                            double t_max, const trace_view& tv) {
 #pragma clang fp contract(off)
-  const uint32_t per_xcd = gridDim.x / KMC_COUNT_COPIES;
-  const uint32_t chunk = (blockIdx.x % KMC_COUNT_COPIES) * per_xcd + blockIdx.x / KMC_COUNT_COPIES;
-  const uint32_t pos = chunk * blockDim.x + threadIdx.x;
-  if (pos >= a.x.n) return;
-  const uint32_t e = a.perm ? a.perm[pos] : pos;
-  double t = a.x.t_next[e];
-  uint64_t h = a.x.hops[e];
-  double dx = a.x.dx[e], dy = a.x.dy[e], dz = a.x.dz[e];
-  uint32_t nd = a.x.drains[e];
-  uint64_t hs = a.x.hash[e];
-  uint32_t k = SAMPLE ? a.x.k_rec[e] : 0u;
+  double t = v.x.t_next[e];
+  uint64_t h = v.x.hops[e];
+  double dx = v.x.dx[e], dy = v.x.dy[e], dz = v.x.dz[e];
+  uint32_t nd = v.x.drains[e];
+  uint64_t hs = v.x.hash[e];
+  uint32_t k = SAMPLE ? v.x.k_rec[e] : 0u;
   int64_t tr = 0;
-  if (TRACE && e < a.n_trace) tr = a.tr_n[e];
-  uint32_t R = a.roff[a.x.site[e]];
-  row_regs row = load_row(a.blocks, R);
-  const uint64_t h_end = h + a.hop_budget;
-  const size_t E = a.x.n;
+  if (TRACE && e < tv.n_trace) tr = tv.tr_n[e];
+  uint32_t R = v.roff[v.x.site[e]];
+  row_regs row = load_row(v.blocks, R);
+  const uint64_t h_end = h + hop_budget;
+  const size_t E = v.x.n;
   count_cache cc;
-  uint32_t* cbase = COUNT ? a.site_count + (size_t)xcd_id() * a.count_stride : nullptr;
+  uint32_t* cbase = COUNT ? v.site_count + (size_t)xcd_id() * v.count_stride : nullptr;
 
   while (true) {
     if (SAMPLE) {
-      while (k < (uint32_t)a.n_bins) {
-        const double tk = (double)(k + 1u) * a.dt_sample;
+      while (k < (uint32_t)v.n_bins) {
+        const double tk = (double)(k + 1u) * v.dt_sample;
         if (!(tk < t)) break;
         const size_t r = (size_t)k * E + e;
-        a.rec_dx[r] = dx;
-        a.rec_dy[r] = dy;
-        a.rec_dz[r] = dz;
-        a.rec_drain[r] = nd;
+        v.rec_dx[r] = dx;
+        v.rec_dy[r] = dy;
+        v.rec_dz[r] = dz;
+        v.rec_drain[r] = nd;
         ++k;
       }
     }
-    if (h >= h_end || trapped(row.n, row.gamma) || !(t <= a.t_max)) break;
+    if (h >= h_end || trapped(row.n, row.gamma) || !(t <= t_max)) break;
 
     // event selection (scatterer::update_state, scatterer.cpp:17-30)
-    const u32x4 w = draw4(a.seed, e, h, STREAM_HOP);
+    const u32x4 w = draw4(v.seed, e, h, STREAM_HOP);
     const double dice = (row.gamma * (double)r31(w.v[0])) / RAND_MAX_D;
-    const uint32_t idx = select_entry(a.blocks, R, row, dice);
-    const uint32_t tg = target_of(a.blocks, R, row, idx);
+    const uint32_t idx = select_entry(v.blocks, R, row, dice);
+    const uint32_t tg = target_of(v.blocks, R, row, idx);
     if (COUNT) cc.add(row.site, cbase);
     const float sx = row.x, sy = row.y, sz = row.z;
     const double t_hop = t;
 
     // dwell at the new site (particle.cpp:183 -> ff_time)
     R = tg & EDGE_COL_MASK;
-    row = load_row(a.blocks, R);
-    dx = dx + min_image((double)row.x - (double)sx, a.per_len[0]);
-    dy = dy + min_image((double)row.y - (double)sy, a.per_len[1]);
-    dz = dz + min_image((double)row.z - (double)sz, a.per_len[2]);
+    row = load_row(v.blocks, R);
+    dx = dx + min_image((double)row.x - (double)sx, v.per_len[0]);
+    dy = dy + min_image((double)row.y - (double)sy, v.per_len[1]);
+    dz = dz + min_image((double)row.z - (double)sz, v.per_len[2]);
     hs = (hs ^ (uint64_t)row.site) * FNV_PRIME;
-    const uint32_t r = nonzero_draw(w.v[1], w.v[3], a.seed, e, h, STREAM_HOP_REDRAW);
+    const uint32_t r = nonzero_draw(w.v[1], w.v[3], v.seed, e, h, STREAM_HOP_REDRAW);
     t = t + dwell_time(row.n, row.gamma, r);
     // drain: count current and re-inject, keeping displacement and dwell (kubo_step,
     // src/monte_carlo/monte_carlo.cpp:330-336)
     if (tg & EDGE_DRAIN_BIT) {
       ++nd;
-      R = a.src_roff[r31(w.v[2]) % a.n_src];
-      row = load_row(a.blocks, R);
+      R = v.src_roff[r31(w.v[2]) % v.n_src];
+      row = load_row(v.blocks, R);
       hs = (hs ^ ((uint64_t)row.site | (1ull << 32))) * FNV_PRIME;
     }
-    if (TRACE && e < a.n_trace && tr < a.tr_max) {
-      a.tr_site[(size_t)e * a.tr_max + tr] = (int32_t)row.site;
-      a.tr_time[(size_t)e * a.tr_max + tr] = t_hop;
+    if (TRACE && e < tv.n_trace && tr < tv.tr_max) {
+      tv.tr_site[(size_t)e * tv.tr_max + tr] = (int32_t)row.site;
+      tv.tr_time[(size_t)e * tv.tr_max + tr] = t_hop;
       ++tr;
     }
     ++h;
   }
   if (COUNT) cc.flush(cbase);
 
-  a.x.site[e] = (int32_t)row.site;
-  a.x.t_next[e] = t;
-  a.x.hops[e] = h;
-  a.x.dx[e] = dx;
-  a.x.dy[e] = dy;
-  a.x.dz[e] = dz;
-  a.x.drains[e] = nd;
-  a.x.hash[e] = hs;
-  if (SAMPLE) a.x.k_rec[e] = k;
-  if (TRACE && e < a.n_trace) a.tr_n[e] = tr;
+  v.x.site[e] = (int32_t)row.site;
+  v.x.t_next[e] = t;
+  v.x.hops[e] = h;
+  v.x.dx[e] = dx;
+  v.x.dy[e] = dy;
+  v.x.dz[e] = dz;
+  v.x.drains[e] = nd;
+  v.x.hash[e] = hs;
+  if (SAMPLE) v.x.k_rec[e] = k;
+  if (TRACE && e < tv.n_trace) tv.tr_n[e] = tr;
+}
+
+// Position of this thread in the sorted exciton order.  Workgroups are dealt to the XCDs in turn,
+// so workgroup b takes the b%8-th eighth of the order: each XCD's L2 serves excitons that sit on
+// neighbouring blocks.  gridDim.x is a multiple of KMC_COUNT_COPIES.
+__device__ __forceinline__ uint32_t xcd_position() {
+  const uint32_t per_xcd = gridDim.x / KMC_COUNT_COPIES;
+  const uint32_t chunk = (blockIdx.x % KMC_COUNT_COPIES) * per_xcd + blockIdx.x / KMC_COUNT_COPIES;
+  return chunk * blockDim.x + threadIdx.x;
+}
+
+// One network: thread -> position -> exciton perm[position] (its id keys every draw, so the order
+// changes nothing but locality).
+template <bool COUNT, bool SAMPLE, bool TRACE>
+__global__ __launch_bounds__(256, 4) void k_hop(hop_args a) {
+  const uint32_t pos = xcd_position();
+  if (pos >= a.v.x.n) return;
+  const uint32_t e = a.perm ? a.perm[pos] : pos;
+  hop_exciton<COUNT, SAMPLE, TRACE>(a.v, e, a.hop_budget, a.t_max, a.tv);
+}
+
+// Several networks in one launch.  Network k's sorted excitons occupy positions
+// [pbase[k], pbase[k] + n_k) of a position space whose network ranges start on wave boundaries, so
+// every wave belongs to one network and reads that network's view with scalar loads; sorted
+// entries of network k sit at [ebase[k], ebase[k] + n_k) of perm, which holds ensemble-wide
+// exciton numbers ebase[k] + local id.
+template <bool COUNT, bool SAMPLE>
+__global__ __launch_bounds__(256, 4) void k_hop_ens(ens_args a) {
+  const uint32_t pos = xcd_position();
+  if (pos >= a.n_positions) return;
+  const uint32_t net = __builtin_amdgcn_readfirstlane(a.wave_net[pos >> 6]);
+  const net_view v = a.nets[net];
+  const int64_t q = (int64_t)pos - a.pbase[net];
+  if (q >= v.x.n) return;
+  const int64_t eb = a.ebase[net];
+  const uint32_t e = (uint32_t)((int64_t)a.perm[eb + q] - eb);
+  hop_exciton<COUNT, SAMPLE, false>(v, e, a.hop_budget, a.t_max, trace_view{});
+}
+
+hipError_t launch_hop_ens(const ens_args& a, bool count, bool sample, hipStream_t st) {
+  const uint64_t groups = (a.n_positions + 255) / 256;
+  const dim3 grid((unsigned)((groups + KMC_COUNT_COPIES - 1) / KMC_COUNT_COPIES * KMC_COUNT_COPIES)),
+      block(256);
+  if (count && sample) hipLaunchKernelGGL((k_hop_ens<true, true>), grid, block, 0, st, a);
+  else if (count) hipLaunchKernelGGL((k_hop_ens<true, false>), grid, block, 0, st, a);
+  else if (sample) hipLaunchKernelGGL((k_hop_ens<false, true>), grid, block, 0, st, a);
+  else hipLaunchKernelGGL((k_hop_ens<false, false>), grid, block, 0, st, a);
+  return hipGetLastError();
+}
+
+// Sort keys of an ensemble's excitons: (network << 32) | first line of the current site's block.
+__global__ __launch_bounds__(256) void k_ens_keys(int64_t n, const net_view* __restrict__ nets,
+                                                  const uint16_t* __restrict__ exc_net,
+                                                  const int64_t* __restrict__ ebase,
+                                                  uint64_t* __restrict__ keys,
+                                                  uint32_t* __restrict__ ids) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  const uint32_t k = exc_net[g];
+  const net_view& v = nets[k];
+  const int64_t e = g - ebase[k];
+  keys[g] = ((uint64_t)k << 32) | (uint64_t)(uint32_t)v.roff[v.x.site[e]];
+  ids[g] = (uint32_t)g;
+}
+
+hipError_t launch_ens_keys(int64_t n, const net_view* nets, const uint16_t* exc_net,
+                           const int64_t* ebase, uint64_t* keys, uint32_t* ids, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_ens_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, nets,
+                     exc_net, ebase, keys, ids);
+  return hipGetLastError();
 }
 
 hipError_t launch_hop(const hop_args& a, bool count, bool sample, bool trace, hipStream_t st) {
   // a multiple of KMC_COUNT_COPIES workgroups, so the per-XCD chunks tile the excitons exactly
-  const uint64_t groups = (a.x.n + 255) / 256;
+  const uint64_t groups = (a.v.x.n + 255) / 256;
   const dim3 grid((unsigned)((groups + KMC_COUNT_COPIES - 1) / KMC_COUNT_COPIES * KMC_COUNT_COPIES)),
       block(256);
 #define KMC_HOP_CASE(C, S, T)                                        \

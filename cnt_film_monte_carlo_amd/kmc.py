@@ -102,6 +102,13 @@ _SIGS = {
     "kmc_linspace": (C.c_int, [C.c_double, C.c_double, C.c_int32, _D]),
     "kmc_build_rates_exact": (C.c_int, [_P, C.c_int32, C.POINTER(_P), _D]),
     "kmc_set_hop_schedule": (C.c_int, [_P, C.c_int64]),
+    "kmc_autotune_schedule": (C.c_int, [_P, C.c_int64, C.c_int32, _I64, _I64, _D]),
+    "kmc_ensemble_create": (C.c_int, [C.c_int32, C.POINTER(_P), C.POINTER(_P)]),
+    "kmc_ensemble_destroy": (C.c_int, [_P]),
+    "kmc_ensemble_last_error": (C.c_char_p, []),
+    "kmc_ensemble_set_hop_schedule": (C.c_int, [_P, C.c_int64]),
+    "kmc_ensemble_run": (C.c_int, [_P, C.c_int64, C.c_double, C.c_int32, _D]),
+    "kmc_ensemble_autotune": (C.c_int, [_P, C.c_int64, C.c_int32, _I64, _I64, _D]),
     "kmc_build_rates_mlp": (C.c_int, [_P, _P, C.c_int32, _I32, C.c_double, C.POINTER(_P), _I64, _D]),
     "kmc_pair_classes": (C.c_int, [_P, C.c_int32, _I64, C.POINTER(_P)]),
     "kmc_get_pair_rates": (C.c_int, [_P, _D]),
@@ -355,6 +362,17 @@ class Context:
         """Hops per sub-launch of run() between exciton sorts (0: one launch)."""
         self._ck(self.lib.kmc_set_hop_schedule(self.h, int(resort_hops)))
 
+    def autotune_schedule(self, trial_hops: int = 1250,
+                          candidates=(0, 2500, 1250, 625, 312, 156)):
+        """Time each schedule on the current state (state restored), install the fastest.
+        Returns (best, hops/s per candidate)."""
+        c = np.ascontiguousarray(candidates, np.int64)
+        rates = np.zeros(c.size)
+        best = C.c_int64(0)
+        self._ck(self.lib.kmc_autotune_schedule(self.h, int(trial_hops), c.size, _ptr(c, _I64),
+                                                C.byref(best), _ptr(rates, _D)))
+        return best.value, rates
+
     def sync(self):
         self._ck(self.lib.kmc_sync(self.h))
 
@@ -395,6 +413,53 @@ class Context:
         ms = C.c_double(0.0)
         self._ck(self.lib.kmc_reduce_samples(self.h, int(n_batches), _ptr(out, _D), C.byref(ms)))
         return out, ms.value
+
+
+class Ensemble:
+    """Several finished contexts (networks) run by one hop launch (kmc_ensemble)."""
+
+    def __init__(self, contexts):
+        self.lib = library()
+        self.contexts = list(contexts)       # keep the members alive
+        arr = (_P * len(self.contexts))(*[c.h for c in self.contexts])
+        h = _P()
+        rc = self.lib.kmc_ensemble_create(len(self.contexts), arr, C.byref(h))
+        if rc < 0:
+            raise KmcError(rc, self.lib.kmc_ensemble_last_error().decode())
+        self.h = h
+
+    def _ck(self, rc):
+        if rc < 0:
+            raise KmcError(rc, self.lib.kmc_ensemble_last_error().decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.kmc_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_hop_schedule(self, resort_hops: int):
+        self._ck(self.lib.kmc_ensemble_set_hop_schedule(self.h, int(resort_hops)))
+
+    def run(self, hops_per_exciton: int, t_max: float = math.inf, count_sites: bool = True) -> float:
+        ms = C.c_double(0.0)
+        self._ck(self.lib.kmc_ensemble_run(self.h, int(hops_per_exciton), float(t_max),
+                                           int(count_sites), C.byref(ms)))
+        return ms.value
+
+    def autotune_schedule(self, trial_hops: int = 1250,
+                          candidates=(0, 2500, 1250, 625, 312, 156)):
+        c = np.ascontiguousarray(candidates, np.int64)
+        rates = np.zeros(c.size)
+        best = C.c_int64(0)
+        self._ck(self.lib.kmc_ensemble_autotune(self.h, int(trial_hops), c.size, _ptr(c, _I64),
+                                                C.byref(best), _ptr(rates, _D)))
+        return best.value, rates
 
 
 # ---------------------------------------------------------------------------------------------

@@ -84,3 +84,27 @@ def make_context(w: Workload, net: dict, seed: int = 7, device: int = 0) -> kmc.
     ctx.set_sites(net["x"], net["y"], net["z"], net["ox"], net["oy"], net["oz"], net["left"],
                   net["right"], net["chirality"], net["flags"])
     return ctx
+
+
+def ensemble(n_networks: int = 256, seed: int = 3003, n_min: int = 5_000, n_max: int = 200_000,
+             t_range=(150.0, 400.0)) -> list:
+    """Independent networks by the rules of film_small with site counts log-uniform in
+    [n_min, n_max]: 200 nm (7,5) tubes of 20 sites at the density of film_small (the film's x
+    extent grows with the tube count, y stays 500 nm periodic, 50 nm thick), excitons = 0.2 x
+    sites, temperatures uniform in t_range.  Returns Workloads with `temperature` set and the
+    network seed distinct per member."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n_networks):
+        n_sites = float(np.exp(rng.uniform(np.log(n_min), np.log(n_max))))
+        n_tubes = max(1, int(round(n_sites / 20)))
+        lx = 500 * NM * n_tubes / 250
+        net = kmc.NetworkParams(n_tubes=n_tubes, len_min=200 * NM, len_max=200 * NM,
+                                box_lo=(0.0, 0.0, 0.0), box_hi=(lx, 500 * NM, 50 * NM),
+                                site_spacing=10 * NM, n_chir=1, wrap_mask=kmc.PERIODIC_Y,
+                                seed=seed * 1000 + k)
+        out.append(Workload(f"ensemble_{k}", net, cutoff=15 * NM, x_source=0.0, x_drain=lx,
+                            period_y=(0.0, 500 * NM), n_excitons=max(1, int(0.2 * 20 * n_tubes)),
+                            hops_per_exciton=10_000,
+                            temperature=float(rng.uniform(*t_range))))
+    return out

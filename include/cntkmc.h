@@ -144,6 +144,27 @@ int kmc_run(kmc_ctx* ctx, int64_t hops_per_exciton, double t_max, int32_t count_
  * budget runs as sub-launches of at most resort_hops hops, each preceded by a sort of the
  * excitons by the memory block of their current site; 0 = one launch (default 625). */
 int kmc_set_hop_schedule(kmc_ctx* ctx, int64_t resort_hops);
+/* Autotuner of that schedule (no reference counterpart): runs trial_hops hops from the current
+ * state once per candidate resort_hops value (counts into scratch, no samples), restores the
+ * state, installs the fastest candidate.  best and hops_per_s[n_cand] may be NULL. */
+int kmc_autotune_schedule(kmc_ctx* ctx, int64_t trial_hops, int32_t n_candidates,
+                          const int64_t* candidates, int64_t* best, double* hops_per_s);
+
+/* Ensembles (no reference counterpart): several independent networks, each a finished context
+ * (CSR built, excitons injected, optional MSD grid; no trace), run by one hop kernel launch per
+ * sub-launch.  Every network keeps its own seed, exciton ids, sample grid and count arrays, so its
+ * trajectories are bit-identical to kmc_run on that context alone; results stay in the member
+ * contexts (read them with kmc_get_excitons, kmc_reduce_samples, ...).  Members must not be
+ * re-injected or rebuilt while the ensemble exists.  Errors: kmc_ensemble_last_error. */
+typedef struct kmc_ensemble kmc_ensemble;
+int kmc_ensemble_create(int32_t n_networks, kmc_ctx* const* networks, kmc_ensemble** out);
+int kmc_ensemble_destroy(kmc_ensemble* ens);
+const char* kmc_ensemble_last_error(void);
+int kmc_ensemble_set_hop_schedule(kmc_ensemble* ens, int64_t resort_hops);
+int kmc_ensemble_run(kmc_ensemble* ens, int64_t hops_per_exciton, double t_max,
+                     int32_t count_sites, double* kernel_ms);
+int kmc_ensemble_autotune(kmc_ensemble* ens, int64_t trial_hops, int32_t n_candidates,
+                          const int64_t* candidates, int64_t* best, double* hops_per_s);
 
 /* Per-exciton state read-back: current site, time of the next hop, hops done, displacement
  * [E][3], drain (current) count, FNV-1a hash of the visited-site sequence.  NULL = skip. */
