@@ -170,6 +170,103 @@ def exact_table_run(w, net, tabs, E, args, t_sample_max, D):
     return Dx, cix, d_rel
 
 
+def ensemble_leg(sur, n_networks=256, hops=10_000, plain_subset=16, bins=16):
+    """256 networks of uneven size (film_5k_sites rules, 5k-200k sites, excitons = 0.2 x sites,
+    T ~ U[150, 400] K), per-pair surrogate rates at each network's T, one hop launch per
+    sub-launch for all of them (schedule from the autotuner); per-network plain runs of a subset
+    with the same seeds for the comparison."""
+    import numpy as np
+
+    from cnt_film_monte_carlo_amd import exciton as X
+    from cnt_film_monte_carlo_amd import kmc
+    from cnt_film_monte_carlo_amd import networks as nw
+
+    t0 = time.perf_counter()
+    ws = nw.ensemble(n_networks)
+    model = X.CntModel(7, 5, 10)
+    nets, ctxs, trs = [], [], []
+    build_ms = 0.0
+    fallback = 0
+    for k, w in enumerate(ws):
+        net = nw.build_network(w)
+        tr = X.Transfer(model, model, w.temperature)
+        tr.set_clamp(sur.domain.box_lo, sur.domain.box_hi)
+        ctx = nw.make_context(w, net, seed=5000 + k)
+        ctx.build_cells()
+        ctx.build_rates(kmc.RATE_PAIR)
+        ms, nfb = ctx.build_rates_mlp(sur, [0], w.temperature, [tr])
+        build_ms += ms
+        fallback += nfb
+        nets.append(net)
+        ctxs.append(ctx)
+        trs.append(tr)
+    sites = sum(c.n_sites for c in ctxs)
+    excitons = sum(w.n_excitons for w in ws)
+    log(f"ensemble: {n_networks} networks, {sites} sites, {excitons} excitons built in "
+        f"{time.perf_counter() - t0:.1f} s (surrogate pair rates {build_ms / 1e3:.2f} s, "
+        f"{fallback} pairs exact)")
+    # pilot: the MSD window of every network (its earliest exciton clock after the hop budget)
+    for c, w in zip(ctxs, ws):
+        c.inject(w.n_excitons)
+    ens = kmc.Ensemble(ctxs)
+    ens.run(hops, count_sites=False)
+    t_s = []
+    for c in ctxs:
+        tn = c.get_excitons()["t_next"]
+        t_s.append(0.8 * float(np.min(tn[np.isfinite(tn)])))
+    ens.close()
+    for c, w, ts in zip(ctxs, ws, t_s):
+        c.inject(w.n_excitons)
+        c.set_samples(bins, ts)
+    ens = kmc.Ensemble(ctxs)
+    best, rates = ens.autotune_schedule(trial_hops=1250)
+    log(f"ensemble autotune: resort_hops {best}; candidates hops/s {np.round(rates / 1e9, 1)} e9")
+    t1 = time.perf_counter()
+    ms = ens.run(hops)
+    wall = time.perf_counter() - t1
+    hops_done = sum(int(c.get_excitons()["hops"].sum()) for c in ctxs)
+    log(f"ensemble run: {hops_done:.3e} hops in {wall:.3f} s ({hops_done / wall:.3e} hops/s)")
+    ens.close()
+    # plain runs of a subset, same seeds
+    sub = list(range(0, n_networks, max(1, n_networks // plain_subset)))[:plain_subset]
+    plain_hops, plain_s, d_diff = 0, 0.0, 0.0
+    for k in sub:
+        w, ts = ws[k], t_s[k]
+        c = nw.make_context(w, nets[k], seed=5000 + k)
+        c.build_cells()
+        c.build_rates(kmc.RATE_PAIR)
+        c.build_rates_mlp(sur, [0], w.temperature, [trs[k]])
+        c.inject(w.n_excitons)
+        c.set_samples(bins, ts)
+        c.set_hop_schedule(best)
+        c.sync()
+        tp = time.perf_counter()
+        c.run(hops)
+        c.sync()
+        plain_s += time.perf_counter() - tp
+        plain_hops += int(c.get_excitons()["hops"].sum())
+        De, _ = kmc.estimate_diffusion(ctxs[k].reduce_samples(8)[0], ts, bins // 4, bins)
+        Dp, _ = kmc.estimate_diffusion(c.reduce_samples(8)[0], ts, bins // 4, bins)
+        d_diff = max(d_diff, float(np.max(np.abs(De - Dp))))
+        same = np.array_equal(ctxs[k].get_excitons()["hash"], c.get_excitons()["hash"])
+        if not same:
+            d_diff = float("inf")
+        c.close()
+    log(f"plain runs of {len(sub)} networks: {plain_hops / plain_s:.3e} hops/s; "
+        f"max |D_ensemble - D_plain| = {d_diff}")
+    for c in ctxs:
+        c.close()
+    return {"networks": n_networks, "sites": sites, "excitons": excitons,
+            "hops_per_exciton": hops, "hops": hops_done, "seconds": wall,
+            "hops_per_s": hops_done / wall, "kernel_ms": ms, "resort_hops_autotuned": best,
+            "autotune_candidates_hops_per_s": rates.tolist(),
+            "rate_build_s": build_ms / 1e3, "pairs_exact_fallback": fallback,
+            "plain_subset_networks": len(sub), "plain_subset_hops_per_s": plain_hops / plain_s,
+            "max_abs_D_diff_vs_plain": d_diff,
+            "note": "ensemble members are bit-identical to per-network runs (same seeds); "
+                    "D compared on the plain-run subset"}
+
+
 def surrogate_roofline(sur, n=1 << 22, reps=5):
     """Fused BF16 inference kernel on n pair geometries: algorithmic FLOP / kernel time."""
     import numpy as np
@@ -231,6 +328,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-exact", action="store_true",
                     help="skip the exact table build and the exact-table comparison run")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="skip the 256-network ensemble leg")
     ap.add_argument("--bins", type=int, default=64)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes (counter calibration)")
@@ -324,6 +423,7 @@ def main():
     if tabs["exact"] is not None:
         Dx, cix, d_rel = exact_table_run(w, net, tabs, E, args, t_sample_max, D)
     sur_roof = surrogate_roofline(tabs["surrogate"])
+    ens = None if args.no_ensemble else ensemble_leg(tabs["surrogate"])
 
 
     traffic = None
@@ -401,6 +501,7 @@ def main():
             "hops_per_launch": hops_per_launch,
         },
         "roofline_surrogate": sur_roof,
+        "ensemble": ens,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
