@@ -87,6 +87,11 @@ struct kmc_ctx {
   dbuf<uint32_t> d_cls_count;
   dbuf<int64_t> d_cls_start, d_sel;
   dbuf<uint8_t> d_mlp_flag;  // surrogate flags of the last surrogate build
+  // drift along the tubes (particle::fly): velocity [m/s], links as block lines, error word
+  double velocity = 0.0;      // set by kmc_set_velocity
+  double exc_velocity = 0.0;  // the velocity of the injected excitons (fixed by kmc_inject)
+  dbuf<uint2> d_links;
+  dbuf<int32_t> d_fly_err;
 
   // excitons
   int64_t n_exc = 0;
@@ -95,6 +100,7 @@ struct kmc_ctx {
   dbuf<uint64_t> x_hops, x_hash;
   dbuf<uint32_t> x_drains, x_krec;
   dbuf<uint8_t> x_heading;
+  dbuf<double> x_soff, x_tlast;
   dbuf<uint32_t> d_site_count;  // KMC_COUNT_COPIES copies, count_stride apart
   dbuf<uint64_t> d_counts64;
   int64_t count_stride = 0;
@@ -194,6 +200,8 @@ exciton_arrays excitons_of(kmc_ctx* c) {
   x.hash = c->x_hash.p;
   x.k_rec = c->x_krec.p;
   x.heading = c->x_heading.p;
+  x.s_off = c->x_soff.p;
+  x.t_last = c->x_tlast.p;
   return x;
 }
 
@@ -216,6 +224,9 @@ net_view view_of(kmc_ctx* c) {
   v.rec_drain = c->r_drain.p;
   v.site_count = c->d_site_count.p;
   v.count_stride = c->count_stride;
+  v.velocity = c->exc_velocity;
+  v.links = c->d_links.p;
+  v.fly_err = c->d_fly_err.p;
   return v;
 }
 
@@ -272,6 +283,9 @@ int pack_rows(kmc_ctx* c) {
   KMC_TRY(dalloc(c, c->d_src_roff, c->h_src.size()));
   KMC_HIP(c, launch_gather_offsets((int64_t)c->h_src.size(), c->d_src.p, c->d_roff.p,
                                    c->d_src_roff.p, c->stream));
+  KMC_TRY(dalloc(c, c->d_links, n));
+  KMC_HIP(c, launch_links(n, c->d_left.p, c->d_right.p, c->d_flags.p, c->d_roff.p, c->d_links.p,
+                          c->stream));
   KMC_HIP(c, hipStreamSynchronize(c->stream));
   return KMC_OK;
 }
@@ -280,6 +294,33 @@ int check_err_flag(kmc_ctx* c, const char* what) {
   int32_t h = 0;
   KMC_TRY(copy_out(c, &h, c->d_err.p, sizeof(int32_t)));
   if (h != 0) return fail(c, KMC_ERR_RATE, std::string(what) + ": negative, NaN or infinite rate");
+  return KMC_OK;
+}
+
+// Every same-tube link must have a nonzero length in the FP32 positions a flight uses (a flight
+// over a zero-length segment would pass sites without using time).
+int check_links(kmc_ctx* c, const char* what) {
+  double per[3];
+  for (int k = 0; k < 3; ++k)
+    per[k] = (c->cfg.periodic_mask & (1 << k)) ? c->cfg.period_len[k] : 0.0;
+  for (int64_t i = 0; i < c->n_sites; ++i) {
+    const int32_t nb[2] = {c->h_left[i], c->h_right[i]};
+    for (int32_t j : nb) {
+      if (j < 0) continue;
+      double d[3];
+      for (int k = 0; k < 3; ++k) {
+        d[k] = (double)(float)c->h_pos[k][j] - (double)(float)c->h_pos[k][i];
+        if (per[k] > 0.0) {
+          if (d[k] > 0.5 * per[k]) d[k] -= per[k];
+          else if (d[k] < -0.5 * per[k]) d[k] += per[k];
+        }
+      }
+      if (!(norm3(d[0], d[1], d[2]) > 0.0))
+        return fail(c, KMC_ERR_GEOMETRY, std::string(what) + ": linked sites " +
+                                             std::to_string(i) + " and " + std::to_string(j) +
+                                             " coincide");
+    }
+  }
   return KMC_OK;
 }
 
@@ -350,7 +391,8 @@ int kmc_destroy(kmc_ctx* c) {
   dfree(c->d_sel); dfree(c->d_mlp_flag);
   dfree(c->d_err); dfree(c->x_site); dfree(c->x_t); dfree(c->x_dx); dfree(c->x_dy);
   dfree(c->x_dz); dfree(c->x_hops); dfree(c->x_hash); dfree(c->x_drains); dfree(c->x_krec);
-  dfree(c->x_heading); dfree(c->d_site_count); dfree(c->r_dx); dfree(c->r_dy); dfree(c->r_dz);
+  dfree(c->x_heading); dfree(c->x_soff); dfree(c->x_tlast); dfree(c->d_links); dfree(c->d_fly_err);
+  dfree(c->d_site_count); dfree(c->r_dx); dfree(c->r_dy); dfree(c->r_dz);
   dfree(c->d_reduced); dfree(c->r_drain); dfree(c->tr_site); dfree(c->tr_time); dfree(c->tr_n);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -865,6 +907,8 @@ int kmc_inject(kmc_ctx* c, int64_t n_excitons) {
   KMC_TRY(dalloc(c, c->x_drains, E));
   KMC_TRY(dalloc(c, c->x_krec, E));
   KMC_TRY(dalloc(c, c->x_heading, E));
+  KMC_TRY(dalloc(c, c->x_soff, E));
+  KMC_TRY(dalloc(c, c->x_tlast, E));
   c->count_stride = (c->n_sites + COUNT_ALIGN - 1) / COUNT_ALIGN * COUNT_ALIGN;
   KMC_TRY(dalloc(c, c->d_site_count, (size_t)KMC_COUNT_COPIES * c->count_stride));
   KMC_HIP(c, hipMemsetAsync(c->d_site_count.p, 0, c->d_site_count.n * sizeof(uint32_t),
@@ -872,6 +916,8 @@ int kmc_inject(kmc_ctx* c, int64_t n_excitons) {
   c->n_exc = E;
   c->n_bins = 0;
   c->n_trace = 0;
+  if (c->velocity > 0.0) KMC_TRY(check_links(c, "kmc_inject"));
+  c->exc_velocity = c->velocity;
   KMC_HIP(c, launch_inject(excitons_of(c), c->d_blocks.p, c->d_src_roff.p,
                            (uint32_t)c->h_src.size(), c->cfg.seed, c->stream));
   KMC_HIP(c, hipStreamSynchronize(c->stream));
@@ -930,6 +976,8 @@ int kmc_set_trace(kmc_ctx* c, int64_t n_trace, int64_t max_hops) {
 static int run_hops(kmc_ctx* c, hop_args a, int64_t hops, int64_t chunk, bool count, bool sample,
                     bool trace) {
   const int64_t E = c->n_exc;
+  const bool fly = c->exc_velocity > 0.0;
+  if (fly) KMC_HIP(c, hipMemsetAsync(c->d_fly_err.p, 0, sizeof(int32_t), c->stream));
   KMC_TRY(dalloc(c, c->d_keys, 2 * E));
   KMC_TRY(dalloc(c, c->d_ids, 2 * E));
   size_t need = 0;
@@ -952,15 +1000,47 @@ static int run_hops(kmc_ctx* c, hop_args a, int64_t hops, int64_t chunk, bool co
                                                   c->d_keys.p + E, c->d_ids.p, c->d_ids.p + E,
                                                   (int)E, 0, 32, c->stream));
     a.hop_budget = (uint64_t)sub;
-    KMC_HIP(c, launch_hop(a, count, sample, trace, c->stream));
+    KMC_HIP(c, launch_hop(a, count, sample, trace, fly, c->stream));
     done += sub;
   } while (done < hops);
+  if (fly) {
+    int32_t flag = 0;
+    KMC_TRY(copy_out(c, &flag, c->d_fly_err.p, sizeof(int32_t)));
+    if (flag)
+      return fail(c, KMC_ERR_STATE, "kmc_run: a flight passed more than 2^22 sites "
+                                    "(velocity x dwell far beyond the tube lengths)");
+  }
   return KMC_OK;
 }
 
 int kmc_set_hop_schedule(kmc_ctx* c, int64_t resort_hops) {
   if (!c || resort_hops < 0) return fail(c, KMC_ERR_ARG, "kmc_set_hop_schedule: bad argument");
   c->resort_hops = resort_hops;
+  return KMC_OK;
+}
+
+int kmc_set_velocity(kmc_ctx* c, double velocity) {
+  if (!c) return fail(nullptr, KMC_ERR_ARG, "kmc_set_velocity: null context");
+  if (!(velocity >= 0.0) || !std::isfinite(velocity))
+    return fail(c, KMC_ERR_ARG, "kmc_set_velocity: need 0 <= velocity < inf");
+  if (velocity > 0.0) {
+    if (c->n_sites == 0) return fail(c, KMC_ERR_STATE, "kmc_set_velocity: call kmc_set_sites first");
+    KMC_TRY(check_links(c, "kmc_set_velocity"));
+    KMC_HIP(c, hipSetDevice(c->cfg.device));
+    if (!c->d_fly_err.p) KMC_TRY(dalloc(c, c->d_fly_err, 1));
+  }
+  c->velocity = velocity;
+  return KMC_OK;
+}
+
+int kmc_get_flight(kmc_ctx* c, double* offset, uint8_t* heading, double* t_last) {
+  if (!c) return fail(nullptr, KMC_ERR_ARG, "kmc_get_flight: null context");
+  if (c->n_exc == 0) return fail(c, KMC_ERR_STATE, "kmc_get_flight: no excitons");
+  KMC_HIP(c, hipSetDevice(c->cfg.device));
+  const int64_t E = c->n_exc;
+  if (offset) KMC_TRY(copy_out(c, offset, c->x_soff.p, E * sizeof(double)));
+  if (heading) KMC_TRY(copy_out(c, heading, c->x_heading.p, E));
+  if (t_last) KMC_TRY(copy_out(c, t_last, c->x_tlast.p, E * sizeof(double)));
   return KMC_OK;
 }
 
@@ -1093,7 +1173,7 @@ namespace {
 struct state_copy {
   uint8_t* buf = nullptr;
   int64_t n = 0;
-  static constexpr size_t per = 4 + 8 + 8 + 24 + 4 + 8 + 4 + 1;  // bytes of state per exciton
+  static constexpr size_t per = 4 + 8 + 8 + 24 + 4 + 8 + 4 + 1 + 16;  // bytes of state per exciton
 };
 
 int save_state(kmc_ctx* c, state_copy& s, bool restore) {
@@ -1105,7 +1185,8 @@ int save_state(kmc_ctx* c, state_copy& s, bool restore) {
   struct part { void* p; size_t b; };
   const part parts[] = {{c->x_site.p, 4}, {c->x_t.p, 8},      {c->x_hops.p, 8}, {c->x_dx.p, 8},
                         {c->x_dy.p, 8},   {c->x_dz.p, 8},     {c->x_drains.p, 4}, {c->x_hash.p, 8},
-                        {c->x_krec.p, 4}, {c->x_heading.p, 1}};
+                        {c->x_krec.p, 4}, {c->x_heading.p, 1}, {c->x_soff.p, 8},
+                        {c->x_tlast.p, 8}};
   size_t off = 0;
   for (const part& q : parts) {
     void* dev = q.p;
@@ -1161,6 +1242,9 @@ int ens_upload(kmc_ensemble* m, F modify) {
       return efail(KMC_ERR_STATE, "kmc_ensemble: a member was re-injected or rebuilt since "
                                   "kmc_ensemble_create");
     if (c->n_trace > 0) return efail(KMC_ERR_STATE, "kmc_ensemble: members may not record traces");
+    if (c->exc_velocity > 0.0)
+      return efail(KMC_ERR_STATE, "kmc_ensemble: members must have velocity 0 (run drifting "
+                                  "networks with kmc_run)");
     ENS_HIP(hipStreamSynchronize(c->stream));
     m->views[k] = view_of(c);
     modify(k, m->views[k]);

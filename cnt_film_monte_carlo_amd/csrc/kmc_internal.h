@@ -17,6 +17,9 @@ namespace cntkmc {
 constexpr double RAND_MAX_D = 2147483647.0;  // glibc RAND_MAX, the divisor of scatterer.cpp:17
 constexpr uint32_t EDGE_DRAIN_BIT = 0x80000000u;
 constexpr uint32_t ROW_UNITS = 8;  // 16-byte units per 128-byte line of the hop kernel's blocks
+// Same-tube links of the drift path: no link (tube end) and the bound on site passes per flight.
+constexpr uint32_t LINK_NONE = 0xFFFFFFFFu;
+constexpr uint32_t FLY_MAX_PASSES = 1u << 22;
 constexpr uint32_t EDGE_COL_MASK = 0x7FFFFFFFu;
 constexpr uint64_t FNV_OFFSET = 0xcbf29ce484222325ull;
 constexpr uint64_t FNV_PRIME = 0x100000001b3ull;

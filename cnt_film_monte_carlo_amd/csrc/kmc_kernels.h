@@ -34,7 +34,9 @@ struct exciton_arrays {
   uint32_t* drains;
   uint64_t* hash;
   uint32_t* k_rec;  // MSD records written so far
-  uint8_t* heading;
+  uint8_t* heading; // 1 = drifting toward the `right` link (particle.h:35)
+  double* s_off;    // v > 0: distance travelled beyond the current site toward the next one
+  double* t_last;   // v > 0: time of the last hop or injection (the flight starts there)
 };
 
 // Everything the hop loop needs of one network (one context).
@@ -54,6 +56,10 @@ struct net_view {
   // per-site departure counts: KMC_COUNT_COPIES arrays, count_stride apart
   uint32_t* site_count;
   int64_t count_stride;
+  // drift along the tubes (particle::fly), used when velocity > 0
+  double velocity;
+  const uint2* links;       // per site: {left, right} block line | drain bit, or LINK_NONE
+  int32_t* fly_err;         // set when a flight exceeds FLY_MAX_PASSES site passes
 };
 
 struct trace_view {         // trace [n_trace][tr_max] (single network only)
@@ -132,7 +138,11 @@ hipError_t launch_pair_scatter(int64_t nnz, const uint8_t* cls, const uint8_t* m
 // counts[cls[p]] += 1 for every entry with mask[p] != 0
 hipError_t launch_mask_count(int64_t nnz, const uint8_t* cls, const uint8_t* mask, uint32_t* counts,
                              hipStream_t st);
-hipError_t launch_hop(const hop_args& a, bool count, bool sample, bool trace, hipStream_t st);
+hipError_t launch_hop(const hop_args& a, bool count, bool sample, bool trace, bool fly,
+                      hipStream_t st);
+// links[i] = {line of left | drain bit, line of right | drain bit}, LINK_NONE for a tube end
+hipError_t launch_links(int64_t n, const int32_t* left, const int32_t* right, const uint8_t* flags,
+                        const int64_t* roff, uint2* links, hipStream_t st);
 hipError_t launch_hop_ens(const ens_args& a, bool count, bool sample, hipStream_t st);
 hipError_t launch_ens_keys(int64_t n, const net_view* nets, const uint16_t* exc_net,
                            const int64_t* ebase, uint64_t* keys, uint32_t* ids, hipStream_t st);

@@ -528,16 +528,144 @@ __global__ __launch_bounds__(256) void k_inject(exciton_arrays x, const uint4* _
   x.hash[e] = (FNV_OFFSET ^ (uint64_t)h.y) * FNV_PRIME;
   x.k_rec[e] = 0;
   x.heading[e] = (uint8_t)(r31(w.v[2]) & 1u);
+  x.s_off[e] = 0.0;
+  x.t_last[e] = 0.0;
+}
+
+// Same-tube links as block lines with the target's drain bit (the flight's view of
+// scatterer::left/right, scatterer.h:29-30).
+__global__ __launch_bounds__(256) void k_links(int64_t n, const int32_t* __restrict__ left,
+                                               const int32_t* __restrict__ right,
+                                               const uint8_t* __restrict__ flags,
+                                               const int64_t* __restrict__ roff,
+                                               uint2* __restrict__ links) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t l = left[i], r = right[i];
+  uint2 o;
+  o.x = l < 0 ? LINK_NONE : ((uint32_t)roff[l] | ((flags[l] & 2u) ? EDGE_DRAIN_BIT : 0u));
+  o.y = r < 0 ? LINK_NONE : ((uint32_t)roff[r] | ((flags[r] & 2u) ? EDGE_DRAIN_BIT : 0u));
+  links[i] = o;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Drift along the tube (particle::fly, src/monte_carlo/particle.cpp:9-54) for v > 0.  The
+// exciton sits s_off beyond its current site on the segment toward the next site along its
+// heading; its displacement is kept as the displacement of the current site's position (hops
+// and passed segments) plus that offset.  One flight covers the whole dwell before a hop (the
+// reference cuts it into 1 fs steps; the path is the same).
+// ---------------------------------------------------------------------------------------------
+struct flight_state {
+  uint32_t R;         // first line of the current site's block
+  uint32_t site;
+  float x, y, z;      // the current site's FP32 position
+  double s;           // distance travelled from the current site toward the next one
+  uint32_t heading;   // 1 = toward `right`
+  double dx, dy, dz;  // displacement of the current site's position
+  uint32_t nd;        // drains
+  uint64_t hs;        // visited-site hash
+};
+
+// Next site along the heading; a tube end reverses the heading (particle.cpp:18-37).
+__device__ __forceinline__ uint32_t fly_next(uint2 lk, uint32_t& heading) {
+  const uint32_t nx = heading ? (lk.y != LINK_NONE ? lk.y : lk.x) : (lk.x != LINK_NONE ? lk.x : lk.y);
+  heading = (nx == lk.y) ? 1u : 0u;
+  return nx;
+}
+
+__device__ __forceinline__ void fly_segment(const net_view& v, const flight_state& f, uint32_t Rn,
+                                            float& px, float& py, float& pz, double& sx,
+                                            double& sy, double& sz) {
+#pragma clang fp contract(off)
+  const uint4 p = v.blocks[(size_t)ROW_UNITS * Rn + 7];
+  px = __uint_as_float(p.x);
+  py = __uint_as_float(p.y);
+  pz = __uint_as_float(p.z);
+  sx = min_image((double)px - (double)f.x, v.per_len[0]);
+  sy = min_image((double)py - (double)f.y, v.per_len[1]);
+  sz = min_image((double)pz - (double)f.z, v.per_len[2]);
+}
+
+// Flight of duration tau.  A site reached on the way becomes the current site; a drain site
+// counts a drain and re-injects on a source site (draw of stream STREAM_FLY_DRAIN + j for the
+// j-th drain of this flight, hop counter h), keeping the displacement, and the flight goes on
+// from there, as the reference's next 1 fs step would.  False when the flight would pass more
+// than FLY_MAX_PASSES sites (the state is then left where the bound stopped it).
+__device__ bool fly(const net_view& v, flight_state& f, double tau, uint32_t e, uint64_t h) {
+#pragma clang fp contract(off)
+  uint2 lk = v.links[f.site];
+  if (lk.x == LINK_NONE && lk.y == LINK_NONE) return true;  // particle.cpp:11-12
+  uint32_t j = 0;
+  for (uint32_t pass = 0; pass < FLY_MAX_PASSES; ++pass) {
+    const uint32_t nx = fly_next(lk, f.heading);
+    const uint32_t Rn = nx & EDGE_COL_MASK;
+    float px, py, pz;
+    double sx, sy, sz;
+    fly_segment(v, f, Rn, px, py, pz, sx, sy, sz);
+    const double len = norm3(sx, sy, sz);
+    const double dist = len - f.s;
+    if (!(dist / v.velocity < tau)) {  // stops on this segment (particle.cpp:46-49)
+      f.s = f.s + v.velocity * tau;
+      return true;
+    }
+    tau = tau - dist / v.velocity;
+    f.dx = f.dx + sx;
+    f.dy = f.dy + sy;
+    f.dz = f.dz + sz;
+    f.s = 0.0;
+    f.R = Rn;
+    f.site = v.blocks[(size_t)ROW_UNITS * Rn].y;
+    f.x = px;
+    f.y = py;
+    f.z = pz;
+    f.hs = (f.hs ^ ((uint64_t)f.site | (2ull << 32))) * FNV_PRIME;
+    if (nx & EDGE_DRAIN_BIT) {
+      ++f.nd;
+      const u32x4 w = draw4(v.seed, e, h, STREAM_FLY_DRAIN + j);
+      ++j;
+      f.R = v.src_roff[r31(w.v[0]) % v.n_src];
+      const uint4 hd = v.blocks[(size_t)ROW_UNITS * f.R];
+      const uint4 p = v.blocks[(size_t)ROW_UNITS * f.R + 7];
+      f.site = hd.y;
+      f.x = __uint_as_float(p.x);
+      f.y = __uint_as_float(p.y);
+      f.z = __uint_as_float(p.z);
+      f.hs = (f.hs ^ ((uint64_t)f.site | (1ull << 32))) * FNV_PRIME;
+    }
+    lk = v.links[f.site];
+    if (lk.x == LINK_NONE && lk.y == LINK_NONE) return true;
+  }
+  return false;
+}
+
+// Position of the exciton relative to its current site: s along the segment toward the next one.
+__device__ __forceinline__ void fly_offset(const net_view& v, const flight_state& f, double& ox,
+                                           double& oy, double& oz) {
+#pragma clang fp contract(off)
+  ox = oy = oz = 0.0;
+  if (!(f.s != 0.0)) return;
+  uint32_t hd = f.heading;
+  const uint32_t Rn = fly_next(v.links[f.site], hd) & EDGE_COL_MASK;
+  float px, py, pz;
+  double sx, sy, sz;
+  fly_segment(v, f, Rn, px, py, pz, sx, sy, sz);
+  const double q = f.s / norm3(sx, sy, sz);
+  ox = sx * q;
+  oy = sy * q;
+  oz = sz * q;
 }
 
 // One exciton through its hop budget (event-driven form of particle::step,
-// src/monte_carlo/particle.cpp:57-80, with v = 0).  v holds everything of the exciton's network;
+// src/monte_carlo/particle.cpp:57-80; FLY adds the drift of particle::fly for v > 0).  v holds everything of the exciton's network;
 // it is uniform across the wave, so its fields live in scalar registers.
-template <bool COUNT, bool SAMPLE, bool TRACE>
+template <bool COUNT, bool SAMPLE, bool TRACE, bool FLY = false>
 __device__ __forceinline__ void hop_exciton(const net_view& v, uint32_t e, uint64_t hop_budget,
                                             double t_max, const trace_view& tv) {
 #pragma clang fp contract(off)
   double t = v.x.t_next[e];
+  double tl = FLY ? v.x.t_last[e] : 0.0;
+  double so = FLY ? v.x.s_off[e] : 0.0;
+  uint32_t hd = FLY ? (uint32_t)v.x.heading[e] : 0u;
   uint64_t h = v.x.hops[e];
   double dx = v.x.dx[e], dy = v.x.dy[e], dz = v.x.dz[e];
   uint32_t nd = v.x.drains[e];
@@ -558,14 +686,57 @@ __device__ __forceinline__ void hop_exciton(const net_view& v, uint32_t e, uint6
         const double tk = (double)(k + 1u) * v.dt_sample;
         if (!(tk < t)) break;
         const size_t r = (size_t)k * E + e;
-        v.rec_dx[r] = dx;
-        v.rec_dy[r] = dy;
-        v.rec_dz[r] = dz;
-        v.rec_drain[r] = nd;
+        if (FLY) {  // the flight from the last event up to tk
+          flight_state g{R, row.site, row.x, row.y, row.z, so, hd, dx, dy, dz, nd, hs};
+          if (!fly(v, g, tk - tl, e, h)) {
+            atomicOr(v.fly_err, 1);
+            break;
+          }
+          double ox, oy, oz;
+          fly_offset(v, g, ox, oy, oz);
+          v.rec_dx[r] = g.dx + ox;
+          v.rec_dy[r] = g.dy + oy;
+          v.rec_dz[r] = g.dz + oz;
+          v.rec_drain[r] = g.nd;
+        } else {
+          v.rec_dx[r] = dx;
+          v.rec_dy[r] = dy;
+          v.rec_dz[r] = dz;
+          v.rec_drain[r] = nd;
+        }
         ++k;
       }
     }
-    if (h >= h_end || trapped(row.n, row.gamma) || !(t <= t_max)) break;
+    if (FLY) {
+      if (h >= h_end || !(t <= t_max) || !(t < INFINITY)) break;
+      // fly through the dwell, then hop from the site reached (particle.cpp:62-75)
+      flight_state g{R, row.site, row.x, row.y, row.z, so, hd, dx, dy, dz, nd, hs};
+      if (!fly(v, g, t - tl, e, h)) {
+        atomicOr(v.fly_err, 1);
+        break;
+      }
+      if (g.R != R) {
+        R = g.R;
+        row = load_row(v.blocks, R);
+      }
+      so = g.s;
+      hd = g.heading;
+      dx = g.dx;
+      dy = g.dy;
+      dz = g.dz;
+      nd = g.nd;
+      hs = g.hs;
+      if (trapped(row.n, row.gamma)) {
+        // no rates here: the event leaves the exciton in place (scatterer.cpp:14-15) and no
+        // dwell ends again; it counts as a hop so that later flights draw on fresh counters
+        tl = t;
+        t = INFINITY;
+        ++h;
+        continue;
+      }
+    } else if (h >= h_end || trapped(row.n, row.gamma) || !(t <= t_max)) {
+      break;
+    }
 
     // event selection (scatterer::update_state, scatterer.cpp:17-30)
     const u32x4 w = draw4(v.seed, e, h, STREAM_HOP);
@@ -593,6 +764,10 @@ __device__ __forceinline__ void hop_exciton(const net_view& v, uint32_t e, uint6
       row = load_row(v.blocks, R);
       hs = (hs ^ ((uint64_t)row.site | (1ull << 32))) * FNV_PRIME;
     }
+    if (FLY) {
+      so = 0.0;
+      tl = t_hop;
+    }
     if (TRACE && e < tv.n_trace && tr < tv.tr_max) {
       tv.tr_site[(size_t)e * tv.tr_max + tr] = (int32_t)row.site;
       tv.tr_time[(size_t)e * tv.tr_max + tr] = t_hop;
@@ -612,6 +787,11 @@ __device__ __forceinline__ void hop_exciton(const net_view& v, uint32_t e, uint6
   v.x.hash[e] = hs;
   if (SAMPLE) v.x.k_rec[e] = k;
   if (TRACE && e < tv.n_trace) tv.tr_n[e] = tr;
+  if (FLY) {
+    v.x.s_off[e] = so;
+    v.x.t_last[e] = tl;
+    v.x.heading[e] = (uint8_t)hd;
+  }
 }
 
 // Position of this thread in the sorted exciton order.  Workgroups are dealt to the XCDs in turn,
@@ -631,6 +811,15 @@ __global__ __launch_bounds__(256, 4) void k_hop(hop_args a) {
   if (pos >= a.v.x.n) return;
   const uint32_t e = a.perm ? a.perm[pos] : pos;
   hop_exciton<COUNT, SAMPLE, TRACE>(a.v, e, a.hop_budget, a.t_max, a.tv);
+}
+
+// The same with drift along the tubes (v > 0).
+template <bool COUNT, bool SAMPLE, bool TRACE>
+__global__ __launch_bounds__(256) void k_hop_fly(hop_args a) {
+  const uint32_t pos = xcd_position();
+  if (pos >= a.v.x.n) return;
+  const uint32_t e = a.perm ? a.perm[pos] : pos;
+  hop_exciton<COUNT, SAMPLE, TRACE, true>(a.v, e, a.hop_budget, a.t_max, a.tv);
 }
 
 // Several networks in one launch.  Network k's sorted excitons occupy positions
@@ -685,15 +874,17 @@ hipError_t launch_ens_keys(int64_t n, const net_view* nets, const uint16_t* exc_
   return hipGetLastError();
 }
 
-hipError_t launch_hop(const hop_args& a, bool count, bool sample, bool trace, hipStream_t st) {
+hipError_t launch_hop(const hop_args& a, bool count, bool sample, bool trace, bool fly,
+                      hipStream_t st) {
   // a multiple of KMC_COUNT_COPIES workgroups, so the per-XCD chunks tile the excitons exactly
   const uint64_t groups = (a.v.x.n + 255) / 256;
   const dim3 grid((unsigned)((groups + KMC_COUNT_COPIES - 1) / KMC_COUNT_COPIES * KMC_COUNT_COPIES)),
       block(256);
-#define KMC_HOP_CASE(C, S, T)                                        \
-  if (count == C && sample == S && trace == T) {                     \
-    hipLaunchKernelGGL((k_hop<C, S, T>), grid, block, 0, st, a);     \
-    return hipGetLastError();                                        \
+#define KMC_HOP_CASE(C, S, T)                                            \
+  if (count == C && sample == S && trace == T) {                         \
+    if (fly) hipLaunchKernelGGL((k_hop_fly<C, S, T>), grid, block, 0, st, a); \
+    else hipLaunchKernelGGL((k_hop<C, S, T>), grid, block, 0, st, a);    \
+    return hipGetLastError();                                            \
   }
   KMC_HOP_CASE(false, false, false)
   KMC_HOP_CASE(true, false, false)
@@ -917,6 +1108,13 @@ hipError_t launch_mask_count(int64_t nnz, const uint8_t* cls, const uint8_t* mas
                              hipStream_t st) {
   if (nnz == 0) return hipSuccess;
   hipLaunchKernelGGL(k_mask_count, grid_for(nnz), dim3(256), 0, st, nnz, cls, mask, counts);
+  return hipGetLastError();
+}
+
+hipError_t launch_links(int64_t n, const int32_t* left, const int32_t* right, const uint8_t* flags,
+                        const int64_t* roff, uint2* links, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_links, grid_for(n), dim3(256), 0, st, n, left, right, flags, roff, links);
   return hipGetLastError();
 }
 

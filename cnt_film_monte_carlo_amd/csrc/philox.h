@@ -48,6 +48,7 @@ enum : uint32_t {
   STREAM_INIT = 1u,           // injection: w0 site, w1 dwell, w2 heading, w3 dwell spare
   STREAM_HOP_REDRAW = 2u,     // + k: further dwell candidates after two zero words
   STREAM_INIT_REDRAW = 0x80000000u,
+  STREAM_FLY_DRAIN = 0x40000000u,  // + j: re-injection site after the j-th drain of one flight
   REDRAW_LIMIT = 16u,         // bounded redraw loop; after it the draw is taken as 1
 };
 

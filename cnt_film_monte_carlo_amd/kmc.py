@@ -102,6 +102,8 @@ _SIGS = {
     "kmc_linspace": (C.c_int, [C.c_double, C.c_double, C.c_int32, _D]),
     "kmc_build_rates_exact": (C.c_int, [_P, C.c_int32, C.POINTER(_P), _D]),
     "kmc_set_hop_schedule": (C.c_int, [_P, C.c_int64]),
+    "kmc_set_velocity": (C.c_int, [_P, C.c_double]),
+    "kmc_get_flight": (C.c_int, [_P, _D, _U8, _D]),
     "kmc_autotune_schedule": (C.c_int, [_P, C.c_int64, C.c_int32, _I64, _I64, _D]),
     "kmc_ensemble_create": (C.c_int, [C.c_int32, C.POINTER(_P), C.POINTER(_P)]),
     "kmc_ensemble_destroy": (C.c_int, [_P]),
@@ -228,6 +230,8 @@ class Context:
         self.n_sites = n
         # FP32 positions: the hop kernel's displacements are differences of these
         self.site_pos = np.stack(arrs[:3], 1).astype(np.float32)
+        self.links = (np.full(n, -1, np.int32) if left is None else left.copy(),
+                      np.full(n, -1, np.int32) if right is None else right.copy())
 
     def build_cells(self):
         self._ck(self.lib.kmc_build_cells(self.h))
@@ -375,6 +379,21 @@ class Context:
 
     def sync(self):
         self._ck(self.lib.kmc_sync(self.h))
+
+    def set_velocity(self, velocity: float):
+        """Drift speed along the tubes [m/s] for the excitons of the next inject() (0: hopping
+        only)."""
+        self._ck(self.lib.kmc_set_velocity(self.h, float(velocity)))
+
+    def get_flight(self):
+        """Flight state (v > 0): offset beyond the current site along the heading, heading,
+        time of the last hop or injection."""
+        E = self.n_exc
+        off = np.zeros(E)
+        hd = np.zeros(E, np.uint8)
+        tl = np.zeros(E)
+        self._ck(self.lib.kmc_get_flight(self.h, _ptr(off, _D), _ptr(hd, _U8), _ptr(tl, _D)))
+        return dict(offset=off, heading=hd, t_last=tl)
 
     def get_excitons(self):
         E = self.n_exc

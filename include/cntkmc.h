@@ -144,6 +144,21 @@ int kmc_run(kmc_ctx* ctx, int64_t hops_per_exciton, double t_max, int32_t count_
  * budget runs as sub-launches of at most resort_hops hops, each preceded by a sort of the
  * excitons by the memory block of their current site; 0 = one launch (default 625). */
 int kmc_set_hop_schedule(kmc_ctx* ctx, int64_t resort_hops);
+/* Exciton velocity along its tube [m/s] (particle::fly, src/monte_carlo/particle.cpp:9-54;
+ * "exciton velocity [m/s]", src/monte_carlo/monte_carlo.h:161).  0 (default): pure hopping.
+ * v > 0: during every dwell the exciton drifts at speed v along the left/right links of its
+ * tube (FP32 site positions, minimum image on periodic axes), reversing at a tube end; a site it
+ * reaches becomes its current site, whose rates the next hop uses; a drain site reached in
+ * flight counts a drain and re-injects (the flight goes on from the source site).  An isolated
+ * site (no links) does not move the exciton.  Each flight is bounded at 2^22 site passes
+ * (kmc_run fails beyond).  Linked sites must not coincide (KMC_ERR_GEOMETRY).  Set it after
+ * kmc_set_sites; it applies to the excitons of the next kmc_inject.  Ensembles need velocity 0. */
+int kmc_set_velocity(kmc_ctx* ctx, double velocity);
+/* Flight state (v > 0): distance beyond the current site toward the next site along the
+ * heading, heading (1 = toward `right`), time of the last hop or injection.  The exciton's
+ * displacement is the kmc_get_excitons displacement (that of its current site) plus the offset
+ * along that segment.  NULL = skip. */
+int kmc_get_flight(kmc_ctx* ctx, double* offset, uint8_t* heading, double* t_last);
 /* Autotuner of that schedule (no reference counterpart): runs trial_hops hops from the current
  * state once per candidate resort_hops value (counts into scratch, no samples), restores the
  * state, installs the fastest candidate.  best and hops_per_s[n_cand] may be NULL. */

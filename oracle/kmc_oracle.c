@@ -545,6 +545,20 @@ int orc_inject(int64_t n_sites, const int64_t* row_ptr, const double* cum, int64
   return 0;
 }
 
+/* Flight state at injection (particle.h:50-51: heading = rand() % 2, drawn after the dwell):
+ * word 2 of the injection draw; offset 0, last event at t = 0. */
+int orc_inject_flight(uint64_t seed, int64_t E, uint8_t* heading, double* s_off, double* t_last) {
+  int64_t e;
+  for (e = 0; e < E; ++e) {
+    uint32_t w[4];
+    draws(seed, (uint32_t)e, 0, 1u, w);
+    heading[e] = (uint8_t)((w[2] >> 1) & 1u);
+    s_off[e] = 0.0;
+    t_last[e] = 0.0;
+  }
+  return 0;
+}
+
 typedef struct orc_run_args {
   /* CSR */
   int64_t n_sites;
@@ -584,7 +598,28 @@ typedef struct orc_run_args {
   const orc_net* net;
   int n_threads;
   double per_len[3]; /* period per axis (0 = open): displacements take the minimum image */
+  /* drift along the tubes (particle::fly) when velocity > 0 */
+  double velocity;
+  const int32_t* left; /* [n_sites] same-tube links, -1 = tube end */
+  const int32_t* right;
+  uint8_t* heading;    /* [E] 1 = toward right */
+  double* s_off;       /* [E] distance beyond the current site toward the next one */
+  double* t_last;      /* [E] time of the last hop or injection */
+  int32_t* fly_err;    /* set to 1 when a flight passes more than ORC_FLY_MAX_PASSES sites */
 } orc_run_args;
+
+#define ORC_FLY_MAX_PASSES (1u << 22)
+
+/* The exciton between events when v > 0: current site, distance travelled from it toward the
+ * next site along the heading, displacement of the current site's position, drains, hash. */
+typedef struct orc_flight {
+  int32_t s;
+  double off;
+  unsigned heading;
+  double d[3];
+  uint32_t nd;
+  uint64_t hs;
+} orc_flight;
 
 /* One coordinate of a hop's displacement: the difference of two FP32 positions in FP64 (exact
  * for sites this close), shifted by one period on a periodic axis when it exceeds half of it. */
@@ -594,6 +629,75 @@ static double orc_min_image(double d, double period) {
     else if (d < -0.5 * period) d += period;
   }
   return d;
+}
+
+/* Next site along the heading; at a tube end the other link, reversing the heading
+ * (particle.cpp:18-37). */
+static int32_t orc_fly_next(const orc_run_args* a, int32_t s, unsigned* heading) {
+  int32_t l = a->left[s], r = a->right[s], nx;
+  if (*heading) nx = r > -1 ? r : l;
+  else nx = l > -1 ? l : r;
+  *heading = (nx == r) ? 1u : 0u;
+  return nx;
+}
+
+static void orc_fly_segment(const orc_run_args* a, int32_t s, int32_t n, double sd[3]) {
+  int k;
+  for (k = 0; k < 3; ++k)
+    sd[k] = orc_min_image((double)a->pos[3 * n + k] - (double)a->pos[3 * s + k], a->per_len[k]);
+}
+
+/* particle::fly (particle.cpp:9-54) over a whole dwell tau: pass sites while the remaining
+ * distance takes less than tau; a drain site reached counts a drain and re-injects at a source
+ * site drawn from stream 0x40000000 + j (the j-th drain of this flight) at hop counter h, and the
+ * flight goes on from there (kubo_step's removal, monte_carlo.cpp:330-336, at the moment the
+ * site is reached).  Returns 0 when the pass bound is exceeded. */
+static int orc_fly(const orc_run_args* a, orc_flight* f, double tau, uint32_t e, uint64_t h) {
+  uint32_t j = 0, pass;
+  if (a->left[f->s] < 0 && a->right[f->s] < 0) return 1; /* particle.cpp:11-12 */
+  for (pass = 0; pass < ORC_FLY_MAX_PASSES; ++pass) {
+    int32_t nx = orc_fly_next(a, f->s, &f->heading);
+    double sd[3], len, dist;
+    orc_fly_segment(a, f->s, nx, sd);
+    len = arma_norm(sd, 3);
+    dist = len - f->off;
+    if (!(dist / a->velocity < tau)) {
+      f->off = f->off + a->velocity * tau;
+      return 1;
+    }
+    tau = tau - dist / a->velocity;
+    f->d[0] = f->d[0] + sd[0];
+    f->d[1] = f->d[1] + sd[1];
+    f->d[2] = f->d[2] + sd[2];
+    f->off = 0.0;
+    f->s = nx;
+    f->hs = (f->hs ^ ((uint64_t)(uint32_t)nx | (2ull << 32))) * FNV_PR;
+    if (a->flags[nx] & 2u) {
+      uint32_t w[4];
+      ++f->nd;
+      draws(a->seed, e, h, 0x40000000u + j, w);
+      ++j;
+      f->s = a->src[(w[0] >> 1) % (uint32_t)a->n_src];
+      f->hs = (f->hs ^ ((uint64_t)(uint32_t)f->s | (1ull << 32))) * FNV_PR;
+    }
+    if (a->left[f->s] < 0 && a->right[f->s] < 0) return 1;
+  }
+  return 0;
+}
+
+/* The exciton's position relative to its current site: off along the segment to the next. */
+static void orc_fly_offset(const orc_run_args* a, const orc_flight* f, double o[3]) {
+  unsigned hd = f->heading;
+  int32_t nx;
+  double sd[3], q;
+  o[0] = o[1] = o[2] = 0.0;
+  if (!(f->off != 0.0)) return;
+  nx = orc_fly_next(a, f->s, &hd);
+  orc_fly_segment(a, f->s, nx, sd);
+  q = f->off / arma_norm(sd, 3);
+  o[0] = sd[0] * q;
+  o[1] = sd[1] * q;
+  o[2] = sd[2] * q;
 }
 
 /* One exciton through its budget: particle::step (particle.cpp:57-80) in event form,
@@ -611,6 +715,9 @@ static void orc_one(const orc_run_args* a, int64_t e, orc_cand* nb, double* rowb
   int64_t tr = (a->tr_n && e < a->n_trace) ? a->tr_n[e] : 0;
   const uint64_t h_end = h + (uint64_t)a->hop_budget;
   const double dts = a->n_bins > 0 ? a->t_sample_max / a->n_bins : 0.0;
+  const int fly = a->velocity > 0.0;
+  double tl = fly ? a->t_last[e] : 0.0, off = fly ? a->s_off[e] : 0.0;
+  unsigned hd = fly ? a->heading[e] : 0u;
 
   for (;;) {
     const double* row;
@@ -628,11 +735,39 @@ static void orc_one(const orc_run_args* a, int64_t e, orc_cand* nb, double* rowb
       size_t rr;
       if (!(tk < t)) break;
       rr = (size_t)k * a->E + e;
-      a->rec_disp[3 * rr] = dx;
-      a->rec_disp[3 * rr + 1] = dy;
-      a->rec_disp[3 * rr + 2] = dz;
-      a->rec_drain[rr] = nd;
+      if (fly) { /* the flight from the last event up to tk */
+        orc_flight g;
+        double o[3];
+        g.s = s; g.off = off; g.heading = hd; g.d[0] = dx; g.d[1] = dy; g.d[2] = dz;
+        g.nd = nd; g.hs = hs;
+        if (!orc_fly(a, &g, tk - tl, (uint32_t)e, h)) {
+          *a->fly_err = 1;
+          break;
+        }
+        orc_fly_offset(a, &g, o);
+        a->rec_disp[3 * rr] = g.d[0] + o[0];
+        a->rec_disp[3 * rr + 1] = g.d[1] + o[1];
+        a->rec_disp[3 * rr + 2] = g.d[2] + o[2];
+        a->rec_drain[rr] = g.nd;
+      } else {
+        a->rec_disp[3 * rr] = dx;
+        a->rec_disp[3 * rr + 1] = dy;
+        a->rec_disp[3 * rr + 2] = dz;
+        a->rec_drain[rr] = nd;
+      }
       ++k;
+    }
+    if (fly) { /* fly through the dwell, then hop from the site reached (particle.cpp:62-75) */
+      orc_flight g;
+      if (h >= h_end || !(t <= a->t_max) || !(t < INFINITY)) break;
+      g.s = s; g.off = off; g.heading = hd; g.d[0] = dx; g.d[1] = dy; g.d[2] = dz;
+      g.nd = nd; g.hs = hs;
+      if (!orc_fly(a, &g, t - tl, (uint32_t)e, h)) {
+        *a->fly_err = 1;
+        break;
+      }
+      s = g.s; off = g.off; hd = g.heading; dx = g.d[0]; dy = g.d[1]; dz = g.d[2];
+      nd = g.nd; hs = g.hs;
     }
     /* current site row: cached CSR or recomputed (find_neighbors on every hop, scatterer.cpp:12) */
     if (a->net) {
@@ -661,7 +796,16 @@ static void orc_one(const orc_run_args* a, int64_t e, orc_cand* nb, double* rowb
       rcol = a->col + a->row_ptr[s];
       n = a->row_ptr[s + 1] - a->row_ptr[s];
     }
-    if (h >= h_end || n == 0 || !(row[n - 1] > 0.0) || !(t <= a->t_max)) break;
+    if (fly) {
+      if (n == 0 || !(row[n - 1] > 0.0)) { /* no rates: the event leaves it in place */
+        tl = t;
+        t = INFINITY;
+        ++h;
+        continue;
+      }
+    } else if (h >= h_end || n == 0 || !(row[n - 1] > 0.0) || !(t <= a->t_max)) {
+      break;
+    }
 
     draws(a->seed, (uint32_t)e, h, 0u, w);
     dice = row[n - 1] * (double)(w[0] >> 1) / ORC_RAND_MAX;
@@ -711,6 +855,10 @@ static void orc_one(const orc_run_args* a, int64_t e, orc_cand* nb, double* rowb
       s = a->src[(w[2] >> 1) % (uint32_t)a->n_src];
       hs = (hs ^ ((uint64_t)(uint32_t)s | (1ull << 32))) * FNV_PR;
     }
+    if (fly) {
+      off = 0.0;
+      tl = t_hop;
+    }
     if (a->tr_site && e < a->n_trace && tr < a->tr_max) {
       a->tr_site[(size_t)e * a->tr_max + tr] = s;
       a->tr_time[(size_t)e * a->tr_max + tr] = t_hop;
@@ -728,6 +876,11 @@ static void orc_one(const orc_run_args* a, int64_t e, orc_cand* nb, double* rowb
   a->hash[e] = hs;
   if (a->k_rec) a->k_rec[e] = k;
   if (a->tr_n && e < a->n_trace) a->tr_n[e] = tr;
+  if (fly) {
+    a->s_off[e] = off;
+    a->t_last[e] = tl;
+    a->heading[e] = (uint8_t)hd;
+  }
 }
 
 int orc_run(const orc_run_args* a) {

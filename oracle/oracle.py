@@ -36,6 +36,8 @@ class RunArgs(C.Structure):
         ("rec_drain", _U32), ("site_count", _U64),
         ("n_trace", C.c_int64), ("tr_max", C.c_int64), ("tr_site", _I32), ("tr_time", _D),
         ("tr_n", _I64), ("net", _P), ("n_threads", C.c_int), ("per_len", C.c_double * 3),
+        ("velocity", C.c_double), ("left", _I32), ("right", _I32), ("heading", _U8),
+        ("s_off", _D), ("t_last", _D), ("fly_err", C.POINTER(C.c_int32)),
     ]
 
 
@@ -71,6 +73,7 @@ def library() -> C.CDLL:
             "orc_inject": (C.c_int, [C.c_int64, _I64, _D, C.c_int64, _I32, C.c_uint64, C.c_int64,
                                      _I32, _D, _U64, _D, _U32, _U64, _U32]),
             "orc_run": (C.c_int, [C.POINTER(RunArgs)]),
+            "orc_inject_flight": (C.c_int, [C.c_uint64, C.c_int64, _U8, _D, _D]),
             "orc_reduce_samples": (C.c_int, [C.c_int64, C.c_int32, C.c_int32, _U32, _D, _U32,
                                              _D]),
             "orc_max_threads": (C.c_int, []),
@@ -188,8 +191,10 @@ class Net:
 class Excitons:
     """Oracle exciton ensemble over a CSR (or recomputing neighbours per hop)."""
 
-    def __init__(self, csr: dict, flags: np.ndarray, seed: int, n_excitons: int):
-        """csr: row_ptr, col, cum, site_pos (FP32 [n][3]) and period (per axis, 0 = open)."""
+    def __init__(self, csr: dict, flags: np.ndarray, seed: int, n_excitons: int,
+                 velocity: float = 0.0, links=None):
+        """csr: row_ptr, col, cum, site_pos (FP32 [n][3]) and period (per axis, 0 = open).
+        velocity > 0 adds the drift along the tubes; links = (left, right) site links."""
         self.lib = library()
         self.row = _c(csr["row_ptr"], np.int64)
         self.col = _c(csr["col"], np.int32)
@@ -224,6 +229,18 @@ class Excitons:
                                  _p(self.drains, _U32), _p(self.hash, _U64), _p(self.k_rec, _U32))
         if rc != 0:
             raise RuntimeError("orc_inject: no source sites")
+        self.velocity = float(velocity)
+        self.heading = np.zeros(E, np.uint8)
+        self.s_off = np.zeros(E)
+        self.t_last = np.zeros(E)
+        self.fly_err = C.c_int32(0)
+        if self.velocity > 0.0:
+            if links is None:
+                raise ValueError("velocity > 0 needs the (left, right) site links")
+            self.left = _c(links[0], np.int32)
+            self.right = _c(links[1], np.int32)
+            self.lib.orc_inject_flight(seed, E, _p(self.heading, _U8), _p(self.s_off, _D),
+                                       _p(self.t_last, _D))
 
     def set_samples(self, n_bins: int, t_sample_max: float):
         self.n_bins, self.t_sample_max = n_bins, t_sample_max
@@ -262,7 +279,15 @@ class Excitons:
             a.tr_n = None
         a.net = net.h if net is not None else None
         a.n_threads = n_threads
+        a.velocity = self.velocity
+        if self.velocity > 0.0:
+            a.left, a.right = _p(self.left, _I32), _p(self.right, _I32)
+            a.heading, a.s_off, a.t_last = (_p(self.heading, _U8), _p(self.s_off, _D),
+                                            _p(self.t_last, _D))
+            a.fly_err = C.pointer(self.fly_err)
         self.lib.orc_run(C.byref(a))
+        if self.fly_err.value:
+            raise RuntimeError("oracle: a flight passed more than 2^22 sites")
 
     def reduce_samples(self, n_batches: int) -> np.ndarray:
         out = np.zeros((self.n_bins, n_batches, 5))
