@@ -25,6 +25,14 @@ using namespace cntkmc;
 
 namespace {
 thread_local std::string g_last_error;
+}  // namespace
+
+namespace cntkmc {
+// The message kmc_last_error(NULL) returns, for entry points without a context.
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+}  // namespace cntkmc
+
+namespace {
 
 template <typename T>
 struct dbuf {

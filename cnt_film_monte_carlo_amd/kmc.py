@@ -100,6 +100,11 @@ _SIGS = {
     "kmc_forster_table": (C.c_int, [C.c_double, C.c_int32, _D, C.c_int32, _D, C.c_int32, _D,
                                     C.c_int32, _D, _D]),
     "kmc_linspace": (C.c_int, [C.c_double, C.c_double, C.c_int32, _D]),
+    "kmc_read_mesh": (C.c_int, [C.c_char_p, C.c_int64, _I64, _I64, _D, _D, _D, _D, _D, _D, _I32,
+                                _I32]),
+    "kmc_trim_sites": (C.c_int, [_I64, _D, _D, _D, _D, _D, _D, _I32, _I32, _U8, _U8, _I64, _D, _D,
+                                 _D]),
+    "kmc_kubo_regions": (C.c_int, [C.c_int64, _D, _D, _D, C.c_int32, _U8, _D, _D]),
     "kmc_build_rates_exact": (C.c_int, [_P, C.c_int32, C.POINTER(_P), _D]),
     "kmc_set_hop_schedule": (C.c_int, [_P, C.c_int64]),
     "kmc_set_velocity": (C.c_int, [_P, C.c_double]),
@@ -528,6 +533,63 @@ def generate_network(params: NetworkParams) -> dict:
                                     _ptr(tube, _I32), C.byref(n)))
     out.update(left=left, right=right, chirality=chir, tube=tube, n=N)
     return out
+
+
+_SITE_KEYS = ("x", "y", "z", "ox", "oy", "oz")
+
+
+def read_mesh(directory) -> dict:
+    """Sites of a CNT mesh directory (six single_cnt.{pos,orient}.{x,y,z}.dat matrices, one row
+    per tube, positions in nm) in the generator's dict format, as create_scatterers builds them
+    (src/monte_carlo/monte_carlo.h:199-271)."""
+    lib = library()
+    d = str(directory).encode()
+    nt, npt = C.c_int64(), C.c_int64()
+    _check(lib.kmc_read_mesh(d, 0, C.byref(nt), C.byref(npt), *([None] * 8)))
+    N = nt.value * npt.value
+    out = {k: np.zeros(N) for k in _SITE_KEYS}
+    left = np.zeros(N, np.int32)
+    right = np.zeros(N, np.int32)
+    _check(lib.kmc_read_mesh(d, N, C.byref(nt), C.byref(npt), *[_ptr(out[k], _D) for k in _SITE_KEYS],
+                             _ptr(left, _I32), _ptr(right, _I32)))
+    out.update(left=left, right=right, chirality=np.zeros(N, np.uint8),
+               tube=np.repeat(np.arange(nt.value, dtype=np.int32), npt.value), n=N)
+    return out
+
+
+def trim_sites(net: dict, xlim, ylim, zlim) -> dict:
+    """Sites of `net` inside the box, in trim_scats order with repaired links
+    (src/monte_carlo/monte_carlo.h:722-782).  Returns a new dict; chirality, flags and tube
+    (when present) follow the permutation, source_index holds each kept site's original index."""
+    lib = library()
+    N = int(np.asarray(net["x"]).size)
+    a = {k: np.array(net[k], np.float64) for k in _SITE_KEYS}
+    left = np.array(net["left"], np.int32)
+    right = np.array(net["right"], np.int32)
+    origin = np.zeros(N, np.int64)
+    n = C.c_int64(N)
+    lim = [np.asarray(v, np.float64) for v in (xlim, ylim, zlim)]
+    _check(lib.kmc_trim_sites(C.byref(n), *[_ptr(a[k], _D) for k in _SITE_KEYS], _ptr(left, _I32),
+                              _ptr(right, _I32), None, None, _ptr(origin, _I64),
+                              *[_ptr(v, _D) for v in lim]))
+    M = n.value
+    out = {k: a[k][:M].copy() for k in _SITE_KEYS}
+    out.update(left=left[:M].copy(), right=right[:M].copy(), n=M, source_index=origin[:M].copy())
+    for k in ("chirality", "flags", "tube"):
+        if k in net and np.asarray(net[k]).size == N:
+            out[k] = np.asarray(net[k])[origin[:M]].copy()
+    return out
+
+
+def kubo_regions(net: dict, n_sections: int):
+    """KMC_SITE_SOURCE / KMC_SITE_DRAIN flags of kubo_init's injection region and removal
+    domain (src/monte_carlo/monte_carlo.cpp:203-251) and the simulation domain (lo, hi)."""
+    a = [_c(net[k], np.float64) for k in ("x", "y", "z")]
+    flags = np.zeros(a[0].size, np.uint8)
+    lo, hi = np.zeros(3), np.zeros(3)
+    _check(library().kmc_kubo_regions(a[0].size, *[_ptr(v, _D) for v in a], int(n_sections),
+                                      _ptr(flags, _U8), _ptr(lo, _D), _ptr(hi, _D)))
+    return flags, lo, hi
 
 
 def linspace(start: float, end: float, n: int) -> np.ndarray:

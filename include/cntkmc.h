@@ -227,6 +227,35 @@ int kmc_generate_network(const kmc_netgen_params* p, int64_t capacity, double* x
                          double* z, double* ox, double* oy, double* oz, int32_t* left,
                          int32_t* right, uint8_t* chirality, int32_t* tube_id, int64_t* n_out);
 
+/* ---- site ingest (host only; no device work) --------------------------------------------
+ * Mesh reader: create_scatterers (src/monte_carlo/monte_carlo.h:199-271).  dir holds the six
+ * ASCII matrices single_cnt.{pos,orient}.{x,y,z}.dat (one row per tube, one column per point,
+ * numbers separated by blanks or commas; positions in nm).  Site s = i * n_points + j of tube i,
+ * point j: position x 1e-9 m, orientation as read, links left = s - 1 (j > 0), right = s + 1
+ * (j + 1 < n_points), else -1.  Call with x == NULL (or capacity 0) for n_tubes and n_points.
+ * Errors (missing file, ragged rows, shape mismatch) read with kmc_last_error(NULL). */
+int kmc_read_mesh(const char* dir, int64_t capacity, int64_t* n_tubes, int64_t* n_points,
+                  double* x, double* y, double* z, double* ox, double* oy, double* oz,
+                  int32_t* left, int32_t* right);
+/* trim_scats (src/monte_carlo/monte_carlo.h:722-782): removes the sites outside
+ * [xlim[0], xlim[1]] x [ylim[0], ylim[1]] x [zlim[0], zlim[1]] in place, in the reference's
+ * order (a site outside swaps with the last kept site, links re-pointed; links to removed
+ * sites become -1).  *n: site count in, kept count out.  chirality and flags may be NULL;
+ * when given they follow the permutation.  origin (may be NULL) receives the original index
+ * of the site in every slot. */
+int kmc_trim_sites(int64_t* n, double* x, double* y, double* z, double* ox, double* oy,
+                   double* oz, int32_t* left, int32_t* right, uint8_t* chirality, uint8_t* flags,
+                   int64_t* origin, const double xlim[2], const double ylim[2],
+                   const double zlim[2]);
+/* Kubo regions of kubo_init (src/monte_carlo/monte_carlo.cpp:296-302): the bounding box
+ * (find_simulation_domain, monte_carlo.h:328-340) cut into n_sections (odd, >= 3) slabs per
+ * axis; sites in the centre cell (closed bounds) get KMC_SITE_SOURCE (injection_region,
+ * monte_carlo.cpp:203-228), sites outside [edge 1, edge n-1] on any axis get KMC_SITE_DRAIN
+ * (get_removal_domain, :231-251: reaching one re-injects, kubo_step :330-336).  flags is
+ * overwritten; domain_lo / domain_hi (may be NULL) receive the box. */
+int kmc_kubo_regions(int64_t n, const double* x, const double* y, const double* z,
+                     int32_t n_sections, uint8_t* flags, double domain_lo[3], double domain_hi[3]);
+
 /* Förster / Wong closed-form table (create_forster_scatt_table, src/monte_carlo/monte_carlo.cpp:
  * 156-200) on the grid given; rates is [n_theta][n_z][n_a1][n_a2].  Host only. */
 int kmc_forster_table(double gamma_0, int32_t n_theta, const double* theta, int32_t n_z,

@@ -59,3 +59,12 @@ def site_arrays(pos, ori, chir=None, flags=None, left=None, right=None):
                 left=np.full(n, -1, np.int32) if left is None else np.asarray(left, np.int32),
                 right=np.full(n, -1, np.int32) if right is None else np.asarray(right, np.int32),
                 n=n)
+
+
+def write_mesh(d, pos_nm, orient, sep=" ", fmt="%.17g"):
+    """The six single_cnt.{pos,orient}.{x,y,z}.dat matrices of a mesh directory (one row per
+    tube, positions in nm), the input of create_scatterers (src/monte_carlo/monte_carlo.h:
+    199-271)."""
+    names = ("pos.x", "pos.y", "pos.z", "orient.x", "orient.y", "orient.z")
+    for name, m in zip(names, list(pos_nm) + list(orient)):
+        np.savetxt(Path(d) / f"single_cnt.{name}.dat", np.asarray(m), fmt=fmt, delimiter=sep)
