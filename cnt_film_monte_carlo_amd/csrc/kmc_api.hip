@@ -280,8 +280,8 @@ int pack_rows(kmc_ctx* c) {
   }
   int64_t lines = 0;
   KMC_TRY(copy_out(c, &lines, c->d_roff.p + n, sizeof(int64_t)));
-  if (lines >= (int64_t)EDGE_DRAIN_BIT)
-    return fail(c, KMC_ERR_CAPACITY, "pack_rows: CSR too large for 31-bit block offsets");
+  if (lines > (int64_t)TGT_LINE_MASK)
+    return fail(c, KMC_ERR_CAPACITY, "pack_rows: CSR too large for 30-bit block offsets");
   c->n_units = lines * ROW_UNITS;
   KMC_TRY(dalloc(c, c->d_blocks, c->n_units));
   KMC_HIP(c, hipMemsetAsync(c->d_blocks.p, 0, c->n_units * sizeof(uint4), c->stream));
@@ -289,11 +289,11 @@ int pack_rows(kmc_ctx* c) {
                                 c->d_pos[0].p, c->d_pos[1].p, c->d_pos[2].p, c->d_blocks.p,
                                 c->stream));
   KMC_TRY(dalloc(c, c->d_src_roff, c->h_src.size()));
-  KMC_HIP(c, launch_gather_offsets((int64_t)c->h_src.size(), c->d_src.p, c->d_roff.p,
+  KMC_HIP(c, launch_gather_offsets((int64_t)c->h_src.size(), c->d_src.p, c->d_roff.p, c->d_hdr.p,
                                    c->d_src_roff.p, c->stream));
   KMC_TRY(dalloc(c, c->d_links, n));
-  KMC_HIP(c, launch_links(n, c->d_left.p, c->d_right.p, c->d_flags.p, c->d_roff.p, c->d_links.p,
-                          c->stream));
+  KMC_HIP(c, launch_links(n, c->d_left.p, c->d_right.p, c->d_flags.p, c->d_roff.p, c->d_hdr.p,
+                          c->d_links.p, c->stream));
   KMC_HIP(c, hipStreamSynchronize(c->stream));
   return KMC_OK;
 }

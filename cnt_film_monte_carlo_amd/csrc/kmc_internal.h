@@ -21,6 +21,10 @@ constexpr uint32_t ROW_UNITS = 8;  // 16-byte units per 128-byte line of the hop
 constexpr uint32_t LINK_NONE = 0xFFFFFFFFu;
 constexpr uint32_t FLY_MAX_PASSES = 1u << 22;
 constexpr uint32_t EDGE_COL_MASK = 0x7FFFFFFFu;
+// Row word of the hop kernel (a target, a source, a link): first line of the site's block |
+// TGT_WIDE_BIT when the block continues past its first line | EDGE_DRAIN_BIT for a drain site.
+constexpr uint32_t TGT_LINE_MASK = 0x3FFFFFFFu;
+constexpr uint32_t TGT_WIDE_BIT = 0x40000000u;
 constexpr uint64_t FNV_OFFSET = 0xcbf29ce484222325ull;
 constexpr uint64_t FNV_PRIME = 0x100000001b3ull;
 constexpr int MAX_CHIR = 16;

@@ -126,8 +126,9 @@ hipError_t launch_exciton_keys(int64_t n, const int32_t* site, const int64_t* ro
 hipError_t launch_fill_blocks(int64_t n, const site_hdr* hdr, const double* cum,
                               const edge_rec* edge, const int64_t* roff, const double* px,
                               const double* py, const double* pz, uint4* blocks, hipStream_t st);
-hipError_t launch_gather_offsets(int64_t n, const int32_t* idx, const int64_t* roff, uint32_t* out,
-                                 hipStream_t st);
+// out[i] = row word of site idx[i] (first line of its block | TGT_WIDE_BIT)
+hipError_t launch_gather_offsets(int64_t n, const int32_t* idx, const int64_t* roff,
+                                 const site_hdr* hdr, uint32_t* out, hipStream_t st);
 hipError_t launch_sum_counts(int64_t n, int64_t stride, const uint32_t* copies, uint64_t* out,
                              hipStream_t st);
 hipError_t launch_pair_classes(int64_t n, int64_t nnz, const site_hdr* hdr, const edge_rec* edge,
@@ -142,7 +143,7 @@ hipError_t launch_hop(const hop_args& a, bool count, bool sample, bool trace, bo
                       hipStream_t st);
 // links[i] = {line of left | drain bit, line of right | drain bit}, LINK_NONE for a tube end
 hipError_t launch_links(int64_t n, const int32_t* left, const int32_t* right, const uint8_t* flags,
-                        const int64_t* roff, uint2* links, hipStream_t st);
+                        const int64_t* roff, const site_hdr* hdr, uint2* links, hipStream_t st);
 hipError_t launch_hop_ens(const ens_args& a, bool count, bool sample, hipStream_t st);
 hipError_t launch_ens_keys(int64_t n, const net_view* nets, const uint16_t* exc_net,
                            const int64_t* ebase, uint64_t* keys, uint32_t* ids, hipStream_t st);

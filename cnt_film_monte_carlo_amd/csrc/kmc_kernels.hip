@@ -305,7 +305,8 @@ __device__ __forceinline__ double dwell_time(uint32_t n, double gamma, uint32_t 
 // 16-byte units:
 //   u0      n | site | Gamma (FP64)
 //   u1..u4  cum[0..8)   FP64 prefix sums (unused slots past n are zero)
-//   u5, u6  tgt[0..8)   line index of the target's block | EDGE_DRAIN_BIT for a drain target
+//   u5, u6  tgt[0..8)   row word of the target: first line of its block | TGT_WIDE_BIT when
+//                       that block has continuation lines | EDGE_DRAIN_BIT for a drain target
 //   u7      x, y, z     FP32 position of the site | 0
 // A site with n > ROW_INLINE continues on the next lines: cum[8..n) then tgt[8..n).  The
 // displacement of a hop is the difference of the two sites' FP32 positions, so the block needs
@@ -316,6 +317,11 @@ constexpr uint32_t ROW_INLINE = 8;
 __host__ __device__ __forceinline__ uint32_t block_lines(uint32_t n) {
   const uint32_t m = n > ROW_INLINE ? n - ROW_INLINE : 0u;
   return 1u + (12u * m + 127u) / 128u;
+}
+
+// Row word of a site whose block starts at line R.
+__device__ __forceinline__ uint32_t row_word(uint32_t R, uint32_t n) {
+  return R | (n > ROW_INLINE ? TGT_WIDE_BIT : 0u);
 }
 
 // Lines of the q-th block in placement order (order[q] = its site; identity when NULL).
@@ -374,7 +380,7 @@ __global__ __launch_bounds__(256) void k_pack_rows(int64_t n, const site_hdr* __
   for (uint32_t k = 0; k < h.n; ++k) {
     const edge_rec e = edge[h.rs + k];
     const uint32_t j = e.col & EDGE_COL_MASK;
-    const uint32_t tg = (uint32_t)roff[j] | (e.col & EDGE_DRAIN_BIT);
+    const uint32_t tg = row_word((uint32_t)roff[j], hdr[j].n) | (e.col & EDGE_DRAIN_BIT);
     if (k < ROW_INLINE) {
       c[k] = cum[h.rs + k];
       t[k] = tg;
@@ -389,10 +395,11 @@ __global__ __launch_bounds__(256) void k_pack_rows(int64_t n, const site_hdr* __
 
 __global__ __launch_bounds__(256) void k_gather_offsets(int64_t n, const int32_t* __restrict__ idx,
                                                         const int64_t* __restrict__ roff,
+                                                        const site_hdr* __restrict__ hdr,
                                                         uint32_t* __restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  out[i] = (uint32_t)roff[idx[i]];
+  out[i] = row_word((uint32_t)roff[idx[i]], hdr[idx[i]].n);
 }
 
 struct row_regs {
@@ -403,15 +410,13 @@ struct row_regs {
   float x, y, z;
 };
 
-// The eight 16-byte loads of one block's first line, issued together.
-__device__ __forceinline__ row_regs load_row(const uint4* __restrict__ blocks, uint32_t R) {
-  const uint4* b = blocks + (size_t)ROW_UNITS * R;
-  const uint4 h = b[0];
-  const uint4 q0 = b[1], q1 = b[2], q2 = b[3], q3 = b[4];
+// Registers of a row from the eight 16-byte units of its first line.
+__device__ __forceinline__ row_regs row_from_units(const uint4& h, const uint4& q0, const uint4& q1,
+                                                   const uint4& q2, const uint4& q3, const uint4& t0,
+                                                   const uint4& t1, const uint4& p) {
   row_regs r;
-  r.t0 = b[5];
-  r.t1 = b[6];
-  const uint4 p = b[7];
+  r.t0 = t0;
+  r.t1 = t1;
   r.n = h.x;
   r.site = h.y;
   r.gamma = __hiloint2double((int)h.w, (int)h.z);
@@ -429,31 +434,75 @@ __device__ __forceinline__ row_regs load_row(const uint4* __restrict__ blocks, u
   return r;
 }
 
+// The eight 16-byte loads of one block's first line (row word W), issued together.
+__device__ __forceinline__ row_regs load_row(const uint4* __restrict__ blocks, uint32_t W) {
+  const uint4* b = blocks + (size_t)ROW_UNITS * (W & TGT_LINE_MASK);
+  return row_from_units(b[0], b[1], b[2], b[3], b[4], b[5], b[6], b[7]);
+}
+
+// Wave-cooperative form of load_row: every lane of the wave (all 64 active) passes the row word
+// of the row it needs; the wave brings the 64 first lines into its 8 KB of LDS with eight
+// direct-to-LDS loads, each fetching eight whole lines (lane j of load k takes one 16-byte unit
+// of row 8k + j/8, so eight lanes cover one line), and each lane then reads its own row.  Load
+// k writes lane j's unit to LDS unit 64k + j = 8 (8k + j/8) + j%8, i.e. row r owns units
+// [8r, 8r + 8); the unit a lane fetches is rotated by r (unit u of row r sits in slot (u + r) % 8)
+// so that the per-lane reads of one unit fall in different banks.  Replaces eight scattered
+// 16-byte gathers per lane (64 lines per instruction) by eight coalesced loads (8 lines each).
+typedef __attribute__((address_space(3))) void lds_void_t;
+typedef __attribute__((address_space(1))) void global_void_t;
+
+__device__ __forceinline__ row_regs coop_load_row(const uint4* __restrict__ blocks, uint32_t W,
+                                                  uint4* lds_wave, uint32_t lane) {
+  const uint32_t line = W & TGT_LINE_MASK;
+  const uint32_t unit = ((lane & 7u) - (lane >> 3)) & 7u;
+  // earlier reads of this LDS region are complete before it is overwritten
+  asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+#pragma unroll
+  for (uint32_t k = 0; k < 8; ++k) {
+    const uint32_t L = (uint32_t)__shfl((int)line, (int)(8u * k + (lane >> 3)), 64);
+    const uint4* src = blocks + (size_t)ROW_UNITS * L + unit;
+    __builtin_amdgcn_global_load_lds((global_void_t*)src, (lds_void_t*)(lds_wave + 64u * k), 16, 0,
+                                     0);
+  }
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  const uint4* my = lds_wave + 8u * lane;
+  return row_from_units(my[(0u + lane) & 7u], my[(1u + lane) & 7u], my[(2u + lane) & 7u],
+                        my[(3u + lane) & 7u], my[(4u + lane) & 7u], my[(5u + lane) & 7u],
+                        my[(6u + lane) & 7u], my[(7u + lane) & 7u]);
+}
+
 // Index picked by update_state's binary search (scatterer.cpp:17-30): the first entry whose
 // cumulative rate exceeds dice, or n-1.  Rows are non-decreasing (rates >= 0 are enforced when
 // the CSR is built), so that index equals the number of leading entries k < n-1 with
 // cum[k] <= dice, counted here without a dependent load chain.
-__device__ __forceinline__ uint32_t select_entry(const uint4* __restrict__ blocks, uint32_t R,
+__device__ __forceinline__ uint32_t select_entry(const uint4* __restrict__ blocks, uint32_t W,
                                                  const row_regs& r, double dice) {
   const uint32_t lim = r.n - 1u;
   uint32_t idx = 0;
 #pragma unroll
   for (uint32_t q = 0; q < 8; ++q) idx += (q < lim && r.c[q] <= dice) ? 1u : 0u;
   if (idx == 8u && lim > 8u) {
-    const double* ce = reinterpret_cast<const double*>(blocks + (size_t)ROW_UNITS * R + ROW_UNITS);
+    // cum[8..16) from the continuation line in one batch of loads (a row with lim > 8 owns at
+    // least two lines), then further entries one at a time
+    const double* ce = reinterpret_cast<const double*>(blocks + (size_t)ROW_UNITS * (W & TGT_LINE_MASK) +
+                                                       ROW_UNITS);
     uint32_t k = 8;
-    while (k < lim && ce[k - 8u] <= dice) ++k;
+#pragma unroll
+    for (uint32_t q = 0; q < 8; ++q) k += (8u + q < lim && ce[q] <= dice) ? 1u : 0u;
+    if (k == 16u)
+      while (k < lim && ce[k - 8u] <= dice) ++k;
     idx = k;
   }
   return idx;
 }
 
 // Target word of entry idx: from the registers for idx < 8, else from the continuation lines.
-__device__ __forceinline__ uint32_t target_of(const uint4* __restrict__ blocks, uint32_t R,
+__device__ __forceinline__ uint32_t target_of(const uint4* __restrict__ blocks, uint32_t W,
                                               const row_regs& r, uint32_t idx) {
   if (idx >= ROW_INLINE) {
     const uint32_t m = r.n - ROW_INLINE;
-    const double* ce = reinterpret_cast<const double*>(blocks + (size_t)ROW_UNITS * R + ROW_UNITS);
+    const double* ce = reinterpret_cast<const double*>(blocks + (size_t)ROW_UNITS * (W & TGT_LINE_MASK) +
+                                                       ROW_UNITS);
     return reinterpret_cast<const uint32_t*>(ce + m)[idx - ROW_INLINE];
   }
   const uint4 q = idx < 4u ? r.t0 : r.t1;
@@ -516,7 +565,7 @@ __global__ __launch_bounds__(256) void k_inject(exciton_arrays x, const uint4* _
   const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= x.n) return;
   const u32x4 w = draw4(seed, e, 0, STREAM_INIT);
-  const uint4 h = blocks[(size_t)ROW_UNITS * src_roff[r31(w.v[0]) % n_src]];
+  const uint4 h = blocks[(size_t)ROW_UNITS * (src_roff[r31(w.v[0]) % n_src] & TGT_LINE_MASK)];
   const uint32_t r = nonzero_draw(w.v[1], w.v[3], seed, e, 0, STREAM_INIT_REDRAW);
   x.site[e] = (int32_t)h.y;
   x.t_next[e] = 0.0 + dwell_time(h.x, __hiloint2double((int)h.w, (int)h.z), r);
@@ -538,13 +587,16 @@ __global__ __launch_bounds__(256) void k_links(int64_t n, const int32_t* __restr
                                                const int32_t* __restrict__ right,
                                                const uint8_t* __restrict__ flags,
                                                const int64_t* __restrict__ roff,
+                                               const site_hdr* __restrict__ hdr,
                                                uint2* __restrict__ links) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int32_t l = left[i], r = right[i];
   uint2 o;
-  o.x = l < 0 ? LINK_NONE : ((uint32_t)roff[l] | ((flags[l] & 2u) ? EDGE_DRAIN_BIT : 0u));
-  o.y = r < 0 ? LINK_NONE : ((uint32_t)roff[r] | ((flags[r] & 2u) ? EDGE_DRAIN_BIT : 0u));
+  o.x = l < 0 ? LINK_NONE
+              : (row_word((uint32_t)roff[l], hdr[l].n) | ((flags[l] & 2u) ? EDGE_DRAIN_BIT : 0u));
+  o.y = r < 0 ? LINK_NONE
+              : (row_word((uint32_t)roff[r], hdr[r].n) | ((flags[r] & 2u) ? EDGE_DRAIN_BIT : 0u));
   links[i] = o;
 }
 
@@ -577,7 +629,7 @@ __device__ __forceinline__ void fly_segment(const net_view& v, const flight_stat
                                             float& px, float& py, float& pz, double& sx,
                                             double& sy, double& sz) {
 #pragma clang fp contract(off)
-  const uint4 p = v.blocks[(size_t)ROW_UNITS * Rn + 7];
+  const uint4 p = v.blocks[(size_t)ROW_UNITS * (Rn & TGT_LINE_MASK) + 7];
   px = __uint_as_float(p.x);
   py = __uint_as_float(p.y);
   pz = __uint_as_float(p.z);
@@ -598,7 +650,7 @@ __device__ bool fly(const net_view& v, flight_state& f, double tau, uint32_t e, 
   uint32_t j = 0;
   for (uint32_t pass = 0; pass < FLY_MAX_PASSES; ++pass) {
     const uint32_t nx = fly_next(lk, f.heading);
-    const uint32_t Rn = nx & EDGE_COL_MASK;
+    const uint32_t Rn = nx & ~EDGE_DRAIN_BIT;
     float px, py, pz;
     double sx, sy, sz;
     fly_segment(v, f, Rn, px, py, pz, sx, sy, sz);
@@ -614,7 +666,7 @@ __device__ bool fly(const net_view& v, flight_state& f, double tau, uint32_t e, 
     f.dz = f.dz + sz;
     f.s = 0.0;
     f.R = Rn;
-    f.site = v.blocks[(size_t)ROW_UNITS * Rn].y;
+    f.site = v.blocks[(size_t)ROW_UNITS * (Rn & TGT_LINE_MASK)].y;
     f.x = px;
     f.y = py;
     f.z = pz;
@@ -624,8 +676,8 @@ __device__ bool fly(const net_view& v, flight_state& f, double tau, uint32_t e, 
       const u32x4 w = draw4(v.seed, e, h, STREAM_FLY_DRAIN + j);
       ++j;
       f.R = v.src_roff[r31(w.v[0]) % v.n_src];
-      const uint4 hd = v.blocks[(size_t)ROW_UNITS * f.R];
-      const uint4 p = v.blocks[(size_t)ROW_UNITS * f.R + 7];
+      const uint4 hd = v.blocks[(size_t)ROW_UNITS * (f.R & TGT_LINE_MASK)];
+      const uint4 p = v.blocks[(size_t)ROW_UNITS * (f.R & TGT_LINE_MASK) + 7];
       f.site = hd.y;
       f.x = __uint_as_float(p.x);
       f.y = __uint_as_float(p.y);
@@ -645,7 +697,7 @@ __device__ __forceinline__ void fly_offset(const net_view& v, const flight_state
   ox = oy = oz = 0.0;
   if (!(f.s != 0.0)) return;
   uint32_t hd = f.heading;
-  const uint32_t Rn = fly_next(v.links[f.site], hd) & EDGE_COL_MASK;
+  const uint32_t Rn = fly_next(v.links[f.site], hd) & ~EDGE_DRAIN_BIT;
   float px, py, pz;
   double sx, sy, sz;
   fly_segment(v, f, Rn, px, py, pz, sx, sy, sz);
@@ -673,7 +725,8 @@ __device__ __forceinline__ void hop_exciton(const net_view& v, uint32_t e, uint6
   uint32_t k = SAMPLE ? v.x.k_rec[e] : 0u;
   int64_t tr = 0;
   if (TRACE && e < tv.n_trace) tr = tv.tr_n[e];
-  uint32_t R = v.roff[v.x.site[e]];
+  const uint32_t R0 = (uint32_t)v.roff[v.x.site[e]];
+  uint32_t R = row_word(R0, v.blocks[(size_t)ROW_UNITS * R0].x);
   row_regs row = load_row(v.blocks, R);
   const uint64_t h_end = h + hop_budget;
   const size_t E = v.x.n;
@@ -748,7 +801,7 @@ __device__ __forceinline__ void hop_exciton(const net_view& v, uint32_t e, uint6
     const double t_hop = t;
 
     // dwell at the new site (particle.cpp:183 -> ff_time)
-    R = tg & EDGE_COL_MASK;
+    R = tg & ~EDGE_DRAIN_BIT;
     row = load_row(v.blocks, R);
     dx = dx + min_image((double)row.x - (double)sx, v.per_len[0]);
     dy = dy + min_image((double)row.y - (double)sy, v.per_len[1]);
@@ -794,6 +847,128 @@ __device__ __forceinline__ void hop_exciton(const net_view& v, uint32_t e, uint6
   }
 }
 
+// The hop loop of one wave (v = 0), built around coop_load_row: every lane takes part in every
+// row load, so the loop runs until no lane of the wave can hop; a lane without an exciton (valid
+// false) or whose exciton has stopped keeps loading its current row.  A hop is finished once its
+// target row is in registers (displacement, visit hash, dwell at the new site), which keeps every
+// exciton's arithmetic and its order identical to hop_exciton.  A drain target's header and
+// position are gathered by the lanes concerned, and their next row load is the source site's.
+template <bool COUNT, bool SAMPLE, bool TRACE>
+__device__ __forceinline__ void hop_wave(const net_view& v, uint32_t e, bool valid,
+                                         uint64_t hop_budget, double t_max, const trace_view& tv,
+                                         uint4* lds_wave, uint32_t lane) {
+#pragma clang fp contract(off)
+  double t = INFINITY, dx = 0.0, dy = 0.0, dz = 0.0;
+  uint64_t h = 0, hs = 0;
+  uint32_t nd = 0, k = 0, W = 0;
+  int64_t tr = 0;
+  if (valid) {
+    t = v.x.t_next[e];
+    h = v.x.hops[e];
+    dx = v.x.dx[e];
+    dy = v.x.dy[e];
+    dz = v.x.dz[e];
+    nd = v.x.drains[e];
+    hs = v.x.hash[e];
+    if (SAMPLE) k = v.x.k_rec[e];
+    if (TRACE && e < tv.n_trace) tr = tv.tr_n[e];
+    const uint32_t R0 = (uint32_t)v.roff[v.x.site[e]];
+    W = row_word(R0, v.blocks[(size_t)ROW_UNITS * R0].x);
+  }
+  const uint64_t h_end = h + hop_budget;
+  const size_t E = v.x.n;
+  bool active = valid;
+  bool pend = false;           // a hop waits for its target row
+  uint32_t r_dw = 0;           // its dwell draw
+  float px = 0.f, py = 0.f, pz = 0.f;  // position of the site it left
+  double t_hop = 0.0;
+  count_cache cc;
+  uint32_t* cbase = COUNT ? v.site_count + (size_t)xcd_id() * v.count_stride : nullptr;
+  row_regs row;
+
+  while (true) {
+    row = coop_load_row(v.blocks, W, lds_wave, lane);
+    if (pend) {
+      // dwell at the new site (particle.cpp:183 -> ff_time)
+      dx = dx + min_image((double)row.x - (double)px, v.per_len[0]);
+      dy = dy + min_image((double)row.y - (double)py, v.per_len[1]);
+      dz = dz + min_image((double)row.z - (double)pz, v.per_len[2]);
+      hs = (hs ^ (uint64_t)row.site) * FNV_PRIME;
+      t = t + dwell_time(row.n, row.gamma, r_dw);
+      if (TRACE && e < tv.n_trace && tr < tv.tr_max) {
+        tv.tr_site[(size_t)e * tv.tr_max + tr] = (int32_t)row.site;
+        tv.tr_time[(size_t)e * tv.tr_max + tr] = t_hop;
+        ++tr;
+      }
+      pend = false;
+    }
+    if (SAMPLE && active) {
+      while (k < (uint32_t)v.n_bins) {
+        const double tk = (double)(k + 1u) * v.dt_sample;
+        if (!(tk < t)) break;
+        const size_t r = (size_t)k * E + e;
+        v.rec_dx[r] = dx;
+        v.rec_dy[r] = dy;
+        v.rec_dz[r] = dz;
+        v.rec_drain[r] = nd;
+        ++k;
+      }
+    }
+    active = active && !(h >= h_end || trapped(row.n, row.gamma) || !(t <= t_max));
+    if (__ballot(active) == 0) break;
+    if (active) {
+      // event selection (scatterer::update_state, scatterer.cpp:17-30)
+      const u32x4 w = draw4(v.seed, e, h, STREAM_HOP);
+      const double dice = (row.gamma * (double)r31(w.v[0])) / RAND_MAX_D;
+      const uint32_t idx = select_entry(v.blocks, W, row, dice);
+      const uint32_t tg = target_of(v.blocks, W, row, idx);
+      if (COUNT) cc.add(row.site, cbase);
+      const uint32_t r = nonzero_draw(w.v[1], w.v[3], v.seed, e, h, STREAM_HOP_REDRAW);
+      t_hop = t;
+      if (tg & EDGE_DRAIN_BIT) {
+        // drain: the dwell drawn at the drain site, then re-injection keeping displacement and
+        // dwell (kubo_step, src/monte_carlo/monte_carlo.cpp:330-336)
+        const uint4* bd = v.blocks + (size_t)ROW_UNITS * (tg & TGT_LINE_MASK);
+        const uint4 hd = bd[0], pd = bd[7];
+        dx = dx + min_image((double)__uint_as_float(pd.x) - (double)row.x, v.per_len[0]);
+        dy = dy + min_image((double)__uint_as_float(pd.y) - (double)row.y, v.per_len[1]);
+        dz = dz + min_image((double)__uint_as_float(pd.z) - (double)row.z, v.per_len[2]);
+        hs = (hs ^ (uint64_t)hd.y) * FNV_PRIME;
+        t = t + dwell_time(hd.x, __hiloint2double((int)hd.w, (int)hd.z), r);
+        ++nd;
+        W = v.src_roff[r31(w.v[2]) % v.n_src];
+        const uint32_t src_site = v.blocks[(size_t)ROW_UNITS * (W & TGT_LINE_MASK)].y;
+        hs = (hs ^ ((uint64_t)src_site | (1ull << 32))) * FNV_PRIME;
+        if (TRACE && e < tv.n_trace && tr < tv.tr_max) {
+          tv.tr_site[(size_t)e * tv.tr_max + tr] = (int32_t)src_site;
+          tv.tr_time[(size_t)e * tv.tr_max + tr] = t_hop;
+          ++tr;
+        }
+      } else {
+        W = tg & ~EDGE_DRAIN_BIT;
+        pend = true;
+        r_dw = r;
+        px = row.x;
+        py = row.y;
+        pz = row.z;
+      }
+      ++h;
+    }
+  }
+  if (COUNT) cc.flush(cbase);
+  if (!valid) return;
+  v.x.site[e] = (int32_t)row.site;
+  v.x.t_next[e] = t;
+  v.x.hops[e] = h;
+  v.x.dx[e] = dx;
+  v.x.dy[e] = dy;
+  v.x.dz[e] = dz;
+  v.x.drains[e] = nd;
+  v.x.hash[e] = hs;
+  if (SAMPLE) v.x.k_rec[e] = k;
+  if (TRACE && e < tv.n_trace) tv.tr_n[e] = tr;
+}
+
 // Position of this thread in the sorted exciton order.  Workgroups are dealt to the XCDs in turn,
 // so workgroup b takes the b%8-th eighth of the order: each XCD's L2 serves excitons that sit on
 // neighbouring blocks.  gridDim.x is a multiple of KMC_COUNT_COPIES.
@@ -807,10 +982,14 @@ __device__ __forceinline__ uint32_t xcd_position() {
 // changes nothing but locality).
 template <bool COUNT, bool SAMPLE, bool TRACE>
 __global__ __launch_bounds__(256, 4) void k_hop(hop_args a) {
+  __shared__ uint4 rows[4 * 64 * ROW_UNITS];  // 8 KB of staged rows per wave
   const uint32_t pos = xcd_position();
-  if (pos >= a.v.x.n) return;
-  const uint32_t e = a.perm ? a.perm[pos] : pos;
-  hop_exciton<COUNT, SAMPLE, TRACE>(a.v, e, a.hop_budget, a.t_max, a.tv);
+  const uint32_t lane = threadIdx.x & 63u;
+  if (__builtin_amdgcn_readfirstlane(pos - lane) >= a.v.x.n) return;  // the whole wave is past the end
+  const bool valid = pos < a.v.x.n;
+  const uint32_t e = valid ? (a.perm ? a.perm[pos] : pos) : 0u;
+  uint4* lds_wave = rows + 64u * ROW_UNITS * __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  hop_wave<COUNT, SAMPLE, TRACE>(a.v, e, valid, a.hop_budget, a.t_max, a.tv, lds_wave, lane);
 }
 
 // The same with drift along the tubes (v > 0).
@@ -831,13 +1010,16 @@ template <bool COUNT, bool SAMPLE>
 __global__ __launch_bounds__(256, 4) void k_hop_ens(ens_args a) {
   const uint32_t pos = xcd_position();
   if (pos >= a.n_positions) return;
+  __shared__ uint4 rows[4 * 64 * ROW_UNITS];  // 8 KB of staged rows per wave
+  const uint32_t lane = threadIdx.x & 63u;
   const uint32_t net = __builtin_amdgcn_readfirstlane(a.wave_net[pos >> 6]);
   const net_view v = a.nets[net];
   const int64_t q = (int64_t)pos - a.pbase[net];
-  if (q >= v.x.n) return;
+  const bool valid = q < v.x.n;
   const int64_t eb = a.ebase[net];
-  const uint32_t e = (uint32_t)((int64_t)a.perm[eb + q] - eb);
-  hop_exciton<COUNT, SAMPLE, false>(v, e, a.hop_budget, a.t_max, trace_view{});
+  const uint32_t e = valid ? (uint32_t)((int64_t)a.perm[eb + q] - eb) : 0u;
+  uint4* lds_wave = rows + 64u * ROW_UNITS * __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  hop_wave<COUNT, SAMPLE, false>(v, e, valid, a.hop_budget, a.t_max, trace_view{}, lds_wave, lane);
 }
 
 hipError_t launch_hop_ens(const ens_args& a, bool count, bool sample, hipStream_t st) {
@@ -1112,16 +1294,17 @@ hipError_t launch_mask_count(int64_t nnz, const uint8_t* cls, const uint8_t* mas
 }
 
 hipError_t launch_links(int64_t n, const int32_t* left, const int32_t* right, const uint8_t* flags,
-                        const int64_t* roff, uint2* links, hipStream_t st) {
+                        const int64_t* roff, const site_hdr* hdr, uint2* links, hipStream_t st) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_links, grid_for(n), dim3(256), 0, st, n, left, right, flags, roff, links);
+  hipLaunchKernelGGL(k_links, grid_for(n), dim3(256), 0, st, n, left, right, flags, roff, hdr,
+                     links);
   return hipGetLastError();
 }
 
-hipError_t launch_gather_offsets(int64_t n, const int32_t* idx, const int64_t* roff, uint32_t* out,
-                                 hipStream_t st) {
+hipError_t launch_gather_offsets(int64_t n, const int32_t* idx, const int64_t* roff,
+                                 const site_hdr* hdr, uint32_t* out, hipStream_t st) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_gather_offsets, grid_for(n), dim3(256), 0, st, n, idx, roff, out);
+  hipLaunchKernelGGL(k_gather_offsets, grid_for(n), dim3(256), 0, st, n, idx, roff, hdr, out);
   return hipGetLastError();
 }
 
