@@ -14,6 +14,9 @@ the default 10 timed steps are the full 10^5 hops per exciton.  A second run on 
 exact-built tables with the same seeds gives the diffusion coefficient the surrogate run is
 compared against.
 
+--workload film_1600k_sites runs the larger film instead: 80,000 CNTs of 200 nm in a
+20x20x0.5 um box (1.6M sites), 16M excitons, with a 64-network ensemble leg.
+
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
 from __future__ import annotations
@@ -324,7 +327,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--excitons", type=int, default=1_000_000)
+    ap.add_argument("--workload", default="film_625k_sites",
+                    choices=["film_625k_sites", "film_1600k_sites"],
+                    help="film_625k_sites: 5,000 CNTs in 5x5x0.2 um, 1M excitons, 256-network "
+                         "ensemble; film_1600k_sites: 80,000 CNTs in 20x20x0.5 um, 16M excitons, "
+                         "64-network ensemble")
+    ap.add_argument("--excitons", type=int, default=0, help="0: the workload's own count")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-exact", action="store_true",
                     help="skip the exact table build and the exact-table comparison run")
@@ -354,7 +362,8 @@ def main():
     from cnt_film_monte_carlo_amd import kmc
     from cnt_film_monte_carlo_amd import networks as nw
 
-    w = nw.film_large()
+    w = nw.film_large() if args.workload == "film_625k_sites" else nw.film_xlarge()
+    n_networks = 256 if args.workload == "film_625k_sites" else 64
     t0 = time.perf_counter()
     net = nw.build_network(w)
     log(f"network: {net['n']} sites in {time.perf_counter() - t0:.2f} s")
@@ -375,7 +384,7 @@ def main():
     csr = ctx.get_csr()
     deg = np.diff(csr["row_ptr"])
 
-    E = args.excitons
+    E = args.excitons or w.n_excitons
     # warmup (untimed) then a fresh injection for the timed region
     ctx.inject(E)
     for i in range(args.warmup):
@@ -423,14 +432,18 @@ def main():
     if tabs["exact"] is not None:
         Dx, cix, d_rel = exact_table_run(w, net, tabs, E, args, t_sample_max, D)
     sur_roof = surrogate_roofline(tabs["surrogate"])
-    ens = None if args.no_ensemble else ensemble_leg(tabs["surrogate"])
+    ens = None if args.no_ensemble else ensemble_leg(tabs["surrogate"], n_networks=n_networks)
 
 
+    # HBM bytes per hop-kernel launch measured with rocprofv3 counters on this workload
+    # (FETCH_SIZE and WRITE_SIZE passes scaled by a plain-copy calibration); None if unmeasured
     traffic = None
     tf = ROOT / "profiles" / "hop_kernel_traffic.json"
     if tf.exists():
         try:
-            traffic = json.loads(tf.read_text()).get("bytes_per_launch")
+            rec = json.loads(tf.read_text()).get(w.name)
+            if rec and int(rec.get("hops_per_launch", 0)) == int(round(hops_per_launch)):
+                traffic = rec.get("bytes_per_launch")
         except Exception:
             traffic = None
 

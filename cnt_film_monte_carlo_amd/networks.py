@@ -52,6 +52,17 @@ def film_large(seed: int = 1002) -> Workload:
                     chiralities=((7, 5), (8, 6), (9, 7)))
 
 
+def film_xlarge(seed: int = 1005) -> Workload:
+    """80,000 CNTs of 200 nm in a 20x20x0.5 um box, 10 nm segments (1.6M sites), chiralities
+    (7,5), (8,6), (9,7) at 1/3 each, cutoff 20 nm, y periodic, 16M excitons x 1e5 hops."""
+    net = kmc.NetworkParams(n_tubes=80_000, len_min=200 * NM, len_max=200 * NM,
+                            box_lo=(0.0, 0.0, 0.0), box_hi=(20_000 * NM, 20_000 * NM, 500 * NM),
+                            site_spacing=10 * NM, n_chir=3, wrap_mask=kmc.PERIODIC_Y, seed=seed)
+    return Workload("film_1600k_sites", net, cutoff=20 * NM, x_source=0.0, x_drain=20_000 * NM,
+                    period_y=(0.0, 20_000 * NM), n_excitons=16_000_000, hops_per_exciton=100_000,
+                    chiralities=((7, 5), (8, 6), (9, 7)))
+
+
 def contact_flags(x: np.ndarray, x_source: float, x_drain: float) -> np.ndarray:
     """KMC_SITE_SOURCE for x <= x_source, KMC_SITE_DRAIN for x >= x_drain."""
     f = np.zeros(x.size, np.uint8)
