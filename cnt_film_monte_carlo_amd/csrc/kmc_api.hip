@@ -86,6 +86,8 @@ struct kmc_ctx {
   dbuf<int64_t> d_roff;
   dbuf<uint4> d_blocks;
   dbuf<int64_t> d_first;         // block offsets in placement order (scan output)
+  dbuf<uint64_t> d_pkeys;        // block placement: Morton keys (2 x n_sites, in and out)
+  dbuf<int32_t> d_place;         // block placement: site ids (2 x n_sites, in and out)
   dbuf<uint32_t> d_keys, d_ids;  // exciton sort: 2 x n_exc each (input and output halves)
   dbuf<uint8_t> d_sort_tmp;
   size_t sort_tmp_bytes = 0;
@@ -269,7 +271,27 @@ int pack_rows(kmc_ctx* c) {
   KMC_TRY(dalloc(c, c->d_roff, n + 1));
   KMC_TRY(dalloc(c, c->d_first, n + 1));
   KMC_TRY(dalloc(c, c->d_scan_tmp, scan_tmp_elems(n)));
-  const int32_t* order = c->cells_ok ? c->d_cell_sites.p : nullptr;
+  const int32_t* order = nullptr;
+  if (c->cells_ok) {
+    // sites in Morton order of their cells (ties: ascending site index, the sort is stable)
+    KMC_TRY(dalloc(c, c->d_pkeys, 2 * n));
+    KMC_TRY(dalloc(c, c->d_place, 2 * n));
+    KMC_HIP(c, launch_morton_keys(n, c->d_cell_of.p, c->cells.nx, c->cells.ny, c->d_pkeys.p,
+                                  c->d_place.p, c->stream));
+    size_t need = 0;
+    KMC_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, need, c->d_pkeys.p, c->d_pkeys.p + n,
+                                                  c->d_place.p, c->d_place.p + n, (int)n, 0, 64,
+                                                  c->stream));
+    if (need > c->sort_tmp_bytes) {
+      KMC_TRY(dalloc(c, c->d_sort_tmp, need));
+      c->sort_tmp_bytes = need;
+    }
+    size_t have = c->sort_tmp_bytes;
+    KMC_HIP(c, hipcub::DeviceRadixSort::SortPairs(c->d_sort_tmp.p, have, c->d_pkeys.p,
+                                                  c->d_pkeys.p + n, c->d_place.p,
+                                                  c->d_place.p + n, (int)n, 0, 64, c->stream));
+    order = c->d_place.p + n;
+  }
   if (order) {
     KMC_HIP(c, launch_block_sizes(n, c->d_hdr.p, order, c->d_units.p, c->stream));
     KMC_HIP(c, exclusive_scan(c->d_units.p, c->d_first.p, n, c->d_scan_tmp.p, c->stream));
@@ -393,6 +415,7 @@ int kmc_destroy(kmc_ctx* c) {
   dfree(c->d_row_ptr); dfree(c->d_hdr); dfree(c->d_cum); dfree(c->d_pair_rates);
   dfree(c->d_edge);
   for (int q = 0; q < 4; ++q) dfree(c->d_geo[q]);
+  dfree(c->d_pkeys); dfree(c->d_place);
   dfree(c->d_units); dfree(c->d_src_roff); dfree(c->d_roff); dfree(c->d_blocks);
   dfree(c->d_first); dfree(c->d_keys); dfree(c->d_ids); dfree(c->d_sort_tmp);
   dfree(c->d_counts64); dfree(c->d_cls); dfree(c->d_cls_count); dfree(c->d_cls_start);

@@ -126,6 +126,9 @@ hipError_t launch_exciton_keys(int64_t n, const int32_t* site, const int64_t* ro
 hipError_t launch_fill_blocks(int64_t n, const site_hdr* hdr, const double* cum,
                               const edge_rec* edge, const int64_t* roff, const double* px,
                               const double* py, const double* pz, uint4* blocks, hipStream_t st);
+// keys[i] = Morton code of site i's cell, ids[i] = i (block placement order, after a sort)
+hipError_t launch_morton_keys(int64_t n, const int32_t* cell_of, int32_t nx, int32_t ny,
+                              uint64_t* keys, int32_t* ids, hipStream_t st);
 // out[i] = row word of site idx[i] (first line of its block | TGT_WIDE_BIT)
 hipError_t launch_gather_offsets(int64_t n, const int32_t* idx, const int64_t* roff,
                                  const site_hdr* hdr, uint32_t* out, hipStream_t st);

@@ -324,6 +324,32 @@ __device__ __forceinline__ uint32_t row_word(uint32_t R, uint32_t n) {
   return R | (n > ROW_INLINE ? TGT_WIDE_BIT : 0u);
 }
 
+// Placement order of the blocks: sites sorted by the Morton (Z-order) code of their cell, so that
+// any contiguous range of blocks -- one XCD's share of the sorted excitons, a workgroup's rows --
+// covers a compact region of the film rather than a slab.  Only the memory layout follows this
+// order; the neighbour order of every row stays the reference's.
+__device__ __forceinline__ uint64_t spread3(uint32_t v) {
+  uint64_t x = v & 0x1FFFFFu;
+  x = (x | (x << 32)) & 0x1F00000000FFFFull;
+  x = (x | (x << 16)) & 0x1F0000FF0000FFull;
+  x = (x | (x << 8)) & 0x100F00F00F00F00Full;
+  x = (x | (x << 4)) & 0x10C30C30C30C30C3ull;
+  x = (x | (x << 2)) & 0x1249249249249249ull;
+  return x;
+}
+
+__global__ __launch_bounds__(256) void k_morton_keys(int64_t n, const int32_t* __restrict__ cell_of,
+                                                     int32_t nx, int32_t ny,
+                                                     uint64_t* __restrict__ keys,
+                                                     int32_t* __restrict__ ids) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t c = cell_of[i];
+  const uint32_t ix = (uint32_t)(c % nx), iy = (uint32_t)((c / nx) % ny), iz = (uint32_t)(c / ((int64_t)nx * ny));
+  keys[i] = spread3(ix) | (spread3(iy) << 1) | (spread3(iz) << 2);
+  ids[i] = (int32_t)i;
+}
+
 // Lines of the q-th block in placement order (order[q] = its site; identity when NULL).
 __global__ __launch_bounds__(256) void k_block_sizes(int64_t n, const site_hdr* __restrict__ hdr,
                                                      const int32_t* __restrict__ order,
@@ -1298,6 +1324,13 @@ hipError_t launch_links(int64_t n, const int32_t* left, const int32_t* right, co
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_links, grid_for(n), dim3(256), 0, st, n, left, right, flags, roff, hdr,
                      links);
+  return hipGetLastError();
+}
+
+hipError_t launch_morton_keys(int64_t n, const int32_t* cell_of, int32_t nx, int32_t ny,
+                              uint64_t* keys, int32_t* ids, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_morton_keys, grid_for(n), dim3(256), 0, st, n, cell_of, nx, ny, keys, ids);
   return hipGetLastError();
 }
 
