@@ -1328,34 +1328,63 @@ hipError_t launch_fill_blocks(int64_t n, const site_hdr* hdr, const double* cum,
 }
 
 // Chirality class (donor * n_chir + acceptor) of every CSR entry, and per-class index lists.
+// Class counts are summed per workgroup in LDS first (a film of one chirality puts every pair
+// in one class), then one device atomic per class and workgroup.
+constexpr int MAX_PAIR_CLASSES = 256;
+
+__device__ __forceinline__ void clear_class_counts(uint32_t* lds) {
+  for (int c = threadIdx.x; c < MAX_PAIR_CLASSES; c += blockDim.x) lds[c] = 0u;
+  __syncthreads();
+}
+
+__device__ __forceinline__ void flush_class_counts(const uint32_t* lds, uint32_t* __restrict__ counts,
+                                                   int n_classes) {
+  __syncthreads();
+  for (int c = threadIdx.x; c < n_classes; c += blockDim.x)
+    if (lds[c]) atomicAdd(&counts[c], lds[c]);
+}
+
 __global__ __launch_bounds__(256) void k_pair_class(int64_t n, const site_hdr* __restrict__ hdr,
                                                     const edge_rec* __restrict__ edge,
                                                     const uint8_t* __restrict__ chir, int32_t n_chir,
                                                     uint8_t* __restrict__ cls,
                                                     uint32_t* __restrict__ counts) {
+  __shared__ uint32_t lds[MAX_PAIR_CLASSES];
+  clear_class_counts(lds);
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const site_hdr h = hdr[i];
-  const int ci = chir[i];
-  for (uint32_t k = 0; k < h.n; ++k) {
-    const int cj = chir[edge[h.rs + k].col & EDGE_COL_MASK];
-    const int c = ci * n_chir + cj;
-    cls[h.rs + k] = (uint8_t)c;
-    atomicAdd(&counts[c], 1u);
+  if (i < n) {
+    const site_hdr h = hdr[i];
+    const int ci = chir[i];
+    for (uint32_t k = 0; k < h.n; ++k) {
+      const int cj = chir[edge[h.rs + k].col & EDGE_COL_MASK];
+      const int c = ci * n_chir + cj;
+      cls[h.rs + k] = (uint8_t)c;
+      atomicAdd(&lds[c], 1u);
+    }
   }
+  flush_class_counts(lds, counts, n_chir * n_chir);
 }
 
 // entries with mask[p] == 0 (mask optional) are left out
+// Slots within a class: a workgroup first numbers its own entries per class in LDS, then
+// reserves one range per class with a single device atomic.
 __global__ __launch_bounds__(256) void k_pair_scatter(int64_t nnz, const uint8_t* __restrict__ cls,
                                                       const uint8_t* __restrict__ mask,
                                                       const int64_t* __restrict__ starts,
                                                       uint32_t* __restrict__ fill,
                                                       int64_t* __restrict__ sel) {
+  __shared__ uint32_t local[MAX_PAIR_CLASSES];
+  __shared__ uint32_t base[MAX_PAIR_CLASSES];
+  clear_class_counts(local);
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= nnz || (mask && !mask[p])) return;
-  const int c = cls[p];
-  const uint32_t slot = atomicAdd(&fill[c], 1u);
-  sel[starts[c] + slot] = p;
+  const bool take = p < nnz && (!mask || mask[p]);
+  const int c = take ? (int)cls[p] : 0;
+  const uint32_t slot = take ? atomicAdd(&local[c], 1u) : 0u;
+  __syncthreads();
+  for (int q = threadIdx.x; q < MAX_PAIR_CLASSES; q += blockDim.x)
+    base[q] = local[q] ? atomicAdd(&fill[q], local[q]) : 0u;
+  __syncthreads();
+  if (take) sel[starts[c] + base[c] + slot] = p;
 }
 
 hipError_t launch_pair_classes(int64_t n, int64_t nnz, const site_hdr* hdr, const edge_rec* edge,
@@ -1378,8 +1407,11 @@ hipError_t launch_pair_scatter(int64_t nnz, const uint8_t* cls, const uint8_t* m
 __global__ __launch_bounds__(256) void k_mask_count(int64_t nnz, const uint8_t* __restrict__ cls,
                                                     const uint8_t* __restrict__ mask,
                                                     uint32_t* __restrict__ counts) {
+  __shared__ uint32_t lds[MAX_PAIR_CLASSES];
+  clear_class_counts(lds);
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < nnz && mask[p]) atomicAdd(&counts[cls[p]], 1u);
+  if (p < nnz && mask[p]) atomicAdd(&lds[cls[p]], 1u);
+  flush_class_counts(lds, counts, MAX_PAIR_CLASSES);
 }
 
 hipError_t launch_mask_count(int64_t nnz, const uint8_t* cls, const uint8_t* mask, uint32_t* counts,
