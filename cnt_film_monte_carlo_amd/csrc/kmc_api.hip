@@ -224,8 +224,10 @@ net_view view_of(kmc_ctx* c) {
   v.src_roff = c->d_src_roff.p;
   v.n_src = (uint32_t)c->h_src.size();
   v.seed = c->cfg.seed;
-  for (int k = 0; k < 3; ++k)
+  for (int k = 0; k < 3; ++k) {
     v.per_len[k] = (c->cfg.periodic_mask & (1 << k)) ? c->cfg.period_len[k] : 0.0;
+    v.half_per[k] = 0.5 * v.per_len[k];
+  }
   v.n_bins = c->n_bins;
   v.dt_sample = c->n_bins > 0 ? c->t_sample_max / c->n_bins : 0.0;
   v.rec_dx = c->r_dx.p;

@@ -48,6 +48,7 @@ struct net_view {
   uint32_t n_src;
   uint64_t seed;
   double per_len[3];        // period per axis (0 = open) for the minimum-image displacement
+  double half_per[3];       // 0.5 * per_len (kernel arguments, so the thresholds stay scalar)
   // MSD records [n_bins][x.n]
   int32_t n_bins;
   double dt_sample;

@@ -481,12 +481,15 @@ __device__ __forceinline__ row_regs coop_load_row(const uint4* __restrict__ bloc
                                                   uint4* lds_wave, uint32_t lane) {
   const uint32_t line = W & TGT_LINE_MASK;
   const uint32_t unit = ((lane & 7u) - (lane >> 3)) & 7u;
+  // the eight source lines first (eight lane exchanges in flight together), then the loads
+  uint32_t L[8];
+#pragma unroll
+  for (uint32_t k = 0; k < 8; ++k) L[k] = (uint32_t)__shfl((int)line, (int)(8u * k + (lane >> 3)), 64);
   // earlier reads of this LDS region are complete before it is overwritten
   asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
 #pragma unroll
   for (uint32_t k = 0; k < 8; ++k) {
-    const uint32_t L = (uint32_t)__shfl((int)line, (int)(8u * k + (lane >> 3)), 64);
-    const uint4* src = blocks + (size_t)ROW_UNITS * L + unit;
+    const uint4* src = blocks + (size_t)ROW_UNITS * L[k] + unit;
     __builtin_amdgcn_global_load_lds((global_void_t*)src, (lds_void_t*)(lds_wave + 64u * k), 16, 0,
                                      0);
   }
@@ -508,15 +511,22 @@ __device__ __forceinline__ uint32_t select_entry(const uint4* __restrict__ block
 #pragma unroll
   for (uint32_t q = 0; q < 8; ++q) idx += (q < lim && r.c[q] <= dice) ? 1u : 0u;
   if (idx == 8u && lim > 8u) {
-    // cum[8..16) from the continuation line in one batch of loads (a row with lim > 8 owns at
-    // least two lines), then further entries one at a time
-    const double* ce = reinterpret_cast<const double*>(blocks + (size_t)ROW_UNITS * (W & TGT_LINE_MASK) +
-                                                       ROW_UNITS);
+    // cum[8..16): the first 64 bytes of the continuation line, which a row with lim > 8 always
+    // owns, read in one batch of four 16-byte loads before any comparison; then further entries
+    // one at a time
+    const uint4* cu = blocks + (size_t)ROW_UNITS * (W & TGT_LINE_MASK) + ROW_UNITS;
+    const uint4 e0 = cu[0], e1 = cu[1], e2 = cu[2], e3 = cu[3];
+    const double c[8] = {__hiloint2double((int)e0.y, (int)e0.x), __hiloint2double((int)e0.w, (int)e0.z),
+                         __hiloint2double((int)e1.y, (int)e1.x), __hiloint2double((int)e1.w, (int)e1.z),
+                         __hiloint2double((int)e2.y, (int)e2.x), __hiloint2double((int)e2.w, (int)e2.z),
+                         __hiloint2double((int)e3.y, (int)e3.x), __hiloint2double((int)e3.w, (int)e3.z)};
     uint32_t k = 8;
 #pragma unroll
-    for (uint32_t q = 0; q < 8; ++q) k += (8u + q < lim && ce[q] <= dice) ? 1u : 0u;
-    if (k == 16u)
+    for (uint32_t q = 0; q < 8; ++q) k += (8u + q < lim && c[q] <= dice) ? 1u : 0u;
+    if (k == 16u) {
+      const double* ce = reinterpret_cast<const double*>(cu);
       while (k < lim && ce[k - 8u] <= dice) ++k;
+    }
     idx = k;
   }
   return idx;
@@ -543,6 +553,17 @@ __device__ __forceinline__ double min_image(double d, double period) {
   if (period > 0.0) {
     if (d > 0.5 * period) d -= period;
     else if (d < -0.5 * period) d += period;
+  }
+  return d;
+}
+
+// The same with 0.5 * period passed in (a kernel argument: the thresholds stay in scalar
+// registers instead of occupying vector registers across the hop loop); halving is exact.
+__device__ __forceinline__ double min_image(double d, double period, double half) {
+#pragma clang fp contract(off)
+  if (period > 0.0) {
+    if (d > half) d -= period;
+    else if (d < -half) d += period;
   }
   return d;
 }
@@ -724,9 +745,9 @@ __device__ __forceinline__ void fly_segment(const net_view& v, const flight_stat
   px = __uint_as_float(p.x);
   py = __uint_as_float(p.y);
   pz = __uint_as_float(p.z);
-  sx = min_image((double)px - (double)f.x, v.per_len[0]);
-  sy = min_image((double)py - (double)f.y, v.per_len[1]);
-  sz = min_image((double)pz - (double)f.z, v.per_len[2]);
+  sx = min_image((double)px - (double)f.x, v.per_len[0], v.half_per[0]);
+  sy = min_image((double)py - (double)f.y, v.per_len[1], v.half_per[1]);
+  sz = min_image((double)pz - (double)f.z, v.per_len[2], v.half_per[2]);
 }
 
 // Flight of duration tau.  A site reached on the way becomes the current site; a drain site
@@ -894,9 +915,9 @@ __device__ __forceinline__ void hop_exciton(const net_view& v, uint32_t e, uint6
     // dwell at the new site (particle.cpp:183 -> ff_time)
     R = tg & ~EDGE_DRAIN_BIT;
     row = load_row(v.blocks, R);
-    dx = dx + min_image((double)row.x - (double)sx, v.per_len[0]);
-    dy = dy + min_image((double)row.y - (double)sy, v.per_len[1]);
-    dz = dz + min_image((double)row.z - (double)sz, v.per_len[2]);
+    dx = dx + min_image((double)row.x - (double)sx, v.per_len[0], v.half_per[0]);
+    dy = dy + min_image((double)row.y - (double)sy, v.per_len[1], v.half_per[1]);
+    dz = dz + min_image((double)row.z - (double)sz, v.per_len[2], v.half_per[2]);
     hs = (hs ^ (uint64_t)row.site) * FNV_PRIME;
     const uint32_t r = nonzero_draw(w.v[1], w.v[3], v.seed, e, h, STREAM_HOP_REDRAW);
     t = t + dwell_time(row.n, row.gamma, r);
@@ -982,9 +1003,9 @@ __device__ __forceinline__ void hop_wave(const net_view& v, uint32_t e, bool val
     row = coop_load_row(v.blocks, W, lds_wave, lane);
     if (pend) {
       // dwell at the new site (particle.cpp:183 -> ff_time)
-      dx = dx + min_image((double)row.x - (double)px, v.per_len[0]);
-      dy = dy + min_image((double)row.y - (double)py, v.per_len[1]);
-      dz = dz + min_image((double)row.z - (double)pz, v.per_len[2]);
+      dx = dx + min_image((double)row.x - (double)px, v.per_len[0], v.half_per[0]);
+      dy = dy + min_image((double)row.y - (double)py, v.per_len[1], v.half_per[1]);
+      dz = dz + min_image((double)row.z - (double)pz, v.per_len[2], v.half_per[2]);
       hs = (hs ^ (uint64_t)row.site) * FNV_PRIME;
       t = t + dwell_time(row.n, row.gamma, r_dw);
       if (TRACE && e < tv.n_trace && tr < tv.tr_max) {
@@ -1022,9 +1043,9 @@ __device__ __forceinline__ void hop_wave(const net_view& v, uint32_t e, bool val
         // dwell (kubo_step, src/monte_carlo/monte_carlo.cpp:330-336)
         const uint4* bd = v.blocks + (size_t)ROW_UNITS * (tg & TGT_LINE_MASK);
         const uint4 hd = bd[0], pd = bd[7];
-        dx = dx + min_image((double)__uint_as_float(pd.x) - (double)row.x, v.per_len[0]);
-        dy = dy + min_image((double)__uint_as_float(pd.y) - (double)row.y, v.per_len[1]);
-        dz = dz + min_image((double)__uint_as_float(pd.z) - (double)row.z, v.per_len[2]);
+        dx = dx + min_image((double)__uint_as_float(pd.x) - (double)row.x, v.per_len[0], v.half_per[0]);
+        dy = dy + min_image((double)__uint_as_float(pd.y) - (double)row.y, v.per_len[1], v.half_per[1]);
+        dz = dz + min_image((double)__uint_as_float(pd.z) - (double)row.z, v.per_len[2], v.half_per[2]);
         hs = (hs ^ (uint64_t)hd.y) * FNV_PRIME;
         t = t + dwell_time(hd.x, __hiloint2double((int)hd.w, (int)hd.z), r);
         ++nd;
