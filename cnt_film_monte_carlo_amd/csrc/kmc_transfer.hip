@@ -9,8 +9,9 @@
 //                 |Q_if|^2 / (L_i L_f) * Lorentzian(E_i - E_f),
 // since M = |Q J| / sqrt(L_i L_f) (exciton_transfer.cpp:415) and J depends on the two states only
 // through their momenta.  Device: J_ab = sum_i sum_j u_a(i) v_b(j) / |R_i - R_j| over the A-site
-// unit cells of both tubes (exciton_transfer.cpp:307-377), one workgroup per geometry, the
-// acceptor atoms staged in LDS tiles after the shift / rotation of make_Ru_3d.
+// unit cells of both tubes (exciton_transfer.cpp:307-377), one workgroup per geometry, each donor
+// atom moved into the acceptor's frame (the inverse of make_Ru_3d's shift / rotation) so that the
+// acceptor atoms and their phases are read unchanged for every geometry.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -30,7 +31,7 @@ namespace {
 
 constexpr int CHUNK = 16;     // acceptor momenta per kernel pass (W_b held in registers)
 constexpr int MAXTERMS = 256; // nonzero C_ab terms per pass (one owning thread each)
-constexpr int TILE = 128;     // acceptor atoms per LDS tile
+constexpr int GEOM_BATCH = 65536;  // geometries per launch (one workgroup each)
 
 struct ex_state {
   int ik_cm_idx, n;
@@ -96,103 +97,118 @@ void tube_atoms(const cnt_model& c, std::vector<double>& r3, std::vector<vec2>& 
 // ---------------------------------------------------------------------------------------------
 struct exact_args {
   int32_t na, nb;                 // atoms per tube
-  int32_t nmb;                    // acceptor momenta of this pass (padded to the kernel's NMB)
+  int32_t nmb;                    // acceptor momenta of this pass (the kernel's NMB)
   int32_t n_terms;                // nonzero (a, b) terms of this pass
   int32_t accumulate;             // 0: out = pass sum; 1: out += pass sum
   const double4* donor;           // [na] centred positions (w unused)
-  const double4* acceptor;        // [nb]
   const double2* u;               // [nma][na]  exp(-i k_a . r)
-  const double2* v;               // [nmb][nb]  exp(+i k_b . r) of this pass's momenta
+  const double* rec;              // [nb][4 + 2 nmb] acceptor records of this pass
   const int32_t* term_a;          // [n_terms] donor momentum slot
   const int32_t* term_b;          // [n_terms] acceptor momentum slot within the pass
   const double* term_c;           // [n_terms] C_ab
   int64_t n_geom;
   const double *theta, *z, *a1, *a2;
   const int64_t* sel;             // optional: geometry g reads entry sel[g] and writes out[sel[g]]
+  const double2* trig;            // [n_geom] (cos, sin) of geometry g's theta
   double* out;
   double clamp_lo[3], clamp_hi[3];  // z, a1, a2 box
 };
 
+// 1/sqrt(x): the hardware estimate y (relative error about 2^-23) and one third-order step of the
+// binomial series, 1/sqrt(x) = y (1 - e)^(-1/2) = y (1 + e/2 + 3 e^2/8 + ...) with e = 1 - x y^2;
+// the dropped 5 e^3/16 term is below 2^-60, so the result is within rounding of FP64 (5 VALU
+// operations where two Newton steps take 7).
 __device__ __forceinline__ double rsqrt_f64(double x) {
-  double y = __builtin_amdgcn_rsq(x);
-  y = y * (1.5 - 0.5 * x * y * y);
-  y = y * (1.5 - 0.5 * x * y * y);
-  return y;
+  const double y = __builtin_amdgcn_rsq(x);
+  const double e = fma(-(x * y), y, 1.0);
+  return fma(y * e, fma(0.375, e, 0.5), y);
 }
 
-// One workgroup of 256 threads per geometry.  Donor atoms go in chunks of 256 (one per
-// thread); for its atom i a thread accumulates W_b(i) = sum_j v_b(j) / |R_i - R_j| over
-// acceptor tiles staged in LDS (positions after make_Ru_3d's shift and rotation).  The chunk's
-// W goes to LDS and thread t < n_terms adds sum_i u_a(i) W_b(i) of its term (a, b) to J_ab.
+// Wave-wide sum; every lane ends with the same value.
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+  return x;
+}
+
+// One workgroup of 256 threads per geometry.  Donor atoms go in chunks of 256 (one per thread).
+// make_Ru_3d places the acceptor at R_j = Rz(theta) (b_j + s2 y) + z_s z and the donor at
+// r_i + s1 y; since Rz leaves z and lengths alone, |R_i - R_j| = |q_i - b_j| with the donor atom
+// taken into the acceptor's frame, q_i = Rz(-theta) (r_i + s1 y - z_s z) - s2 y.  The acceptor
+// records (b_j and v_b(j) of the pass's momenta) are then the same for every geometry and are
+// read with wave-uniform addresses (scalar loads): nothing is staged in LDS and the per-pair cost
+// is the distance, one reciprocal square root and 2 NMB FMAs on the VALU.  For its atom i a
+// thread accumulates W_b(i) = sum_j v_b(j) / |q_i - b_j|; the chunk's share of
+// J_ab = sum_i u_a(i) W_b(i) is then summed per term across each wave and the four wave sums
+// are added by the term's owning thread (t < n_terms).
 template <int NMB>
-__global__ __launch_bounds__(256) void k_exact_rate(exact_args a) {
-  __shared__ double4 s_pos[TILE];
-  __shared__ double2 s_v[NMB][TILE];
-  __shared__ double2 s_w[NMB][256];
+__global__ __launch_bounds__(256) void k_exact_rate(exact_args a, const double* __restrict__ rec,
+                                                    const int32_t* __restrict__ term_a,
+                                                    const int32_t* __restrict__ term_b) {
+  constexpr int S = 4 + 2 * NMB;   // doubles per acceptor record: x, y, z, 0, (Re, Im) v_b
+  __shared__ double2 s_part[4][MAXTERMS];
   __shared__ double s_red[4];
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   for (int64_t g = blockIdx.x; g < a.n_geom; g += gridDim.x) {
     const int64_t e = a.sel ? a.sel[g] : g;
-    const double th = a.theta[e];
     const double zs = fmin(fmax(a.z[e], a.clamp_lo[0]), a.clamp_hi[0]);
     const double s1 = fmin(fmax(a.a1[e], a.clamp_lo[1]), a.clamp_hi[1]);
     const double s2 = fmin(fmax(a.a2[e], a.clamp_lo[2]), a.clamp_hi[2]);
-    double sth, cth;
-    sincos(th, &sth, &cth);
+    const double cth = a.trig[g].x, sth = a.trig[g].y;
     const bool owner = tid < a.n_terms;
-    const int ta = owner ? a.term_a[tid] : 0, tb = owner ? a.term_b[tid] : 0;
     double jr = 0.0, ji = 0.0;
     for (int i0 = 0; i0 < a.na; i0 += 256) {
       const int i = i0 + tid;
       const bool live = i < a.na;
-      double4 ri = live ? a.donor[i] : make_double4(0, 0, 0, 0);
-      ri.y += s1;
+      const double4 ri = live ? a.donor[i] : make_double4(0, 0, 0, 0);
+      const double ry = ri.y + s1;
+      const double qx = cth * ri.x + sth * ry;
+      const double qy = cth * ry - sth * ri.x - s2;
+      const double qz = ri.z - zs;
       double wr[NMB], wi[NMB];
 #pragma unroll
       for (int q = 0; q < NMB; q++) wr[q] = wi[q] = 0.0;
-      for (int j0 = 0; j0 < a.nb; j0 += TILE) {
-        __syncthreads();
-        if (tid < TILE && j0 + tid < a.nb) {
-          const int j = j0 + tid;
-          const double4 b = a.acceptor[j];
-          const double by = b.y + s2;
-          s_pos[tid] = make_double4(b.x * cth - by * sth, b.x * sth + by * cth, b.z + zs, 0.0);
+      // the next record's position is fetched with this record's v_b (one zero record follows
+      // the last), so each pair waits on the scalar loads once
+      const double* b = rec;
+      double bx = b[0], by = b[1], bz = b[2];
+      for (int j = 0; j < a.nb; j++) {
+        const double dx = qx - bx, dy = qy - by, dz = qz - bz;
+        const double r = rsqrt_f64(dx * dx + dy * dy + dz * dz);
+        const double* nxt = b + S;
+        bx = nxt[0];
+        by = nxt[1];
+        bz = nxt[2];
 #pragma unroll
-          for (int q = 0; q < NMB; q++) s_v[q][tid] = a.v[(size_t)q * a.nb + j];
+        for (int q = 0; q < NMB; q++) {
+          wr[q] += b[4 + 2 * q] * r;
+          wi[q] += b[5 + 2 * q] * r;
         }
-        __syncthreads();
-        const int tn = min(TILE, a.nb - j0);
-        if (live) {
-          for (int t = 0; t < tn; t++) {
-            const double4 p = s_pos[t];
-            const double dx = ri.x - p.x, dy = ri.y - p.y, dz = ri.z - p.z;
-            const double r = rsqrt_f64(dx * dx + dy * dy + dz * dz);
+        b = nxt;
+      }
+      // this chunk's u_a(i) W_b(i), summed over the wave for every term (a, b)
 #pragma unroll
-            for (int q = 0; q < NMB; q++) {
-              wr[q] += s_v[q][t].x * r;
-              wi[q] += s_v[q][t].y * r;
-            }
-          }
+      for (int q = 0; q < NMB; q++) {
+        const double w_r = live ? wr[q] : 0.0, w_i = live ? wi[q] : 0.0;
+        for (int t = 0; t < a.n_terms; t++) {
+          if (term_b[t] != q) continue;
+          const double2 u = live ? a.u[(size_t)term_a[t] * a.na + i] : make_double2(0, 0);
+          const double pr = wave_sum(u.x * w_r - u.y * w_i);
+          const double pi = wave_sum(u.x * w_i + u.y * w_r);
+          if (lane == 0) s_part[wave][t] = make_double2(pr, pi);
         }
       }
-#pragma unroll
-      for (int q = 0; q < NMB; q++) s_w[q][tid] = live ? make_double2(wr[q], wi[q]) : make_double2(0, 0);
       __syncthreads();
       if (owner) {
-        const int cn = min(256, a.na - i0);
-        const double2* ua = a.u + (size_t)ta * a.na + i0;
-        for (int k = 0; k < cn; k++) {
-          const double2 uu = ua[k];
-          const double2 w = s_w[tb][k];
-          jr += uu.x * w.x - uu.y * w.y;
-          ji += uu.x * w.y + uu.y * w.x;
-        }
+        jr += (s_part[0][tid].x + s_part[1][tid].x) + (s_part[2][tid].x + s_part[3][tid].x);
+        ji += (s_part[0][tid].y + s_part[1][tid].y) + (s_part[2][tid].y + s_part[3][tid].y);
       }
+      __syncthreads();
     }
     // rate = sum over terms of C_ab |J_ab|^2
     double part = owner ? a.term_c[tid] * (jr * jr + ji * ji) : 0.0;
     for (int off = 32; off > 0; off >>= 1) part += __shfl_down(part, off, 64);
-    if ((tid & 63) == 0) s_red[tid >> 6] = part;
+    if (lane == 0) s_red[wave] = part;
     __syncthreads();
     if (tid == 0) {
       const double sum = (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]);
@@ -202,7 +218,17 @@ __global__ __launch_bounds__(256) void k_exact_rate(exact_args a) {
   }
 }
 
-int padded_nmb(int nmb) { return nmb <= 2 ? 2 : nmb <= 4 ? 4 : nmb <= 8 ? 8 : nmb <= 12 ? 12 : 16; }
+int pass_width(int nmb) { return std::max(1, std::min(nmb, CHUNK)); }
+
+// cos and sin of each geometry's theta, ahead of the rate passes (sincos' argument reduction would
+// otherwise set the register budget, and with it the occupancy, of every rate kernel).
+__global__ void k_trig(int64_t n, const int64_t* sel, const double* theta, double2* trig) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  double s, c;
+  sincos(theta[sel ? sel[g] : g], &s, &c);
+  trig[g] = make_double2(c, s);
+}
 
 // Geometries whose transfer has no matched state pair: the rate is exactly zero.
 __global__ void k_zero_rates(int64_t n, const int64_t* sel, double* out) {
@@ -212,13 +238,15 @@ __global__ void k_zero_rates(int64_t n, const int64_t* sel, double* out) {
 
 // One pass over a chunk of at most 16 acceptor momenta.
 hipError_t launch_pass(const exact_args& a, hipStream_t st) {
-  const unsigned grid = (unsigned)std::min<int64_t>(a.n_geom, 65536);
+  const unsigned grid = (unsigned)std::min<int64_t>(a.n_geom, GEOM_BATCH);
+  const dim3 g(grid), b(256);
   switch (a.nmb) {
-    case 2: hipLaunchKernelGGL(k_exact_rate<2>, dim3(grid), dim3(256), 0, st, a); break;
-    case 4: hipLaunchKernelGGL(k_exact_rate<4>, dim3(grid), dim3(256), 0, st, a); break;
-    case 8: hipLaunchKernelGGL(k_exact_rate<8>, dim3(grid), dim3(256), 0, st, a); break;
-    case 12: hipLaunchKernelGGL(k_exact_rate<12>, dim3(grid), dim3(256), 0, st, a); break;
-    case 16: hipLaunchKernelGGL(k_exact_rate<16>, dim3(grid), dim3(256), 0, st, a); break;
+#define EXACT_CASE(n) \
+  case n: hipLaunchKernelGGL(k_exact_rate<n>, g, b, 0, st, a, a.rec, a.term_a, a.term_b); break;
+    EXACT_CASE(1) EXACT_CASE(2) EXACT_CASE(3) EXACT_CASE(4) EXACT_CASE(5) EXACT_CASE(6)
+    EXACT_CASE(7) EXACT_CASE(8) EXACT_CASE(9) EXACT_CASE(10) EXACT_CASE(11) EXACT_CASE(12)
+    EXACT_CASE(13) EXACT_CASE(14) EXACT_CASE(15) EXACT_CASE(16)
+#undef EXACT_CASE
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -240,15 +268,19 @@ struct kmc_transfer {
   std::vector<int> mom_a, mom_b;            // ik_cm of each momentum slot
   std::vector<int32_t> term_a, term_b;      // nonzero terms
   std::vector<double> term_c;
-  struct pass { int b0, nmb_pad, t0, nt; };  // acceptor slots [b0, b0 + 16), terms [t0, t0 + nt)
+  // acceptor slots [b0, b0 + 16), terms [t0, t0 + nt), records at h_rec[rec0]
+  struct pass { int b0, width, t0, nt; size_t rec0; };
   std::vector<pass> passes;
   double clamp_lo[3] = {-1e300, -1e300, -1e300}, clamp_hi[3] = {1e300, 1e300, 1e300};
   std::vector<double> h_donor, h_acc;
   std::vector<vec2> r2d, r2a;
-  std::vector<cplx> h_u, h_v;
+  std::vector<cplx> h_u;
+  std::vector<double> h_rec;                // per pass: [nb][4 + 2 width] acceptor records
   bool on_device = false;
-  double4 *d_donor = nullptr, *d_acc = nullptr;
-  double2 *d_u = nullptr, *d_v = nullptr;
+  double4* d_donor = nullptr;
+  double2* d_u = nullptr;
+  double* d_rec = nullptr;
+  double2* d_trig = nullptr;                // [GEOM_BATCH] (cos, sin) of the batch's geometries
   int32_t *d_ta = nullptr, *d_tb = nullptr;
   double* d_tc = nullptr;
   double* d_geo = nullptr;  // scratch for host-array calls: 4 n doubles
@@ -272,9 +304,9 @@ int upload(kmc_transfer* t) {
   if (e == hipSuccess) e = hipEventCreate(&t->ev0);
   if (e == hipSuccess) e = hipEventCreate(&t->ev1);
   if (e == hipSuccess) e = put((double**)&t->d_donor, t->h_donor);
-  if (e == hipSuccess) e = put((double**)&t->d_acc, t->h_acc);
   if (e == hipSuccess) e = put((cplx**)&t->d_u, t->h_u);
-  if (e == hipSuccess) e = put((cplx**)&t->d_v, t->h_v);
+  if (e == hipSuccess) e = put(&t->d_rec, t->h_rec);
+  if (e == hipSuccess) e = hipMalloc((void**)&t->d_trig, GEOM_BATCH * sizeof(double2));
   if (e == hipSuccess) e = put(&t->d_ta, t->term_a);
   if (e == hipSuccess) e = put(&t->d_tb, t->term_b);
   if (e == hipSuccess) e = put(&t->d_tc, t->term_c);
@@ -290,8 +322,8 @@ exact_args args_of(const kmc_transfer* t) {
   a.na = t->info.n_atoms_donor;
   a.nb = t->info.n_atoms_acceptor;
   a.donor = t->d_donor;
-  a.acceptor = t->d_acc;
   a.u = t->d_u;
+  a.trig = t->d_trig;
   for (int k = 0; k < 3; k++) {
     a.clamp_lo[k] = t->clamp_lo[k];
     a.clamp_hi[k] = t->clamp_hi[k];
@@ -300,7 +332,8 @@ exact_args args_of(const kmc_transfer* t) {
 }
 
 // rate = sum over passes of sum_{terms of the pass} C_ab |J_ab|^2; the passes run in order on
-// one stream, the first writes and the rest add.
+// one stream, the first writes and the rest add.  Geometries go in batches of GEOM_BATCH, each
+// with its (cos, sin) table computed first.
 hipError_t launch_exact(const kmc_transfer* t, exact_args a, hipStream_t st) {
   if (a.n_geom == 0) return hipSuccess;
   if (t->passes.empty()) {
@@ -309,16 +342,32 @@ hipError_t launch_exact(const kmc_transfer* t, exact_args a, hipStream_t st) {
     return hipGetLastError();
   }
   hipError_t e = hipSuccess;
-  for (size_t k = 0; k < t->passes.size() && e == hipSuccess; k++) {
-    const auto& p = t->passes[k];
-    a.nmb = p.nmb_pad;
-    a.n_terms = p.nt;
-    a.accumulate = k > 0;
-    a.v = t->d_v + (size_t)p.b0 * a.nb;
-    a.term_a = t->d_ta + p.t0;
-    a.term_b = t->d_tb + p.t0;
-    a.term_c = t->d_tc + p.t0;
-    e = launch_pass(a, st);
+  for (int64_t g0 = 0; g0 < a.n_geom && e == hipSuccess; g0 += GEOM_BATCH) {
+    exact_args b = a;
+    b.n_geom = std::min<int64_t>(GEOM_BATCH, a.n_geom - g0);
+    if (a.sel) {
+      b.sel = a.sel + g0;
+    } else {
+      b.theta += g0;
+      b.z += g0;
+      b.a1 += g0;
+      b.a2 += g0;
+      b.out += g0;
+    }
+    hipLaunchKernelGGL(k_trig, dim3((unsigned)((b.n_geom + 255) / 256)), dim3(256), 0, st, b.n_geom,
+                       b.sel, b.theta, t->d_trig);
+    e = hipGetLastError();
+    for (size_t k = 0; k < t->passes.size() && e == hipSuccess; k++) {
+      const auto& p = t->passes[k];
+      b.nmb = p.width;
+      b.n_terms = p.nt;
+      b.accumulate = k > 0;
+      b.rec = t->d_rec + p.rec0;
+      b.term_a = t->d_ta + p.t0;
+      b.term_b = t->d_tb + p.t0;
+      b.term_c = t->d_tc + p.t0;
+      e = launch_pass(b, st);
+    }
   }
   return e;
 }
@@ -389,7 +438,7 @@ int kmc_transfer_create(int32_t device, const kmc_cnt* donor, const kmc_cnt* acc
     if (x.c == 0.0) continue;
     const int b0 = x.q / CHUNK * CHUNK;
     if (t->passes.empty() || t->passes.back().b0 != b0)
-      t->passes.push_back({b0, padded_nmb(std::min(CHUNK, nmb - b0)), (int)t->term_c.size(), 0});
+      t->passes.push_back({b0, pass_width(std::min(CHUNK, nmb - b0)), (int)t->term_c.size(), 0, 0});
     if (++t->passes.back().nt > MAXTERMS) {
       delete t;
       return tfail(KMC_ERR_CAPACITY, "kmc_transfer_create: more than 256 momentum terms "
@@ -399,7 +448,7 @@ int kmc_transfer_create(int32_t device, const kmc_cnt* donor, const kmc_cnt* acc
     t->term_b.push_back(x.q - b0);
     t->term_c.push_back(x.c);
   }
-  const int v_rows = t->passes.empty() ? 0 : t->passes.back().b0 + t->passes.back().nmb_pad;
+  const int v_rows = t->passes.empty() ? 0 : t->passes.back().b0 + t->passes.back().width;
   kmc_transfer_info& in = t->info;
   in.n_donor_states = (int)ds.size();
   in.n_acceptor_states = (int)as.size();
@@ -415,7 +464,7 @@ int kmc_transfer_create(int32_t device, const kmc_cnt* donor, const kmc_cnt* acc
   tube_atoms(D, t->h_donor, t->r2d);
   tube_atoms(A, t->h_acc, t->r2a);
   t->h_u.assign((size_t)nma * t->r2d.size(), cplx(0, 0));
-  t->h_v.assign((size_t)std::max(v_rows, nmb) * t->r2a.size(), cplx(0, 0));
+  std::vector<cplx> h_v((size_t)std::max(v_rows, nmb) * t->r2a.size(), cplx(0, 0));
   for (int p = 0; p < nma; p++) {
     const double kx = t->mom_a[p] * D.dk_l.x, ky = t->mom_a[p] * D.dk_l.y;
     for (size_t i = 0; i < t->r2d.size(); i++)
@@ -424,7 +473,21 @@ int kmc_transfer_create(int32_t device, const kmc_cnt* donor, const kmc_cnt* acc
   for (int q = 0; q < nmb; q++) {
     const double kx = t->mom_b[q] * A.dk_l.x, ky = t->mom_b[q] * A.dk_l.y;
     for (size_t j = 0; j < t->r2a.size(); j++)
-      t->h_v[q * t->r2a.size() + j] = std::exp(+cplx(0, 1) * (kx * t->r2a[j].x + ky * t->r2a[j].y));
+      h_v[q * t->r2a.size() + j] = std::exp(+cplx(0, 1) * (kx * t->r2a[j].x + ky * t->r2a[j].y));
+  }
+  // acceptor records of each pass: position, 0, then v_b(j) of the pass's (padded) momenta
+  const size_t nb = t->r2a.size();
+  for (auto& p : t->passes) {
+    p.rec0 = t->h_rec.size();
+    for (size_t j = 0; j < nb; j++) {
+      for (int k = 0; k < 3; k++) t->h_rec.push_back(t->h_acc[4 * j + k]);
+      t->h_rec.push_back(0.0);
+      for (int q = 0; q < p.width; q++) {
+        t->h_rec.push_back(h_v[(size_t)(p.b0 + q) * nb + j].real());
+        t->h_rec.push_back(h_v[(size_t)(p.b0 + q) * nb + j].imag());
+      }
+    }
+    t->h_rec.resize(t->h_rec.size() + 4 + 2 * p.width, 0.0);  // read ahead by the last pair
   }
   *out = t;
   return KMC_OK;
@@ -435,7 +498,7 @@ int kmc_transfer_destroy(kmc_transfer* t) {
   if (t->on_device) {
     (void)hipSetDevice(t->device);
     if (t->stream) (void)hipStreamSynchronize(t->stream);
-    void* bufs[] = {t->d_donor, t->d_acc, t->d_u, t->d_v, t->d_ta, t->d_tb, t->d_tc, t->d_geo, t->d_out};
+    void* bufs[] = {t->d_donor, t->d_u, t->d_rec, t->d_trig, t->d_ta, t->d_tb, t->d_tc, t->d_geo, t->d_out};
     for (void* b : bufs)
       if (b) (void)hipFree(b);
     if (t->ev0) (void)hipEventDestroy(t->ev0);
