@@ -102,7 +102,7 @@ struct exact_args {
   int32_t accumulate;             // 0: out = pass sum; 1: out += pass sum
   const double4* donor;           // [na] centred positions (w unused)
   const double2* u;               // [nma][na]  exp(-i k_a . r)
-  const double* rec;              // [nb][4 + 2 nmb] acceptor records of this pass
+  const double* rec;              // acceptor records of this pass: 4 + nb (2 nmb + 4) doubles
   const int32_t* term_a;          // [n_terms] donor momentum slot
   const int32_t* term_b;          // [n_terms] acceptor momentum slot within the pass
   const double* term_c;           // [n_terms] C_ab
@@ -145,7 +145,7 @@ template <int NMB>
 __global__ __launch_bounds__(256) void k_exact_rate(exact_args a, const double* __restrict__ rec,
                                                     const int32_t* __restrict__ term_a,
                                                     const int32_t* __restrict__ term_b) {
-  constexpr int S = 4 + 2 * NMB;   // doubles per acceptor record: x, y, z, 0, (Re, Im) v_b
+  constexpr int S = 4 + 2 * NMB;   // doubles per acceptor record: (Re, Im) v_b, next x, y, z, 0
   __shared__ double2 s_part[4][MAXTERMS];
   __shared__ double s_red[4];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -168,23 +168,35 @@ __global__ __launch_bounds__(256) void k_exact_rate(exact_args a, const double* 
       double wr[NMB], wi[NMB];
 #pragma unroll
       for (int q = 0; q < NMB; q++) wr[q] = wi[q] = 0.0;
-      // the next record's position is fetched with this record's v_b (one zero record follows
-      // the last), so each pair waits on the scalar loads once
-      const double* b = rec;
-      double bx = b[0], by = b[1], bz = b[2];
+      // record j carries v_b(j) and the position of atom j + 1 (the first position leads the
+      // array), so each pair's scalar loads are issued together and waited on once
+      double bx = rec[0], by = rec[1], bz = rec[2];
+      asm volatile("" : "+s"(bx), "+s"(by), "+s"(bz));  // loaded before the loop, not pending in it
+      const double* b = rec + 4;
       for (int j = 0; j < a.nb; j++) {
+        // the record's loads go out first and the distance and square root cover them (a
+        // scheduling fence keeps them there, a second one keeps the position hand-over after
+        // the FMAs so that the loads are waited on only there)
+        double v[2 * NMB];
+#pragma unroll
+        for (int k = 0; k < 2 * NMB; k++) v[k] = b[k];
+        const double nx = b[2 * NMB], ny = b[2 * NMB + 1], nz = b[2 * NMB + 2];
+        __builtin_amdgcn_sched_barrier(0);
         const double dx = qx - bx, dy = qy - by, dz = qz - bz;
-        const double r = rsqrt_f64(dx * dx + dy * dy + dz * dz);
-        const double* nxt = b + S;
-        bx = nxt[0];
-        by = nxt[1];
-        bz = nxt[2];
+        const double r = rsqrt_f64(fma(dz, dz, fma(dy, dy, dx * dx)));
 #pragma unroll
         for (int q = 0; q < NMB; q++) {
-          wr[q] += b[4 + 2 * q] * r;
-          wi[q] += b[5 + 2 * q] * r;
+          wr[q] += v[2 * q] * r;
+          wi[q] += v[2 * q + 1] * r;
         }
-        b = nxt;
+        __builtin_amdgcn_sched_barrier(0);
+        bx = nx;
+        by = ny;
+        bz = nz;
+        // carry the next position in SGPRs (the compiler would otherwise reload it at the top
+        // of the next pair)
+        asm volatile("" : "+s"(bx), "+s"(by), "+s"(bz));
+        b += S;
       }
       // this chunk's u_a(i) W_b(i), summed over the wave for every term (a, b)
 #pragma unroll
@@ -275,7 +287,7 @@ struct kmc_transfer {
   std::vector<double> h_donor, h_acc;
   std::vector<vec2> r2d, r2a;
   std::vector<cplx> h_u;
-  std::vector<double> h_rec;                // per pass: [nb][4 + 2 width] acceptor records
+  std::vector<double> h_rec;                // per pass: 4 + nb (2 width + 4) doubles of acceptor records
   bool on_device = false;
   double4* d_donor = nullptr;
   double2* d_u = nullptr;
@@ -475,19 +487,24 @@ int kmc_transfer_create(int32_t device, const kmc_cnt* donor, const kmc_cnt* acc
     for (size_t j = 0; j < t->r2a.size(); j++)
       h_v[q * t->r2a.size() + j] = std::exp(+cplx(0, 1) * (kx * t->r2a[j].x + ky * t->r2a[j].y));
   }
-  // acceptor records of each pass: position, 0, then v_b(j) of the pass's (padded) momenta
+  // acceptor records of each pass: the first atom's position (x, y, z, 0), then for every atom
+  // j its v_b(j) over the pass's momenta followed by the position of atom j + 1 (zeros after the
+  // last atom)
   const size_t nb = t->r2a.size();
+  auto put_pos = [&](size_t j) {
+    for (int k = 0; k < 3; k++) t->h_rec.push_back(j < nb ? t->h_acc[4 * j + k] : 0.0);
+    t->h_rec.push_back(0.0);
+  };
   for (auto& p : t->passes) {
     p.rec0 = t->h_rec.size();
+    put_pos(0);
     for (size_t j = 0; j < nb; j++) {
-      for (int k = 0; k < 3; k++) t->h_rec.push_back(t->h_acc[4 * j + k]);
-      t->h_rec.push_back(0.0);
       for (int q = 0; q < p.width; q++) {
         t->h_rec.push_back(h_v[(size_t)(p.b0 + q) * nb + j].real());
         t->h_rec.push_back(h_v[(size_t)(p.b0 + q) * nb + j].imag());
       }
+      put_pos(j + 1);
     }
-    t->h_rec.resize(t->h_rec.size() + 4 + 2 * p.width, 0.0);  // read ahead by the last pair
   }
   *out = t;
   return KMC_OK;
