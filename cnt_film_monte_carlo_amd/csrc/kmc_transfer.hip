@@ -32,6 +32,12 @@ namespace {
 constexpr int CHUNK = 16;     // acceptor momenta per kernel pass (W_b held in registers)
 constexpr int MAXTERMS = 256; // nonzero C_ab terms per pass (one owning thread each)
 constexpr int GEOM_BATCH = 65536;  // geometries per launch (one workgroup each)
+// Launches of fewer geometries than this (the surrogate's exact fallback, single-geometry calls)
+// give each (geometry, 256-atom donor chunk) its own workgroup, so that the GPU holds enough waves
+// to cover the scalar-load latency; the chunks' shares of J go through PART_CAP entries of device
+// memory and a finishing kernel.
+constexpr int64_t SPLIT_BELOW = 4096;
+constexpr size_t PART_CAP = size_t(1) << 21;
 
 struct ex_state {
   int ik_cm_idx, n;
@@ -110,6 +116,8 @@ struct exact_args {
   const double *theta, *z, *a1, *a2;
   const int64_t* sel;             // optional: geometry g reads entry sel[g] and writes out[sel[g]]
   const double2* trig;            // [n_geom] (cos, sin) of geometry g's theta
+  int32_t split;                  // 1: workgroup (g, blockIdx.y) does donor chunk blockIdx.y only
+  double2* part;                  // split: [n_geom][chunks][n_terms] shares of J_ab
   double* out;
   double clamp_lo[3], clamp_hi[3];  // z, a1, a2 box
 };
@@ -131,6 +139,22 @@ __device__ __forceinline__ double wave_sum(double x) {
   return x;
 }
 
+// rate = sum over terms of C_ab |J_ab|^2 (thread t < n_terms holds term t's J), written or added
+// to out[e] by thread 0.  Products are written as explicit FMAs here and in the J sums, so that
+// every kernel that shares this arithmetic rounds it the same way.
+__device__ __forceinline__ void finish_rate(const exact_args& a, int64_t e, bool owner, double jr,
+                                            double ji, double* s_red, int tid) {
+  double part = owner ? a.term_c[tid] * fma(ji, ji, jr * jr) : 0.0;
+  for (int off = 32; off > 0; off >>= 1) part += __shfl_down(part, off, 64);
+  if ((tid & 63) == 0) s_red[tid >> 6] = part;
+  __syncthreads();
+  if (tid == 0) {
+    const double sum = (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]);
+    a.out[e] = a.accumulate ? a.out[e] + sum : sum;
+  }
+  __syncthreads();
+}
+
 // One workgroup of 256 threads per geometry.  Donor atoms go in chunks of 256 (one per thread).
 // make_Ru_3d places the acceptor at R_j = Rz(theta) (b_j + s2 y) + z_s z and the donor at
 // r_i + s1 y; since Rz leaves z and lengths alone, |R_i - R_j| = |q_i - b_j| with the donor atom
@@ -140,7 +164,9 @@ __device__ __forceinline__ double wave_sum(double x) {
 // is the distance, one reciprocal square root and 2 NMB FMAs on the VALU.  For its atom i a
 // thread accumulates W_b(i) = sum_j v_b(j) / |q_i - b_j|; the chunk's share of
 // J_ab = sum_i u_a(i) W_b(i) is then summed per term across each wave and the four wave sums
-// are added by the term's owning thread (t < n_terms).
+// are added by the term's owning thread (t < n_terms).  Split launches run one chunk per workgroup
+// and leave each chunk's four-wave sum in part[] for k_exact_finish, which adds the chunks in the
+// same order, so both launch shapes give the same bits.
 template <int NMB>
 __global__ __launch_bounds__(256) void k_exact_rate(exact_args a, const double* __restrict__ rec,
                                                     const int32_t* __restrict__ term_a,
@@ -157,7 +183,9 @@ __global__ __launch_bounds__(256) void k_exact_rate(exact_args a, const double* 
     const double cth = a.trig[g].x, sth = a.trig[g].y;
     const bool owner = tid < a.n_terms;
     double jr = 0.0, ji = 0.0;
-    for (int i0 = 0; i0 < a.na; i0 += 256) {
+    const int i_lo = a.split ? 256 * (int)blockIdx.y : 0;
+    const int i_hi = a.split ? min(a.na, i_lo + 256) : a.na;
+    for (int i0 = i_lo; i0 < i_hi; i0 += 256) {
       const int i = i0 + tid;
       const bool live = i < a.na;
       const double4 ri = live ? a.donor[i] : make_double4(0, 0, 0, 0);
@@ -205,28 +233,43 @@ __global__ __launch_bounds__(256) void k_exact_rate(exact_args a, const double* 
         for (int t = 0; t < a.n_terms; t++) {
           if (term_b[t] != q) continue;
           const double2 u = live ? a.u[(size_t)term_a[t] * a.na + i] : make_double2(0, 0);
-          const double pr = wave_sum(u.x * w_r - u.y * w_i);
-          const double pi = wave_sum(u.x * w_i + u.y * w_r);
+          const double pr = wave_sum(fma(u.x, w_r, -(u.y * w_i)));
+          const double pi = wave_sum(fma(u.x, w_i, u.y * w_r));
           if (lane == 0) s_part[wave][t] = make_double2(pr, pi);
         }
       }
       __syncthreads();
       if (owner) {
-        jr += (s_part[0][tid].x + s_part[1][tid].x) + (s_part[2][tid].x + s_part[3][tid].x);
-        ji += (s_part[0][tid].y + s_part[1][tid].y) + (s_part[2][tid].y + s_part[3][tid].y);
+        const double pr = (s_part[0][tid].x + s_part[1][tid].x) + (s_part[2][tid].x + s_part[3][tid].x);
+        const double pi = (s_part[0][tid].y + s_part[1][tid].y) + (s_part[2][tid].y + s_part[3][tid].y);
+        if (a.split) {
+          a.part[((size_t)g * gridDim.y + blockIdx.y) * a.n_terms + tid] = make_double2(pr, pi);
+        } else {
+          jr += pr;
+          ji += pi;
+        }
       }
       __syncthreads();
     }
-    // rate = sum over terms of C_ab |J_ab|^2
-    double part = owner ? a.term_c[tid] * (jr * jr + ji * ji) : 0.0;
-    for (int off = 32; off > 0; off >>= 1) part += __shfl_down(part, off, 64);
-    if (lane == 0) s_red[wave] = part;
-    __syncthreads();
-    if (tid == 0) {
-      const double sum = (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]);
-      a.out[e] = a.accumulate ? a.out[e] + sum : sum;
-    }
-    __syncthreads();
+    if (!a.split) finish_rate(a, e, owner, jr, ji, s_red, tid);
+  }
+}
+
+// Split launches: J_ab = sum over the chunks' shares in chunk order, then the rate.
+__global__ __launch_bounds__(256) void k_exact_finish(exact_args a, int n_chunks) {
+  __shared__ double s_red[4];
+  const int tid = threadIdx.x;
+  for (int64_t g = blockIdx.x; g < a.n_geom; g += gridDim.x) {
+    const int64_t e = a.sel ? a.sel[g] : g;
+    const bool owner = tid < a.n_terms;
+    double jr = 0.0, ji = 0.0;
+    if (owner)
+      for (int c = 0; c < n_chunks; c++) {
+        const double2 p = a.part[((size_t)g * n_chunks + c) * a.n_terms + tid];
+        jr += p.x;
+        ji += p.y;
+      }
+    finish_rate(a, e, owner, jr, ji, s_red, tid);
   }
 }
 
@@ -251,7 +294,8 @@ __global__ void k_zero_rates(int64_t n, const int64_t* sel, double* out) {
 // One pass over a chunk of at most 16 acceptor momenta.
 hipError_t launch_pass(const exact_args& a, hipStream_t st) {
   const unsigned grid = (unsigned)std::min<int64_t>(a.n_geom, GEOM_BATCH);
-  const dim3 g(grid), b(256);
+  const int chunks = (a.na + 255) / 256;
+  const dim3 g(grid, a.split ? chunks : 1), b(256);
   switch (a.nmb) {
 #define EXACT_CASE(n) \
   case n: hipLaunchKernelGGL(k_exact_rate<n>, g, b, 0, st, a, a.rec, a.term_a, a.term_b); break;
@@ -260,6 +304,11 @@ hipError_t launch_pass(const exact_args& a, hipStream_t st) {
     EXACT_CASE(13) EXACT_CASE(14) EXACT_CASE(15) EXACT_CASE(16)
 #undef EXACT_CASE
     default: return hipErrorInvalidValue;
+  }
+  if (a.split) {
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_exact_finish, dim3(grid), b, 0, st, a, chunks);
   }
   return hipGetLastError();
 }
@@ -293,6 +342,7 @@ struct kmc_transfer {
   double2* d_u = nullptr;
   double* d_rec = nullptr;
   double2* d_trig = nullptr;                // [GEOM_BATCH] (cos, sin) of the batch's geometries
+  double2* d_part = nullptr;                // [PART_CAP] chunk shares of J for split launches
   int32_t *d_ta = nullptr, *d_tb = nullptr;
   double* d_tc = nullptr;
   double* d_geo = nullptr;  // scratch for host-array calls: 4 n doubles
@@ -319,6 +369,7 @@ int upload(kmc_transfer* t) {
   if (e == hipSuccess) e = put((cplx**)&t->d_u, t->h_u);
   if (e == hipSuccess) e = put(&t->d_rec, t->h_rec);
   if (e == hipSuccess) e = hipMalloc((void**)&t->d_trig, GEOM_BATCH * sizeof(double2));
+  if (e == hipSuccess) e = hipMalloc((void**)&t->d_part, PART_CAP * sizeof(double2));
   if (e == hipSuccess) e = put(&t->d_ta, t->term_a);
   if (e == hipSuccess) e = put(&t->d_tb, t->term_b);
   if (e == hipSuccess) e = put(&t->d_tc, t->term_c);
@@ -336,6 +387,7 @@ exact_args args_of(const kmc_transfer* t) {
   a.donor = t->d_donor;
   a.u = t->d_u;
   a.trig = t->d_trig;
+  a.part = t->d_part;
   for (int k = 0; k < 3; k++) {
     a.clamp_lo[k] = t->clamp_lo[k];
     a.clamp_hi[k] = t->clamp_hi[k];
@@ -344,8 +396,8 @@ exact_args args_of(const kmc_transfer* t) {
 }
 
 // rate = sum over passes of sum_{terms of the pass} C_ab |J_ab|^2; the passes run in order on
-// one stream, the first writes and the rest add.  Geometries go in batches of GEOM_BATCH, each
-// with its (cos, sin) table computed first.
+// one stream, the first writes and the rest add.  Geometries go in batches (GEOM_BATCH, or what
+// the chunk shares of a split launch fit in PART_CAP), each with its (cos, sin) table first.
 hipError_t launch_exact(const kmc_transfer* t, exact_args a, hipStream_t st) {
   if (a.n_geom == 0) return hipSuccess;
   if (t->passes.empty()) {
@@ -353,10 +405,16 @@ hipError_t launch_exact(const kmc_transfer* t, exact_args a, hipStream_t st) {
     hipLaunchKernelGGL(k_zero_rates, dim3(grid), dim3(256), 0, st, a.n_geom, a.sel, a.out);
     return hipGetLastError();
   }
+  int max_terms = 1;
+  for (const auto& p : t->passes) max_terms = std::max(max_terms, p.nt);
+  const size_t per_geom = (size_t)((a.na + 255) / 256) * max_terms;  // chunk shares per geometry
+  a.split = a.n_geom < SPLIT_BELOW && per_geom <= PART_CAP;
+  const int64_t batch = a.split ? std::min<int64_t>(GEOM_BATCH, (int64_t)(PART_CAP / per_geom))
+                                : GEOM_BATCH;
   hipError_t e = hipSuccess;
-  for (int64_t g0 = 0; g0 < a.n_geom && e == hipSuccess; g0 += GEOM_BATCH) {
+  for (int64_t g0 = 0; g0 < a.n_geom && e == hipSuccess; g0 += batch) {
     exact_args b = a;
-    b.n_geom = std::min<int64_t>(GEOM_BATCH, a.n_geom - g0);
+    b.n_geom = std::min<int64_t>(batch, a.n_geom - g0);
     if (a.sel) {
       b.sel = a.sel + g0;
     } else {
@@ -515,7 +573,7 @@ int kmc_transfer_destroy(kmc_transfer* t) {
   if (t->on_device) {
     (void)hipSetDevice(t->device);
     if (t->stream) (void)hipStreamSynchronize(t->stream);
-    void* bufs[] = {t->d_donor, t->d_u, t->d_rec, t->d_trig, t->d_ta, t->d_tb, t->d_tc, t->d_geo, t->d_out};
+    void* bufs[] = {t->d_donor, t->d_u, t->d_rec, t->d_trig, t->d_part, t->d_ta, t->d_tb, t->d_tc, t->d_geo, t->d_out};
     for (void* b : bufs)
       if (b) (void)hipFree(b);
     if (t->ev0) (void)hipEventDestroy(t->ev0);

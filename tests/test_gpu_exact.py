@@ -45,6 +45,27 @@ def test_transfer_matches_literal_first_order(models, orc, pair, T, geoms):
         assert got[k] == pytest.approx(ref, rel=1e-9, abs=1e-300), (k, got[k], ref)
 
 
+@pytest.mark.parametrize("pair,T", [(((7, 5), (8, 6)), 300.0), (((7, 5), (9, 7)), 400.0)])
+def test_split_launch_carries_the_same_bits(models, pair, T):
+    # fewer than 4096 geometries: one workgroup per (geometry, 256-atom donor chunk) and a
+    # finishing kernel; 4096 or more: one workgroup per geometry.  Same sums in the same order,
+    # so the same bits (the second pair runs two passes of acceptor momenta: 16 + 3).
+    from cnt_film_monte_carlo_amd import exciton as X
+    t = X.Transfer(models[pair[0]], models[pair[1]], T)
+    rng = np.random.default_rng(3)
+    n = 4200
+    th = rng.uniform(0, np.pi, n)
+    z = rng.uniform(1.5e-9, 10e-9, n)
+    a1 = rng.uniform(-10e-9, 10e-9, n)
+    a2 = rng.uniform(-10e-9, 10e-9, n)
+    say(f"split vs whole launches {pair} T={T}: {n} geometries")
+    whole = t.rates(th, z, a1, a2)
+    idx = rng.choice(n, 37, replace=False)
+    part = t.rates(th[idx], z[idx], a1[idx], a2[idx])
+    assert np.all(whole > 0)
+    assert np.array_equal(part, whole[idx])
+
+
 def test_no_matched_pairs_gives_zero(models):
     # (9,7) donor to (7,5) acceptor at 150 K: no state pair within the Lorentzian window
     from cnt_film_monte_carlo_amd import exciton as X
