@@ -615,41 +615,43 @@ struct count_cache_lds {
     vic[1] = make_uint4(0u, 0u, 0u, 0u);
   }
   __device__ __forceinline__ void add(uint32_t s, uint32_t* __restrict__ base) {
-    if (s == s0) {
-      ++n0;
-    } else if (s == s1) {
-      const uint32_t t = n1;
-      s1 = s0; n1 = n0; s0 = s; n0 = t + 1u;
-    } else if (s == s2) {
-      const uint32_t t = n2;
-      s2 = s1; n2 = n1; s1 = s0; n1 = n0; s0 = s; n0 = t + 1u;
-    } else if (s == s3) {
-      const uint32_t t = n3;
-      s3 = s2; n3 = n2; s2 = s1; n2 = n1; s1 = s0; n1 = n0; s0 = s; n0 = t + 1u;
-    } else {
-      // (s3, n3) leaves the registers for the front of the victim entries; s comes back from
-      // them with its count if it is there, else the oldest victim entry goes to the device
+    // register entries: selects instead of branches (lanes of a wave hit different entries); the
+    // entries above the hit move down one, a miss moves all four and (s3, n3) to the victims
+    const bool h0 = s == s0, h1 = s == s1, h2 = s == s2, h3 = s == s3;
+    uint32_t c = h0 ? n0 : h1 ? n1 : h2 ? n2 : n3;
+    const bool m1 = !h0, m2 = m1 && !h1, m3 = m2 && !h2;
+    const uint32_t out_s = s3, out_n = n3;
+    s3 = m3 ? s2 : s3;
+    n3 = m3 ? n2 : n3;
+    s2 = m2 ? s1 : s2;
+    n2 = m2 ? n1 : n2;
+    s1 = m1 ? s0 : s1;
+    n1 = m1 ? n0 : n1;
+    if (m3 && !h3) {
+      // (out_s, out_n) leaves the registers for the front of the victim entries; s comes back
+      // from them with its count if it is there, else the oldest victim entry goes to the device
       uint4 vs = vic[0], vc = vic[1];
-      uint32_t c = 0;
+      c = 0;
       if (vs.x == s) {
         c = vc.x;
-        vs.x = s3; vc.x = n3;
+        vs.x = out_s; vc.x = out_n;
       } else if (vs.y == s) {
         c = vc.y;
-        vs.y = vs.x; vc.y = vc.x; vs.x = s3; vc.x = n3;
+        vs.y = vs.x; vc.y = vc.x; vs.x = out_s; vc.x = out_n;
       } else if (vs.z == s) {
         c = vc.z;
-        vs.z = vs.y; vc.z = vc.y; vs.y = vs.x; vc.y = vc.x; vs.x = s3; vc.x = n3;
+        vs.z = vs.y; vc.z = vc.y; vs.y = vs.x; vc.y = vc.x; vs.x = out_s; vc.x = out_n;
       } else {
         if (vs.w == s) c = vc.w;
         else if (vc.w) atomicAdd(base + vs.w, vc.w);
         vs.w = vs.z; vc.w = vc.z; vs.z = vs.y; vc.z = vc.y; vs.y = vs.x; vc.y = vc.x;
-        vs.x = s3; vc.x = n3;
+        vs.x = out_s; vc.x = out_n;
       }
       vic[0] = vs;
       vic[1] = vc;
-      s3 = s2; n3 = n2; s2 = s1; n2 = n1; s1 = s0; n1 = n0; s0 = s; n0 = c + 1u;
     }
+    s0 = s;
+    n0 = c + 1u;
   }
   __device__ __forceinline__ void flush(uint32_t* __restrict__ base) {
     if (n0) atomicAdd(base + s0, n0);
