@@ -1,6 +1,8 @@
-// kmc_ingest.hip -- host-side site ingest of the KMC path: the mesh-file reader
-// (create_scatterers), the trim to a box (trim_scats) and the Kubo injection / removal regions
-// (injection_region, get_removal_domain).  No device work.
+// kmc_ingest.hip -- host-side file and site ingest of the KMC path: the mesh-file reader
+// (create_scatterers), the trim to a box (trim_scats), the Kubo injection / removal regions
+// (injection_region, get_removal_domain) and the rate-table files (scattering_struct::save and
+// a reader for them).  No device work.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -69,9 +71,136 @@ int read_matrix(const std::string& path, std::vector<double>& v, int64_t& rows, 
   return KMC_OK;
 }
 
+// Every number of a text file, in order (blanks, tabs, commas and line ends separate them).
+int read_numbers(const std::string& path, const std::string& text, size_t from,
+                 std::vector<double>& v, const char* who) {
+  v.clear();
+  const char* p = text.c_str() + from;
+  const char* end = text.c_str() + text.size();
+  while (p < end) {
+    while (p < end && (*p == ' ' || *p == '\t' || *p == '\r' || *p == '\n' || *p == ',')) ++p;
+    if (p >= end) break;
+    char* q = nullptr;
+    const double d = std::strtod(p, &q);
+    if (q == p)
+      return ingest_fail(KMC_ERR_ARG, std::string(who) + ": bad number in " + path + " after " +
+                                          std::to_string(v.size()) + " values");
+    v.push_back(d);
+    p = q;
+  }
+  return KMC_OK;
+}
+
+int slurp(const std::string& path, std::string& text, const char* who) {
+  FILE* f = std::fopen(path.c_str(), "rb");
+  if (!f) return ingest_fail(KMC_ERR_ARG, std::string(who) + ": cannot open " + path);
+  text.clear();
+  char buf[1 << 16];
+  size_t got;
+  while ((got = std::fread(buf, 1, sizeof(buf), f)) > 0) text.append(buf, got);
+  std::fclose(f);
+  return KMC_OK;
+}
+
+const char* const kAxisFiles[4] = {".theta.dat", ".z_shift.dat", ".axis_shift_1.dat",
+                                   ".axis_shift_2.dat"};
+
 }  // namespace
 
 extern "C" {
+
+int kmc_save_rate_table(const char* dir, const char* name, int64_t n_theta, int64_t n_z,
+                        int64_t n_a1, int64_t n_a2, const double* theta, const double* z_shift,
+                        const double* axis_shift_1, const double* axis_shift_2,
+                        const double* rates) {
+  const char* who = "kmc_save_rate_table";
+  if (!dir || !theta || !z_shift || !axis_shift_1 || !axis_shift_2 || !rates)
+    return ingest_fail(KMC_ERR_ARG, std::string(who) + ": null argument");
+  if (n_theta < 1 || n_z < 1 || n_a1 < 1 || n_a2 < 1)
+    return ingest_fail(KMC_ERR_ARG, std::string(who) + ": empty axis");
+  const std::string base = std::string(dir) + "/" + (name && *name ? name : "scat_table");
+  const int64_t dims[4] = {n_theta, n_z, n_a1, n_a2};
+  const double* axes[4] = {theta, z_shift, axis_shift_1, axis_shift_2};
+  // one value per line; 17 significant digits, so that reading back gives the same doubles
+  for (int k = 0; k < 4; ++k) {
+    FILE* f = std::fopen((base + kAxisFiles[k]).c_str(), "wb");
+    if (!f) return ingest_fail(KMC_ERR_ARG, std::string(who) + ": cannot write " + base + kAxisFiles[k]);
+    for (int64_t i = 0; i < dims[k]; ++i) std::fprintf(f, "%.17g\n", axes[k][i]);
+    if (std::fclose(f) != 0)
+      return ingest_fail(KMC_ERR_ARG, std::string(who) + ": write failed: " + base + kAxisFiles[k]);
+  }
+  // the four header lines and the theta-outer, axis_shift_2-inner order of
+  // scattering_struct::save (src/monte_carlo/scattering_struct.h:75-91)
+  FILE* f = std::fopen((base + ".rates.dat").c_str(), "wb");
+  if (!f) return ingest_fail(KMC_ERR_ARG, std::string(who) + ": cannot write " + base + ".rates.dat");
+  std::fprintf(f, "sizes:\ntheta, z_shift, axis_shift_1, axis_shift_2\n%lld,%lld,%lld,%lld\n\n",
+               (long long)n_theta, (long long)n_z, (long long)n_a1, (long long)n_a2);
+  const int64_t total = n_theta * n_z * n_a1 * n_a2;
+  for (int64_t i = 0; i < total; ++i) std::fprintf(f, "%.17g\n", rates[i]);
+  if (std::fclose(f) != 0)
+    return ingest_fail(KMC_ERR_ARG, std::string(who) + ": write failed: " + base + ".rates.dat");
+  return KMC_OK;
+}
+
+int kmc_load_rate_table(const char* dir, const char* name, int64_t dims[4], double* theta,
+                        double* z_shift, double* axis_shift_1, double* axis_shift_2,
+                        double* rates) {
+  const char* who = "kmc_load_rate_table";
+  if (!dir || !dims) return ingest_fail(KMC_ERR_ARG, std::string(who) + ": null argument");
+  const std::string base = std::string(dir) + "/" + (name && *name ? name : "scat_table");
+  std::string text;
+  int st = slurp(base + ".rates.dat", text, who);
+  if (st != KMC_OK) return st;
+  // header: "sizes:", the axis names, "n_theta,n_z,n_a1,n_a2", a blank line
+  // (scattering_struct.h:76-79; the reference's reader skips four rows,
+  // visualization/monte_carlo_results.py:264-270)
+  size_t pos = 0;
+  std::string line[4];
+  for (int k = 0; k < 4; ++k) {
+    const size_t e = text.find('\n', pos);
+    if (e == std::string::npos)
+      return ingest_fail(KMC_ERR_ARG, std::string(who) + ": short header in " + base + ".rates.dat");
+    line[k] = text.substr(pos, e - pos);
+    pos = e + 1;
+  }
+  long long d[4];
+  if (line[0].rfind("sizes:", 0) != 0 ||
+      std::sscanf(line[2].c_str(), "%lld , %lld , %lld , %lld", &d[0], &d[1], &d[2], &d[3]) != 4)
+    return ingest_fail(KMC_ERR_ARG, std::string(who) + ": bad header in " + base + ".rates.dat");
+  for (int k = 0; k < 4; ++k)
+    if (d[k] < 1 || d[k] > (1 << 24))
+      return ingest_fail(KMC_ERR_ARG, std::string(who) + ": bad size in " + base + ".rates.dat");
+  const bool fill = theta != nullptr;
+  if (fill)
+    for (int k = 0; k < 4; ++k)
+      if (dims[k] != d[k])
+        return ingest_fail(KMC_ERR_CAPACITY, std::string(who) + ": table sizes differ from dims");
+  for (int k = 0; k < 4; ++k) dims[k] = d[k];
+  if (!fill) return KMC_OK;  // size query
+  if (!z_shift || !axis_shift_1 || !axis_shift_2 || !rates)
+    return ingest_fail(KMC_ERR_ARG, std::string(who) + ": null output array");
+  std::vector<double> v;
+  st = read_numbers(base + ".rates.dat", text, pos, v, who);
+  if (st != KMC_OK) return st;
+  const int64_t total = d[0] * d[1] * d[2] * d[3];
+  if ((int64_t)v.size() != total)
+    return ingest_fail(KMC_ERR_ARG, std::string(who) + ": " + base + ".rates.dat holds " +
+                                        std::to_string(v.size()) + " rates, the sizes give " +
+                                        std::to_string(total));
+  std::copy(v.begin(), v.end(), rates);
+  double* axes[4] = {theta, z_shift, axis_shift_1, axis_shift_2};
+  for (int k = 0; k < 4; ++k) {
+    st = slurp(base + kAxisFiles[k], text, who);
+    if (st == KMC_OK) st = read_numbers(base + kAxisFiles[k], text, 0, v, who);
+    if (st != KMC_OK) return st;
+    if ((int64_t)v.size() != d[k])
+      return ingest_fail(KMC_ERR_ARG, std::string(who) + ": " + base + kAxisFiles[k] + " holds " +
+                                          std::to_string(v.size()) + " values, the header gives " +
+                                          std::to_string(d[k]));
+    std::copy(v.begin(), v.end(), axes[k]);
+  }
+  return KMC_OK;
+}
 
 int kmc_read_mesh(const char* dir, int64_t capacity, int64_t* n_tubes, int64_t* n_points,
                   double* x, double* y, double* z, double* ox, double* oy, double* oz,

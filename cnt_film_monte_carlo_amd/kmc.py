@@ -100,6 +100,9 @@ _SIGS = {
     "kmc_forster_table": (C.c_int, [C.c_double, C.c_int32, _D, C.c_int32, _D, C.c_int32, _D,
                                     C.c_int32, _D, _D]),
     "kmc_linspace": (C.c_int, [C.c_double, C.c_double, C.c_int32, _D]),
+    "kmc_save_rate_table": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int64, C.c_int64, C.c_int64,
+                                      C.c_int64, _D, _D, _D, _D, _D]),
+    "kmc_load_rate_table": (C.c_int, [C.c_char_p, C.c_char_p, _I64, _D, _D, _D, _D, _D]),
     "kmc_read_mesh": (C.c_int, [C.c_char_p, C.c_int64, _I64, _I64, _D, _D, _D, _D, _D, _D, _I32,
                                 _I32]),
     "kmc_trim_sites": (C.c_int, [_I64, _D, _D, _D, _D, _D, _D, _I32, _I32, _U8, _U8, _I64, _D, _D,
@@ -605,6 +608,34 @@ def forster_table(gamma_0: float, theta, z_shift, a1, a2) -> np.ndarray:
                                        _ptr(g[1], _D), g[2].size, _ptr(g[2], _D), g[3].size,
                                        _ptr(g[3], _D), _ptr(out, _D)))
     return out
+
+
+def save_rate_table(directory, theta, z_shift, a1, a2, rates, name: str = "scat_table"):
+    """Writes one rate table as scattering_struct::save lays it out
+    (src/monte_carlo/scattering_struct.h:55-94): <name>.{theta,z_shift,axis_shift_1,
+    axis_shift_2,rates}.dat in `directory`, with 17 significant digits so that load_rate_table
+    gives back the same doubles.  rates has shape (n_theta, n_z, n_a1, n_a2)."""
+    g = [_c(v, np.float64) for v in (theta, z_shift, a1, a2)]
+    r = _c(rates, np.float64)
+    if r.shape != tuple(v.size for v in g):
+        raise ValueError(f"rates shape {r.shape} does not match the axes {tuple(v.size for v in g)}")
+    _check(library().kmc_save_rate_table(str(directory).encode(), name.encode(),
+                                         *[v.size for v in g], *[_ptr(v, _D) for v in g],
+                                         _ptr(r, _D)))
+
+
+def load_rate_table(directory, name: str = "scat_table"):
+    """(theta, z_shift, a1, a2, rates) read from the files save_rate_table (or the reference's
+    scattering_struct::save) wrote; rates has shape (n_theta, n_z, n_a1, n_a2)."""
+    lib = library()
+    d, nm = str(directory).encode(), name.encode()
+    dims = np.zeros(4, np.int64)
+    _check(lib.kmc_load_rate_table(d, nm, _ptr(dims, _I64), None, None, None, None, None))
+    axes = [np.zeros(int(n)) for n in dims]
+    rates = np.zeros(tuple(int(n) for n in dims))
+    _check(lib.kmc_load_rate_table(d, nm, _ptr(dims, _I64), *[_ptr(v, _D) for v in axes],
+                                   _ptr(rates, _D)))
+    return (*axes, rates)
 
 
 def estimate_diffusion(reduced: np.ndarray, t_sample_max: float, k_lo: int = 0,

@@ -264,6 +264,27 @@ int kmc_forster_table(double gamma_0, int32_t n_theta, const double* theta, int3
 /* arma::linspace as used for the table grids (monte_carlo.cpp:66-75).  Host only. */
 int kmc_linspace(double start, double end, int32_t n, double* out);
 
+/* Rate-table files of scattering_struct::save (src/monte_carlo/scattering_struct.h:55-94):
+ * <dir>/<name>.theta.dat, .z_shift.dat, .axis_shift_1.dat, .axis_shift_2.dat (one value per
+ * line) and <name>.rates.dat (the lines "sizes:", the axis names, "n_theta,n_z,n_a1,n_a2", a
+ * blank line, then one rate per line, theta outermost, axis_shift_2 innermost).  name NULL or
+ * "" means "scat_table", the reference's.  Values are written with 17 significant digits, so a
+ * save and a load give back the same doubles; the reference itself writes 6 (rates) and
+ * Armadillo's 4-digit vector format (axes), which the loader reads too.  rates is
+ * [n_theta][n_z][n_a1][n_a2], as kmc_set_rate_table takes one table.  Host only. */
+int kmc_save_rate_table(const char* dir, const char* name, int64_t n_theta, int64_t n_z,
+                        int64_t n_a1, int64_t n_a2, const double* theta, const double* z_shift,
+                        const double* axis_shift_1, const double* axis_shift_2,
+                        const double* rates);
+/* Reads the files above.  With theta == NULL: fills dims[4] = {n_theta, n_z, n_a1, n_a2} and
+ * returns (size query).  Otherwise dims must hold the file's sizes (KMC_ERR_CAPACITY if not)
+ * and every array is filled.  The reference has no reader (its tables are never reloaded);
+ * this one accepts what visualization/monte_carlo_results.py:261-282 reads.  Errors (missing
+ * file, bad header, counts that disagree with the header) read with kmc_last_error(NULL). */
+int kmc_load_rate_table(const char* dir, const char* name, int64_t dims[4], double* theta,
+                        double* z_shift, double* axis_shift_1, double* axis_shift_2,
+                        double* rates);
+
 /* ---- CNT exciton model (host only; no device work) ------------------------------------
  * Restates cnt::calculate_exciton_dispersion (src/exciton_transfer/cnt.cpp:1056-1081):
  * get_parameters (:16-117), get_atom_coordinates (:120-209), electron_energy in the
