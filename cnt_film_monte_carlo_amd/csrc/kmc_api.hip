@@ -126,6 +126,25 @@ struct kmc_ctx {
   dbuf<int32_t> tr_site;
   dbuf<double> tr_time;
   dbuf<int64_t> tr_n;
+
+  // contact (steady-state) mode
+  bool c_on = false;
+  int32_t c_nseg = 0;
+  int64_t c_pop[2] = {0, 0}, c_cap = 0;
+  double c_ymin = 0.0, c_dy = 0.0, c_time = 0.0;
+  double c_rm_lo[2] = {0, 0}, c_rm_hi[2] = {0, 0};  // repopulate ranges of the two contacts
+  std::vector<double> c_area;
+  int64_t c_list_off[2] = {0, 0}, c_list_n[2] = {0, 0};
+  dbuf<int32_t> c_sites;                  // the two contacts' site lists, then the sections'
+  dbuf<int32_t> c_site[2];
+  dbuf<double> c_t[2];
+  dbuf<uint64_t> c_hops[2];
+  dbuf<uint32_t> c_id[2];
+  int c_buf = 0;                          // the buffer holding the live particles
+  dbuf<uint32_t> c_keep;
+  dbuf<int64_t> c_off, c_scan_tmp, c_live, c_live_out;
+  dbuf<uint64_t> c_next;
+  dbuf<int32_t> c_err, c_pop_out, c_cur_out;
 };
 
 namespace {
@@ -427,6 +446,11 @@ int kmc_destroy(kmc_ctx* c) {
   dfree(c->x_heading); dfree(c->x_soff); dfree(c->x_tlast); dfree(c->d_links); dfree(c->d_fly_err);
   dfree(c->d_site_count); dfree(c->r_dx); dfree(c->r_dy); dfree(c->r_dz);
   dfree(c->d_reduced); dfree(c->r_drain); dfree(c->tr_site); dfree(c->tr_time); dfree(c->tr_n);
+  dfree(c->c_sites); dfree(c->c_keep); dfree(c->c_off); dfree(c->c_scan_tmp); dfree(c->c_live);
+  dfree(c->c_live_out); dfree(c->c_next); dfree(c->c_err); dfree(c->c_pop_out); dfree(c->c_cur_out);
+  for (int b = 0; b < 2; ++b) {
+    dfree(c->c_site[b]); dfree(c->c_t[b]); dfree(c->c_hops[b]); dfree(c->c_id[b]);
+  }
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -448,6 +472,7 @@ int kmc_set_sites(kmc_ctx* c, int64_t n, const double* x, const double* y, const
   if (n <= 0 || n >= (int64_t)EDGE_COL_MASK || !x || !y || !z || !ox || !oy || !oz)
     return fail(c, KMC_ERR_ARG, "kmc_set_sites: need 0 < n < 2^31 and non-null x,y,z,ox,oy,oz");
   KMC_HIP(c, hipSetDevice(c->cfg.device));
+  c->c_on = false;  // a contact run's sections and lists belong to the old sites
   const double* pin[3] = {x, y, z};
   const double* oin[3] = {ox, oy, oz};
   for (int a = 0; a < 3; ++a) {
@@ -1535,6 +1560,255 @@ int kmc_ensemble_autotune(kmc_ensemble* m, int64_t trial_hops, int32_t n_cand, c
   if (st != KMC_OK) return efail(st, "kmc_ensemble_autotune: a trial failed");
   m->resort_hops = pick;
   if (best) *best = pick;
+  return KMC_OK;
+}
+
+}  // extern "C"
+
+// =============================================================================================
+// Contact (steady-state) mode: init() / step() / save_metrics() / repopulate_contacts()
+// (src/monte_carlo/monte_carlo.h:157-195, 274-316, 343-355, 443-684), velocity 0.
+// =============================================================================================
+namespace {
+
+contact_slots contact_buf(kmc_ctx* c, int b) {
+  return contact_slots{c->c_site[b].p, c->c_t[b].p, c->c_hops[b].p, c->c_id[b].p};
+}
+
+contact_args contact_args_of(kmc_ctx* c) {
+  contact_args a{};
+  a.blocks = c->d_blocks.p;
+  a.roff = c->d_roff.p;
+  a.py = c->d_pos[1].p;
+  a.seed = c->cfg.seed;
+  a.s = contact_buf(c, c->c_buf);
+  a.n_live = c->c_live.p;
+  a.capacity = c->c_cap;
+  a.n_seg = c->c_nseg;
+  a.ymin = c->c_ymin;
+  a.dy = c->c_dy;
+  a.err = c->c_err.p;
+  return a;
+}
+
+int contact_check_err(kmc_ctx* c, const char* who) {
+  int32_t err = 0;
+  KMC_HIP(c, hipMemcpy(&err, c->c_err.p, sizeof(err), hipMemcpyDeviceToHost));
+  if (err & CONTACT_ERR_HOPS)
+    return fail(c, KMC_ERR_CAPACITY, std::string(who) + ": a particle reached max_hops_per_step "
+                                         "within one step (shorten dt)");
+  if (err & CONTACT_ERR_CAPACITY)
+    return fail(c, KMC_ERR_CAPACITY, std::string(who) + ": more particles than the capacity");
+  if (err & CONTACT_ERR_IDS)
+    return fail(c, KMC_ERR_CAPACITY, std::string(who) + ": more than 2^32 particles injected");
+  return KMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kmc_contact_init(kmc_ctx* c, int32_t n_seg, int64_t c1_pop, int64_t c2_pop, int64_t capacity) {
+#pragma clang fp contract(off)
+  const char* who = "kmc_contact_init";
+  if (!c) return fail(nullptr, KMC_ERR_ARG, "kmc_contact_init: null context");
+  if (c->csr_state != 2 || !c->d_blocks.p)
+    return fail(c, KMC_ERR_STATE, "kmc_contact_init: CSR not complete");
+  if (c->velocity != 0.0)
+    return fail(c, KMC_ERR_STATE, "kmc_contact_init: the contact mode runs with velocity 0 only");
+  if (n_seg < 2 || n_seg > CONTACT_MAX_SEG)
+    return fail(c, KMC_ERR_ARG, "kmc_contact_init: need 2 <= n_seg <= 1024");
+  if (c1_pop < 0 || c2_pop < 0 || capacity <= 0 || capacity > (int64_t)0x7FFFFFFF)
+    return fail(c, KMC_ERR_ARG, "kmc_contact_init: need populations >= 0, 0 < capacity < 2^31");
+  KMC_HIP(c, hipSetDevice(c->cfg.device));
+  const std::vector<double>& X = c->h_pos[0];
+  const std::vector<double>& Y = c->h_pos[1];
+  const std::vector<double>& Z = c->h_pos[2];
+  const int64_t N = c->n_sites;
+  // the domain is the sites' bounding box (find_simulation_domain, monte_carlo.h:328-340)
+  const double ymin = c->bb_lo[1], ymax = c->bb_hi[1];
+  const double dy = (ymax - ymin) / double(n_seg);
+  // get_area (monte_carlo.h:644-684), with its min-else-max update
+  std::vector<double> xmax(n_seg, c->bb_lo[0]), xmin(n_seg, c->bb_hi[0]);
+  std::vector<double> zmax(n_seg, c->bb_lo[2]), zmin(n_seg, c->bb_hi[2]);
+  for (int64_t s = 0; s < N; ++s) {
+    int i = (int)((Y[s] - ymin) / dy);
+    i = i < 0 ? 0 : (i < n_seg ? i : n_seg - 1);
+    if (xmin[i] > X[s]) xmin[i] = X[s];
+    else if (xmax[i] < X[s]) xmax[i] = X[s];
+    if (zmin[i] > Z[s]) zmin[i] = Z[s];
+    else if (zmax[i] < Z[s]) zmax[i] = Z[s];
+  }
+  c->c_area.assign(n_seg, 0.0);
+  for (int i = 0; i < n_seg; ++i) c->c_area[i] = (zmax[i] - zmin[i]) * (xmax[i] - xmin[i]);
+  // site lists: contact_scats(i = 1, n_seg) (monte_carlo.h:494-516, closed bounds), then the
+  // sections of create_particles (monte_carlo.h:274-316, [y1, y1 + dy))
+  std::vector<int32_t> lists;
+  std::vector<int64_t> off, cnt;
+  for (int ci : {1, (int)n_seg}) {
+    const double y1 = ymin + double(ci - 1) * dy;
+    const double y2 = ymin + double(ci) * dy;
+    off.push_back((int64_t)lists.size());
+    for (int64_t s = 0; s < N; ++s)
+      if (Y[s] >= y1 && Y[s] <= y2) lists.push_back((int32_t)s);
+    cnt.push_back((int64_t)lists.size() - off.back());
+  }
+  const double dp = double(c2_pop - c1_pop) / (double(n_seg) - 1);
+  std::vector<int64_t> n_init(n_seg);
+  int64_t total = 0;
+  for (int i = 0; i < n_seg; ++i) {
+    n_init[i] = (int64_t)std::round(double(c1_pop) + double(i) * dp);
+    const double y1 = ymin + double(i) * dy;
+    const double y2 = y1 + dy;
+    off.push_back((int64_t)lists.size());
+    for (int64_t s = 0; s < N; ++s)
+      if (y1 <= Y[s] && Y[s] < y2) lists.push_back((int32_t)s);
+    cnt.push_back((int64_t)lists.size() - off.back());
+    if (n_init[i] > 0 && cnt.back() == 0)
+      return fail(c, KMC_ERR_ARG, "kmc_contact_init: section " + std::to_string(i) +
+                                      " has no sites for its initial particles");
+    total += n_init[i];
+  }
+  if ((c1_pop > 0 && cnt[0] == 0) || (c2_pop > 0 && cnt[1] == 0))
+    return fail(c, KMC_ERR_ARG, "kmc_contact_init: a contact holding particles has no sites");
+  if (total > capacity || c1_pop + c2_pop > capacity)
+    return fail(c, KMC_ERR_CAPACITY, "kmc_contact_init: capacity below the populations");
+  c->c_nseg = n_seg;
+  c->c_pop[0] = c1_pop;
+  c->c_pop[1] = c2_pop;
+  c->c_cap = capacity;
+  c->c_ymin = ymin;
+  c->c_dy = dy;
+  c->c_time = 0.0;
+  // repopulate_contacts (monte_carlo.h:443-455): [ymin, ymin + dy] and [ymin + (n-1) dy, ymax]
+  c->c_rm_lo[0] = ymin;
+  c->c_rm_hi[0] = ymin + dy;
+  c->c_rm_lo[1] = ymin + double(n_seg - 1) * dy;
+  c->c_rm_hi[1] = ymax;
+  for (int k = 0; k < 2; ++k) {
+    c->c_list_off[k] = off[k];
+    c->c_list_n[k] = cnt[k];
+  }
+  KMC_TRY(dalloc(c, c->c_sites, std::max<size_t>(1, lists.size())));
+  if (!lists.empty())
+    KMC_HIP(c, hipMemcpy(c->c_sites.p, lists.data(), lists.size() * sizeof(int32_t),
+                         hipMemcpyHostToDevice));
+  for (int b = 0; b < 2; ++b) {
+    KMC_TRY(dalloc(c, c->c_site[b], capacity));
+    KMC_TRY(dalloc(c, c->c_t[b], capacity));
+    KMC_TRY(dalloc(c, c->c_hops[b], capacity));
+    KMC_TRY(dalloc(c, c->c_id[b], capacity));
+  }
+  KMC_TRY(dalloc(c, c->c_keep, capacity));
+  KMC_TRY(dalloc(c, c->c_off, capacity + 1));
+  KMC_TRY(dalloc(c, c->c_scan_tmp, scan_tmp_elems(capacity)));
+  KMC_TRY(dalloc(c, c->c_live, 1));
+  KMC_TRY(dalloc(c, c->c_next, 1));
+  KMC_TRY(dalloc(c, c->c_err, 1));
+  KMC_HIP(c, hipMemsetAsync(c->c_live.p, 0, sizeof(int64_t), c->stream));
+  KMC_HIP(c, hipMemsetAsync(c->c_next.p, 0, sizeof(uint64_t), c->stream));
+  KMC_HIP(c, hipMemsetAsync(c->c_err.p, 0, sizeof(int32_t), c->stream));
+  c->c_buf = 0;
+  // the initial linear profile, section by section (create_particles)
+  contact_args a = contact_args_of(c);
+  for (int i = 0; i < n_seg; ++i)
+    if (n_init[i] > 0)
+      KMC_HIP(c, launch_contact_inject(a, c->c_live.p, n_init[i], c->c_sites.p + off[2 + i],
+                                       cnt[2 + i], c->c_next.p, 0.0, c->c_live.p, c->stream));
+  KMC_HIP(c, hipStreamSynchronize(c->stream));
+  KMC_TRY(contact_check_err(c, who));
+  c->c_on = true;
+  return KMC_OK;
+}
+
+int kmc_contact_run(kmc_ctx* c, double dt, int64_t n_steps, uint64_t max_hops_per_step,
+                    int32_t* population, int32_t* crossings, int64_t* n_live) {
+#pragma clang fp contract(off)
+  const char* who = "kmc_contact_run";
+  if (!c) return fail(nullptr, KMC_ERR_ARG, "kmc_contact_run: null context");
+  if (!c->c_on) return fail(c, KMC_ERR_STATE, "kmc_contact_run: call kmc_contact_init first");
+  if (!(dt > 0.0) || !std::isfinite(dt) || n_steps < 0 || max_hops_per_step == 0)
+    return fail(c, KMC_ERR_ARG, "kmc_contact_run: need dt > 0, n_steps >= 0, max_hops > 0");
+  if (c->csr_state != 2 || !c->d_blocks.p)
+    return fail(c, KMC_ERR_STATE, "kmc_contact_run: CSR not complete");
+  if (n_steps == 0) return KMC_OK;
+  KMC_HIP(c, hipSetDevice(c->cfg.device));
+  const int64_t S = c->c_nseg;
+  KMC_TRY(dalloc(c, c->c_pop_out, (size_t)n_steps * S));
+  KMC_TRY(dalloc(c, c->c_cur_out, (size_t)n_steps * (S - 1)));
+  KMC_TRY(dalloc(c, c->c_live_out, (size_t)n_steps));
+  KMC_HIP(c, hipMemsetAsync(c->c_pop_out.p, 0, c->c_pop_out.n * sizeof(int32_t), c->stream));
+  KMC_HIP(c, hipMemsetAsync(c->c_cur_out.p, 0, c->c_cur_out.n * sizeof(int32_t), c->stream));
+  for (int64_t k = 0; k < n_steps; ++k) {
+    // monte_carlo::step, then save_metrics (main.cpp:95-97): every particle to the step's end
+    contact_args a = contact_args_of(c);
+    a.t_end = c->c_time + dt;
+    a.max_hops = max_hops_per_step;
+    a.pop = c->c_pop_out.p + k * S;
+    a.cur = c->c_cur_out.p + k * (S - 1);
+    KMC_HIP(c, launch_contact_step(a, c->stream));
+    c->c_time = a.t_end;  // _time += dt (monte_carlo.h:354)
+    // repopulate_contacts (monte_carlo.h:443-491): contact 1, then contact n_seg; the kept
+    // particles keep their order, the new ones follow them
+    for (int side = 0; side < 2; ++side) {
+      a.s = contact_buf(c, c->c_buf);
+      KMC_HIP(c, launch_contact_keep(a, c->c_rm_lo[side], c->c_rm_hi[side], c->c_keep.p,
+                                     c->stream));
+      KMC_HIP(c, exclusive_scan(c->c_keep.p, c->c_off.p, c->c_cap, c->c_scan_tmp.p, c->stream));
+      KMC_HIP(c, launch_contact_compact(c->c_cap, contact_buf(c, c->c_buf),
+                                        contact_buf(c, 1 - c->c_buf), c->c_keep.p, c->c_off.p,
+                                        c->stream));
+      c->c_buf = 1 - c->c_buf;
+      a.s = contact_buf(c, c->c_buf);
+      KMC_HIP(c, launch_contact_inject(a, c->c_off.p + c->c_cap, c->c_pop[side],
+                                       c->c_sites.p + c->c_list_off[side], c->c_list_n[side],
+                                       c->c_next.p, c->c_time, c->c_live.p, c->stream));
+    }
+    KMC_HIP(c, hipMemcpyAsync(c->c_live_out.p + k, c->c_live.p, sizeof(int64_t),
+                              hipMemcpyDeviceToDevice, c->stream));
+  }
+  KMC_HIP(c, hipStreamSynchronize(c->stream));
+  KMC_TRY(contact_check_err(c, who));
+  if (population)
+    KMC_HIP(c, hipMemcpy(population, c->c_pop_out.p, c->c_pop_out.n * sizeof(int32_t),
+                         hipMemcpyDeviceToHost));
+  if (crossings)
+    KMC_HIP(c, hipMemcpy(crossings, c->c_cur_out.p, c->c_cur_out.n * sizeof(int32_t),
+                         hipMemcpyDeviceToHost));
+  if (n_live)
+    KMC_HIP(c, hipMemcpy(n_live, c->c_live_out.p, (size_t)n_steps * sizeof(int64_t),
+                         hipMemcpyDeviceToHost));
+  return KMC_OK;
+}
+
+int kmc_contact_sections(const kmc_ctx* c, double* area, double* ymin, double* dy, double* time) {
+  if (!c) return fail(nullptr, KMC_ERR_ARG, "kmc_contact_sections: null context");
+  if (!c->c_on) return fail(const_cast<kmc_ctx*>(c), KMC_ERR_STATE,
+                            "kmc_contact_sections: call kmc_contact_init first");
+  if (area) std::copy(c->c_area.begin(), c->c_area.end(), area);
+  if (ymin) *ymin = c->c_ymin;
+  if (dy) *dy = c->c_dy;
+  if (time) *time = c->c_time;
+  return KMC_OK;
+}
+
+int kmc_contact_particles(kmc_ctx* c, int64_t capacity, int64_t* n, uint32_t* id, int32_t* site,
+                          double* t_next, uint64_t* hops) {
+  if (!c || !n) return fail(c, KMC_ERR_ARG, "kmc_contact_particles: null argument");
+  if (!c->c_on) return fail(c, KMC_ERR_STATE, "kmc_contact_particles: call kmc_contact_init first");
+  KMC_HIP(c, hipSetDevice(c->cfg.device));
+  int64_t live = 0;
+  KMC_HIP(c, hipMemcpy(&live, c->c_live.p, sizeof(live), hipMemcpyDeviceToHost));
+  *n = live;
+  if (!id && !site && !t_next && !hops) return KMC_OK;  // size query
+  if (capacity < live) return fail(c, KMC_ERR_CAPACITY, "kmc_contact_particles: capacity below n");
+  const int b = c->c_buf;
+  if (live > 0) {
+    if (id) KMC_HIP(c, hipMemcpy(id, c->c_id[b].p, live * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (site) KMC_HIP(c, hipMemcpy(site, c->c_site[b].p, live * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (t_next) KMC_HIP(c, hipMemcpy(t_next, c->c_t[b].p, live * sizeof(double), hipMemcpyDeviceToHost));
+    if (hops) KMC_HIP(c, hipMemcpy(hops, c->c_hops[b].p, live * sizeof(uint64_t), hipMemcpyDeviceToHost));
+  }
   return KMC_OK;
 }
 

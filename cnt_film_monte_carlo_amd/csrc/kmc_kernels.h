@@ -63,6 +63,32 @@ struct net_view {
   int32_t* fly_err;         // set when a flight exceeds FLY_MAX_PASSES site passes
 };
 
+// Contact (steady-state) mode: particle slots [0, *n_live) of one buffer.
+constexpr int CONTACT_MAX_SEG = 1024;
+enum : int32_t { CONTACT_ERR_HOPS = 1, CONTACT_ERR_CAPACITY = 2, CONTACT_ERR_IDS = 4 };
+struct contact_slots {
+  int32_t* site;
+  double* t_next;
+  uint64_t* hops;   // hops done by the particle (its draw counter)
+  uint32_t* id;     // the particle's draw key, unique over the run
+};
+struct contact_args {
+  const uint4* blocks;
+  const int64_t* roff;
+  const double* py;         // FP64 site y: sections and interface crossings
+  uint64_t seed;
+  contact_slots s;
+  const int64_t* n_live;
+  int64_t capacity;
+  double t_end;             // the step ends at t_end (hops at t <= t_end happen in it)
+  uint64_t max_hops;        // hops one particle may make in one step
+  int32_t n_seg;
+  double ymin, dy;
+  int32_t* pop;             // [n_seg] particles per section after the step
+  int32_t* cur;             // [n_seg - 1] net crossings of each interface during the step
+  int32_t* err;             // CONTACT_ERR_* bits
+};
+
 struct trace_view {         // trace [n_trace][tr_max] (single network only)
   int64_t n_trace = 0, tr_max = 0;
   int32_t* tr_site = nullptr;
@@ -152,5 +178,14 @@ hipError_t launch_hop_ens(const ens_args& a, bool count, bool sample, hipStream_
 hipError_t launch_ens_keys(int64_t n, const net_view* nets, const uint16_t* exc_net,
                            const int64_t* ebase, uint64_t* keys, uint32_t* ids, hipStream_t st);
 hipError_t launch_reduce(const reduce_args& a, hipStream_t st);
+hipError_t launch_contact_step(const contact_args& a, hipStream_t st);
+hipError_t launch_contact_keep(const contact_args& a, double y_lo, double y_hi, uint32_t* keep,
+                               hipStream_t st);
+hipError_t launch_contact_compact(int64_t capacity, const contact_slots& src,
+                                  const contact_slots& dst, const uint32_t* keep,
+                                  const int64_t* off, hipStream_t st);
+hipError_t launch_contact_inject(const contact_args& a, const int64_t* base, int64_t count,
+                                 const int32_t* sites, int64_t n_sites, uint64_t* next_id,
+                                 double t_now, int64_t* n_live, hipStream_t st);
 
 }  // namespace cntkmc

@@ -1260,6 +1260,145 @@ hipError_t launch_reduce(const reduce_args& a, hipStream_t st) {
 // ---------------------------------------------------------------------------------------------
 // launch wrappers for (a) and (b)
 // ---------------------------------------------------------------------------------------------
+// =============================================================================================
+// Contact (steady-state) mode with v = 0: monte_carlo::step, save_population_profile,
+// save_currents and repopulate_contacts (src/monte_carlo/monte_carlo.h:343-355, 443-516, 525-630).
+// A particle's draws are keyed by its id (unique over the run) and its own hop count, so slots
+// can be compacted and reused without any draw repeating.
+// =============================================================================================
+__device__ __forceinline__ int contact_section(double y, double ymin, double dy, int n_seg) {
+#pragma clang fp contract(off)
+  // save_population_profile (monte_carlo.h:567-568): truncation toward zero, then clamped
+  int i = (int)((y - ymin) / dy);
+  return i < 0 ? 0 : (i < n_seg ? i : n_seg - 1);
+}
+
+__global__ __launch_bounds__(256) void k_contact_step(contact_args a) {
+#pragma clang fp contract(off)
+  __shared__ int32_t s_pop[CONTACT_MAX_SEG], s_cur[CONTACT_MAX_SEG];
+  for (int i = threadIdx.x; i < a.n_seg; i += blockDim.x) s_pop[i] = s_cur[i] = 0;
+  __syncthreads();
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < *a.n_live) {
+    const uint32_t id = a.s.id[e];
+    double t = a.s.t_next[e];
+    uint64_t h = a.s.hops[e];
+    const int32_t site0 = a.s.site[e];
+    const uint32_t L0 = (uint32_t)a.roff[site0];
+    uint32_t R = row_word(L0, a.blocks[(size_t)ROW_UNITS * L0].x);
+    row_regs row = load_row(a.blocks, R);
+    // particle::step (src/monte_carlo/particle.cpp:57-80): every hop due by the end of the step
+    uint64_t made = 0;
+    while (t <= a.t_end && !trapped(row.n, row.gamma)) {
+      if (made == a.max_hops) {
+        atomicOr(a.err, CONTACT_ERR_HOPS);
+        break;
+      }
+      const u32x4 w = draw4(a.seed, id, h, STREAM_HOP);
+      const double dice = (row.gamma * (double)r31(w.v[0])) / RAND_MAX_D;
+      const uint32_t idx = select_entry(a.blocks, R, row, dice);
+      R = target_of(a.blocks, R, row, idx) & ~EDGE_DRAIN_BIT;
+      row = load_row(a.blocks, R);
+      const uint32_t r = nonzero_draw(w.v[1], w.v[3], a.seed, id, h, STREAM_HOP_REDRAW);
+      t = t + dwell_time(row.n, row.gamma, r);
+      ++h;
+      ++made;
+    }
+    a.s.site[e] = (int32_t)row.site;
+    a.s.t_next[e] = t;
+    a.s.hops[e] = h;
+    const double y_old = a.py[site0], y_new = a.py[row.site];
+    const int i_new = contact_section(y_new, a.ymin, a.dy, a.n_seg);
+    atomicAdd(&s_pop[i_new], 1);
+    // save_currents (monte_carlo.h:598-626): interface k at ymin + dy (k + 1); +1 for a crossing
+    // upward (old < y_k <= new), -1 downward.  Interfaces more than one section away from both
+    // positions cannot lie between them, so only those near the two sections are compared.
+    if (y_old != y_new) {
+      const int i_old = contact_section(y_old, a.ymin, a.dy, a.n_seg);
+      const int k_lo = max(0, min(i_old, i_new) - 1);
+      const int k_hi = min(a.n_seg - 2, max(i_old, i_new) + 1);
+      for (int k = k_lo; k <= k_hi; ++k) {
+        const double yk = a.ymin + a.dy * (double)(k + 1);
+        if (y_old < yk && y_new >= yk) atomicAdd(&s_cur[k], 1);
+        else if (y_old >= yk && y_new < yk) atomicAdd(&s_cur[k], -1);
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < a.n_seg; i += blockDim.x) {
+    if (s_pop[i]) atomicAdd(a.pop + i, s_pop[i]);
+    if (i + 1 < a.n_seg && s_cur[i]) atomicAdd(a.cur + i, s_cur[i]);
+  }
+}
+
+// repopulate (monte_carlo.h:461-474): a particle whose y lies in [y_lo, y_hi] leaves
+__global__ __launch_bounds__(256) void k_contact_keep(contact_args a, double y_lo, double y_hi,
+                                                      uint32_t* keep) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= a.capacity) return;
+  uint32_t k = 0;
+  if (e < *a.n_live) {
+    const double y = a.py[a.s.site[e]];
+    k = (y >= y_lo && y <= y_hi) ? 0u : 1u;
+  }
+  keep[e] = k;
+}
+
+// The kept particles, in slot order, to the front of the other buffer.
+__global__ __launch_bounds__(256) void k_contact_compact(int64_t capacity, contact_slots src,
+                                                         contact_slots dst,
+                                                         const uint32_t* __restrict__ keep,
+                                                         const int64_t* __restrict__ off) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= capacity || !keep[e]) return;
+  const int64_t o = off[e];
+  dst.site[o] = src.site[e];
+  dst.t_next[o] = src.t_next[e];
+  dst.hops[o] = src.hops[e];
+  dst.id[o] = src.id[e];
+}
+
+// `count` new particles in slots *base + k on sites of `sites`, as the particle constructor
+// makes them (particle.h:49-52): a random site of the list (rand() % size), the first dwell
+// from the site (ff_time).  Ids continue from *next_id.
+__global__ __launch_bounds__(256) void k_contact_inject(contact_args a, const int64_t* base,
+                                                        int64_t count,
+                                                        const int32_t* __restrict__ sites,
+                                                        int64_t n_sites,
+                                                        const uint64_t* __restrict__ next_id,
+                                                        double t_now) {
+#pragma clang fp contract(off)
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  const int64_t slot = *base + k;
+  const uint64_t id64 = *next_id + (uint64_t)k;
+  if (slot >= a.capacity) {
+    atomicOr(a.err, CONTACT_ERR_CAPACITY);
+    return;
+  }
+  if (id64 > 0xFFFFFFFFull) {
+    atomicOr(a.err, CONTACT_ERR_IDS);
+    return;
+  }
+  const uint32_t id = (uint32_t)id64;
+  const u32x4 w = draw4(a.seed, id, 0, STREAM_CONTACT);
+  const int32_t site = sites[r31(w.v[0]) % (uint32_t)n_sites];
+  const uint4 hh = a.blocks[(size_t)ROW_UNITS * a.roff[site]];
+  const uint32_t r = nonzero_draw(w.v[1], w.v[3], a.seed, id, 0, STREAM_CONTACT_REDRAW);
+  a.s.site[slot] = (int32_t)hh.y;
+  a.s.t_next[slot] = t_now + dwell_time(hh.x, __hiloint2double((int)hh.w, (int)hh.z), r);
+  a.s.hops[slot] = 0;
+  a.s.id[slot] = id;
+}
+
+__global__ void k_contact_finish(const int64_t* base, int64_t count, int64_t capacity,
+                                 uint64_t* next_id, int64_t* n_live) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int64_t n = *base + count;
+  *n_live = n < capacity ? n : capacity;
+  *next_id += (uint64_t)count;
+}
+
 static inline dim3 grid_for(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
 hipError_t launch_cell_assign(const site_arrays& s, const cell_desc& c, int32_t* cell_of,
@@ -1470,6 +1609,44 @@ hipError_t launch_sum_counts(int64_t n, int64_t stride, const uint32_t* copies, 
                              hipStream_t st) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_sum_counts, grid_for(n), dim3(256), 0, st, n, stride, copies, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_contact_step(const contact_args& a, hipStream_t st) {
+  if (a.capacity == 0) return hipSuccess;
+  if (a.n_seg < 1 || a.n_seg > CONTACT_MAX_SEG) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_contact_step, grid_for(a.capacity), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_contact_keep(const contact_args& a, double y_lo, double y_hi, uint32_t* keep,
+                               hipStream_t st) {
+  if (a.capacity == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_contact_keep, grid_for(a.capacity), dim3(256), 0, st, a, y_lo, y_hi, keep);
+  return hipGetLastError();
+}
+
+hipError_t launch_contact_compact(int64_t capacity, const contact_slots& src,
+                                  const contact_slots& dst, const uint32_t* keep,
+                                  const int64_t* off, hipStream_t st) {
+  if (capacity == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_contact_compact, grid_for(capacity), dim3(256), 0, st, capacity, src, dst,
+                     keep, off);
+  return hipGetLastError();
+}
+
+hipError_t launch_contact_inject(const contact_args& a, const int64_t* base, int64_t count,
+                                 const int32_t* sites, int64_t n_sites, uint64_t* next_id,
+                                 double t_now, int64_t* n_live, hipStream_t st) {
+  if (count > 0) {
+    if (n_sites < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_contact_inject, grid_for(count), dim3(256), 0, st, a, base, count, sites,
+                       n_sites, (const uint64_t*)next_id, t_now);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_contact_finish, dim3(1), dim3(64), 0, st, base, count, a.capacity, next_id,
+                     n_live);
   return hipGetLastError();
 }
 

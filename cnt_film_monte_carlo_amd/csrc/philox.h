@@ -58,6 +58,8 @@ enum : uint32_t {
   STREAM_HOP_REDRAW = 2u,     // + k: further dwell candidates after two zero words
   STREAM_INIT_REDRAW = 0x80000000u,
   STREAM_FLY_DRAIN = 0x40000000u,  // + j: re-injection site after the j-th drain of one flight
+  STREAM_CONTACT = 0x20000000u,    // contact mode, a new particle: w0 site, w1 dwell, w3 spare
+  STREAM_CONTACT_REDRAW = 0x20000001u,  // + k: its further dwell candidates
   REDRAW_LIMIT = 16u,         // bounded redraw loop; after it the draw is taken as 1
 };
 

@@ -112,6 +112,11 @@ _SIGS = {
     "kmc_set_hop_schedule": (C.c_int, [_P, C.c_int64]),
     "kmc_set_velocity": (C.c_int, [_P, C.c_double]),
     "kmc_get_flight": (C.c_int, [_P, _D, _U8, _D]),
+    "kmc_contact_init": (C.c_int, [_P, C.c_int32, C.c_int64, C.c_int64, C.c_int64]),
+    "kmc_contact_run": (C.c_int, [_P, C.c_double, C.c_int64, C.c_uint64, _I32, _I32, _I64]),
+    "kmc_contact_sections": (C.c_int, [_P, _D, _D, _D, _D]),
+    "kmc_contact_particles": (C.c_int, [_P, C.c_int64, _I64, C.POINTER(C.c_uint32), _I32, _D,
+                                        C.POINTER(C.c_uint64)]),
     "kmc_autotune_schedule": (C.c_int, [_P, C.c_int64, C.c_int32, _I64, _I64, _D]),
     "kmc_ensemble_create": (C.c_int, [C.c_int32, C.POINTER(_P), C.POINTER(_P)]),
     "kmc_ensemble_destroy": (C.c_int, [_P]),
@@ -403,6 +408,46 @@ class Context:
         self._ck(self.lib.kmc_get_flight(self.h, _ptr(off, _D), _ptr(hd, _U8), _ptr(tl, _D)))
         return dict(offset=off, heading=hd, t_last=tl)
 
+    # ---- contact (steady-state) mode (monte_carlo.h:157-195, 443-684), velocity 0 ----
+    def contact_init(self, n_seg: int, c1_pop: int, c2_pop: int, capacity: int):
+        """Sections, contact site lists, areas and the initial linear profile; see
+        kmc_contact_init."""
+        self._ck(self.lib.kmc_contact_init(self.h, int(n_seg), int(c1_pop), int(c2_pop),
+                                           int(capacity)))
+        self.contact_nseg = int(n_seg)
+
+    def contact_run(self, dt: float, n_steps: int, max_hops_per_step: int = 1 << 20):
+        """n_steps steps of dt: (population [n_steps, n_seg], crossings [n_steps, n_seg - 1],
+        live particles after each step's repopulation [n_steps])."""
+        S = self.contact_nseg
+        pop = np.zeros((n_steps, S), np.int32)
+        cur = np.zeros((n_steps, S - 1), np.int32)
+        live = np.zeros(n_steps, np.int64)
+        self._ck(self.lib.kmc_contact_run(self.h, float(dt), int(n_steps), int(max_hops_per_step),
+                                          _ptr(pop, _I32), _ptr(cur, _I32), _ptr(live, _I64)))
+        return pop, cur, live
+
+    def contact_sections(self):
+        """(area [n_seg], ymin, dy, time)."""
+        area = np.zeros(self.contact_nseg)
+        v = [C.c_double() for _ in range(3)]
+        self._ck(self.lib.kmc_contact_sections(self.h, _ptr(area, _D), *[C.byref(x) for x in v]))
+        return area, v[0].value, v[1].value, v[2].value
+
+    def contact_particles(self) -> dict:
+        """The live particles in slot order: id, site, t_next, hops."""
+        n = C.c_int64()
+        self._ck(self.lib.kmc_contact_particles(self.h, 0, C.byref(n), None, None, None, None))
+        N = n.value
+        pid = np.zeros(N, np.uint32)
+        site = np.zeros(N, np.int32)
+        t = np.zeros(N)
+        hops = np.zeros(N, np.uint64)
+        self._ck(self.lib.kmc_contact_particles(
+            self.h, N, C.byref(n), pid.ctypes.data_as(C.POINTER(C.c_uint32)), _ptr(site, _I32),
+            _ptr(t, _D), hops.ctypes.data_as(C.POINTER(C.c_uint64))))
+        return dict(id=pid, site=site, t_next=t, hops=hops)
+
     def get_excitons(self):
         E = self.n_exc
         site = np.zeros(E, np.int32)
@@ -636,6 +681,36 @@ def load_rate_table(directory, name: str = "scat_table"):
     _check(lib.kmc_load_rate_table(d, nm, _ptr(dims, _I64), *[_ptr(v, _D) for v in axes],
                                    _ptr(rates, _D)))
     return (*axes, rates)
+
+
+def fick_diffusion(population, crossings, area, ymin: float, dy: float, dt: float,
+                   steady_from: int = 0, interior_fit: bool = False):
+    """Fick's law D = J / |dn/dy| from a contact run: densities n = population / (area dy) and
+    currents J = crossings / (interface area dt) (save_population_profile / save_currents,
+    monte_carlo.h:525-626) averaged over the steps from steady_from on; J is the mean of |J| at
+    the first and the last interface.  The gradient is the reference profiler's
+    (visualization/profiler.py:55-65: the density difference of the two end sections over the
+    distance of their centres) or, with interior_fit, the least-squares slope over the sections
+    between the contacts (the contacts are reset every step, which bends the profile next to
+    them).  Returns (D, J, gradient, mean density profile)."""
+    pop = np.asarray(population, np.float64)[steady_from:]
+    cur = np.asarray(crossings, np.float64)[steady_from:]
+    area = np.asarray(area, np.float64)
+    S = area.size
+    dens = pop / (area[None, :] * dy)
+    a_if = (area[:-1] + area[1:]) / 2
+    J = cur / (a_if[None, :] * dt)
+    avg_j = J.mean(0)
+    j = 0.5 * (abs(avg_j[0]) + abs(avg_j[-1]))
+    n_avg = dens.mean(0)
+    pos = ymin + (np.arange(S) + 0.5) * dy
+    if interior_fit:
+        if S < 4:
+            raise ValueError("interior_fit needs at least two sections between the contacts")
+        grad = abs(np.polyfit(pos[1:-1], n_avg[1:-1], 1)[0])
+    else:
+        grad = abs((n_avg[0] - n_avg[-1]) / (pos[-1] - pos[0]))
+    return j / grad, j, grad, n_avg
 
 
 def estimate_diffusion(reduced: np.ndarray, t_sample_max: float, k_lo: int = 0,

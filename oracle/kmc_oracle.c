@@ -935,6 +935,171 @@ int orc_reduce_samples(int64_t E, int32_t n_bins, int32_t n_batches, const uint3
   return 0;
 }
 
+/* ------------------------------------------------------------------------------------------
+ * Contact (steady-state) mode, velocity 0: monte_carlo::init / step / save_metrics /
+ * repopulate_contacts (monte_carlo.h:157-195, 274-316, 343-355, 443-684; main.cpp:91-99),
+ * written literally over one particle list.  Draw mapping of this mode: a particle carries an
+ * id (0, 1, 2, ... in creation order); its hops draw (seed, id, its hop count, stream 0) with
+ * dwell redraws on streams 2+k, its creation draws (seed, id, 0, 0x20000000) -- w0 the site
+ * (rand() % size), w1 the dwell -- with redraws on 0x20000001+k.
+ * ---------------------------------------------------------------------------------------- */
+typedef struct {
+  uint32_t id;
+  int32_t site;
+  double t_next;
+  uint64_t hops;
+} orc_cparticle;
+
+static int orc_new_particle(uint64_t seed, uint32_t id, const int32_t* list, int64_t n_list,
+                            const int64_t* row_ptr, const double* cum, double t_now,
+                            orc_cparticle* p) {
+  uint32_t w[4], r;
+  int32_t s;
+  if (n_list <= 0) return -1;
+  draws(seed, id, 0, 0x20000000u, w);
+  s = list[(w[0] >> 1) % (uint32_t)n_list];
+  r = dwell_draw(seed, id, 0, w[1], w[3], 0x20000001u);
+  p->id = id;
+  p->site = s;
+  p->t_next = t_now + orc_dwell(cum + row_ptr[s], row_ptr[s + 1] - row_ptr[s], r);
+  p->hops = 0;
+  return 0;
+}
+
+/* Returns 0, -1 for a bad argument or an empty list, -2 when the particles exceed capacity. */
+int orc_contact(int64_t n_sites, const double* x, const double* y, const double* z,
+                const int64_t* row_ptr, const int32_t* col, const double* cum, uint64_t seed,
+                int n_seg, int64_t c1_pop, int64_t c2_pop, int64_t capacity, double dt,
+                int64_t n_steps, int32_t* pop, int32_t* cur, int64_t* n_live, double* area,
+                int64_t* n_final, uint32_t* p_id, int32_t* p_site, double* p_t, uint64_t* p_hops) {
+  double lo[3], hi[3], ymin, ymax, dy, dp, T = 0.0;
+  double *xmn, *xmx, *zmn, *zmx;
+  int32_t *c1, *cn, *sec;
+  int64_t n1 = 0, nn = 0, s, k, np_ = 0, step, i;
+  uint32_t next_id = 0;
+  orc_cparticle *P, *Q;
+  int a, side, ret = 0;
+  if (n_sites <= 0 || n_seg < 2 || capacity <= 0) return -1;
+  /* find_simulation_domain (monte_carlo.h:328-340) */
+  for (a = 0; a < 3; ++a) {
+    const double* v = a == 0 ? x : (a == 1 ? y : z);
+    lo[a] = hi[a] = v[0];
+    for (s = 1; s < n_sites; ++s) {
+      if (v[s] < lo[a]) lo[a] = v[s];
+      if (v[s] > hi[a]) hi[a] = v[s];
+    }
+  }
+  ymin = lo[1];
+  ymax = hi[1];
+  dy = (ymax - ymin) / (double)n_seg;
+  /* get_area (monte_carlo.h:644-684) */
+  xmn = malloc(sizeof(double) * n_seg); xmx = malloc(sizeof(double) * n_seg);
+  zmn = malloc(sizeof(double) * n_seg); zmx = malloc(sizeof(double) * n_seg);
+  for (i = 0; i < n_seg; ++i) { xmx[i] = lo[0]; xmn[i] = hi[0]; zmx[i] = lo[2]; zmn[i] = hi[2]; }
+  for (s = 0; s < n_sites; ++s) {
+    int j = (int)((y[s] - ymin) / dy);
+    j = j < 0 ? 0 : (j < n_seg ? j : n_seg - 1);
+    if (xmn[j] > x[s]) xmn[j] = x[s];
+    else if (xmx[j] < x[s]) xmx[j] = x[s];
+    if (zmn[j] > z[s]) zmn[j] = z[s];
+    else if (zmx[j] < z[s]) zmx[j] = z[s];
+  }
+  for (i = 0; i < n_seg; ++i) area[i] = (zmx[i] - zmn[i]) * (xmx[i] - xmn[i]);
+  free(xmn); free(xmx); free(zmn); free(zmx);
+  /* contact_scats(1) and contact_scats(n_seg) (monte_carlo.h:494-516) */
+  c1 = malloc(sizeof(int32_t) * n_sites);
+  cn = malloc(sizeof(int32_t) * n_sites);
+  sec = malloc(sizeof(int32_t) * n_sites);
+  for (s = 0; s < n_sites; ++s) {
+    if (y[s] >= ymin + 0.0 * dy && y[s] <= ymin + 1.0 * dy) c1[n1++] = (int32_t)s;
+    if (y[s] >= ymin + (double)(n_seg - 1) * dy && y[s] <= ymin + (double)n_seg * dy)
+      cn[nn++] = (int32_t)s;
+  }
+  P = malloc(sizeof(orc_cparticle) * capacity);
+  Q = malloc(sizeof(orc_cparticle) * capacity);
+  /* create_particles (monte_carlo.h:274-316): the linear profile, section by section */
+  dp = (double)(c2_pop - c1_pop) / ((double)n_seg - 1);
+  for (i = 0; i < n_seg && ret == 0; ++i) {
+    int64_t ns = 0, want = (int64_t)round((double)c1_pop + (double)i * dp);
+    const double y1 = ymin + (double)i * dy;
+    const double y2 = y1 + dy;
+    for (s = 0; s < n_sites; ++s)
+      if (y1 <= y[s] && y[s] < y2) sec[ns++] = (int32_t)s;
+    for (k = 0; k < want && ret == 0; ++k) {
+      if (np_ >= capacity) { ret = -2; break; }
+      if (orc_new_particle(seed, next_id++, sec, ns, row_ptr, cum, 0.0, &P[np_++]) != 0) ret = -1;
+    }
+  }
+  for (step = 0; step < n_steps && ret == 0; ++step) {
+    const double t_end = T + dt;
+    int32_t* pp = pop + step * n_seg;
+    int32_t* cc = cur + step * (n_seg - 1);
+    memset(pp, 0, sizeof(int32_t) * n_seg);
+    memset(cc, 0, sizeof(int32_t) * (n_seg - 1));
+    /* monte_carlo::step -> particle::step (particle.cpp:57-80) */
+    for (k = 0; k < np_; ++k) {
+      orc_cparticle* p = &P[k];
+      const double y_old = y[p->site];
+      double y_new;
+      int j, q;
+      while (p->t_next <= t_end) {
+        const int64_t b = row_ptr[p->site], n = row_ptr[p->site + 1] - b;
+        uint32_t w[4], r;
+        double dice;
+        if (n == 0 || !(cum[b + n - 1] > 0.0)) break;          /* no rates: stays */
+        draws(seed, p->id, p->hops, 0u, w);
+        dice = (cum[b + n - 1] * (double)(w[0] >> 1)) / ORC_RAND_MAX;
+        p->site = col[b + orc_select(cum + b, (unsigned)n, dice)];
+        r = dwell_draw(seed, p->id, p->hops, w[1], w[3], 2u);
+        p->t_next = p->t_next + orc_dwell(cum + row_ptr[p->site],
+                                          row_ptr[p->site + 1] - row_ptr[p->site], r);
+        p->hops++;
+      }
+      y_new = y[p->site];
+      /* save_population_profile (monte_carlo.h:565-571) */
+      j = (int)((y_new - ymin) / dy);
+      j = j < 0 ? 0 : (j < n_seg ? j : n_seg - 1);
+      pp[j]++;
+      /* save_currents (monte_carlo.h:598-626): every interface */
+      for (q = 1; q < n_seg; ++q) {
+        const double yq = ymin + dy * (double)q;
+        if (y_old < yq && y_new >= yq) cc[q - 1]++;
+        else if (y_old >= yq && y_new < yq) cc[q - 1]--;
+      }
+    }
+    T = t_end;
+    /* repopulate_contacts (monte_carlo.h:443-491): contact 1, then contact n_seg; the kept
+     * particles in their order, the new ones after them */
+    for (side = 0; side < 2 && ret == 0; ++side) {
+      const double ylo = side == 0 ? ymin : ymin + (double)(n_seg - 1) * dy;
+      const double yhi = side == 0 ? ymin + dy : ymax;
+      const int64_t want = side == 0 ? c1_pop : c2_pop;
+      int64_t nk = 0;
+      for (k = 0; k < np_; ++k) {
+        const double yy = y[P[k].site];
+        if (!(yy >= ylo && yy <= yhi)) Q[nk++] = P[k];
+      }
+      for (k = 0; k < want; ++k) {
+        if (nk >= capacity) { ret = -2; break; }
+        if (orc_new_particle(seed, next_id++, side == 0 ? c1 : cn, side == 0 ? n1 : nn, row_ptr,
+                             cum, T, &Q[nk++]) != 0) { ret = -1; break; }
+      }
+      memcpy(P, Q, sizeof(orc_cparticle) * nk);
+      np_ = nk;
+    }
+    n_live[step] = np_;
+  }
+  *n_final = np_;
+  for (k = 0; k < np_; ++k) {
+    p_id[k] = P[k].id;
+    p_site[k] = P[k].site;
+    p_t[k] = P[k].t_next;
+    p_hops[k] = P[k].hops;
+  }
+  free(c1); free(cn); free(sec); free(P); free(Q);
+  return ret;
+}
+
 int orc_max_threads(void) {
 #ifdef _OPENMP
   return omp_get_max_threads();

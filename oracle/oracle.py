@@ -77,6 +77,9 @@ def library() -> C.CDLL:
             "orc_reduce_samples": (C.c_int, [C.c_int64, C.c_int32, C.c_int32, _U32, _D, _U32,
                                              _D]),
             "orc_max_threads": (C.c_int, []),
+            "orc_contact": (C.c_int, [C.c_int64, _D, _D, _D, _I64, _I32, _D, C.c_uint64, C.c_int,
+                                      C.c_int64, C.c_int64, C.c_int64, C.c_double, C.c_int64,
+                                      _I32, _I32, _I64, _D, _I64, _U32, _I32, _D, _U64]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)
@@ -353,3 +356,36 @@ def first_order(donor: dict, acceptor: dict, temperature: float, broadening: flo
     r = fn(C.byref(sd), C.byref(sa), temperature, broadening, z_shift, axis_shift_1, axis_shift_2,
            theta, C.byref(npairs))
     return r, npairs.value
+
+
+def contact(net: dict, csr: dict, seed: int, n_seg: int, c1_pop: int, c2_pop: int,
+            capacity: int, dt: float, n_steps: int) -> dict:
+    """Contact (steady-state) mode over the oracle CSR (orc_contact: init, step, save_metrics,
+    repopulate_contacts of src/monte_carlo/monte_carlo.h, written literally): per step the
+    population [n_steps, n_seg], the interface crossings [n_steps, n_seg - 1] and the live count;
+    the section areas; the final particles (id, site, t_next, hops) in list order."""
+    lib = library()
+    pos = [_c(net[k], np.float64) for k in ("x", "y", "z")]
+    row = _c(csr["row_ptr"], np.int64)
+    col = _c(csr["col"], np.int32)
+    cum = _c(csr["cum"], np.float64)
+    pop = np.zeros((n_steps, n_seg), np.int32)
+    cur = np.zeros((n_steps, n_seg - 1), np.int32)
+    live = np.zeros(n_steps, np.int64)
+    area = np.zeros(n_seg)
+    n_final = np.zeros(1, np.int64)
+    pid = np.zeros(capacity, np.uint32)
+    site = np.zeros(capacity, np.int32)
+    t = np.zeros(capacity)
+    hops = np.zeros(capacity, np.uint64)
+    rc = lib.orc_contact(pos[0].size, *[_p(v, _D) for v in pos], _p(row, _I64), _p(col, _I32),
+                         _p(cum, _D), seed, n_seg, c1_pop, c2_pop, capacity, dt, n_steps,
+                         _p(pop, _I32), _p(cur, _I32), _p(live, _I64), _p(area, _D),
+                         _p(n_final, _I64), _p(pid, _U32), _p(site, _I32), _p(t, _D),
+                         _p(hops, _U64))
+    if rc != 0:
+        raise RuntimeError(f"orc_contact failed ({rc})")
+    n = int(n_final[0])
+    return dict(population=pop, crossings=cur, n_live=live, area=area, id=pid[:n],
+                site=site[:n], t_next=t[:n], hops=hops[:n])
+

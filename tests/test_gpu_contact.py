@@ -1,0 +1,111 @@
+"""Contact (steady-state) mode on the GPU: init / step / save_metrics / repopulate_contacts
+(src/monte_carlo/monte_carlo.h:157-195, 274-316, 343-355, 443-684) against the oracle's literal
+restatement (orc_contact) bit for bit -- populations, interface crossings, live counts, section
+areas, particles -- and Fick's law D = J / |dn/dy| (visualization/profiler.py:25-65) on a
+simple cubic lattice, where D = k a^2 is known."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+NM = 1e-9
+
+
+def say(*a):
+    print("[gpu-contact]", *a, file=sys.stderr, flush=True)
+
+
+def _film(kmc, nw, seed):
+    # the 5k-site film with open axes (transport along y) and no Kubo flags
+    w = nw.film_small()
+    net = nw.build_network(w)
+    net["flags"] = np.zeros(net["n"], np.uint8)
+    ctx = kmc.Context(kmc.Config(cutoff=w.cutoff, min_distance=0.4 * NM, seed=seed))
+    ctx.set_sites(net["x"], net["y"], net["z"], net["ox"], net["oy"], net["oz"], net["left"],
+                  net["right"], net["chirality"], net["flags"])
+    tab = nw.forster_tables(1)
+    ctx.set_rate_table(*tab)
+    ctx.build_cells()
+    ctx.build_rates(kmc.RATE_TABLE)
+    return w, net, ctx, tab
+
+
+@pytest.mark.parametrize("n_seg,c1,c2,dt,steps", [(10, 400, 40, 2e-8, 80), (3, 150, 0, 5e-8, 40)])
+def test_contact_matches_oracle(kmc, orc, nw, n_seg, c1, c2, dt, steps):
+    say(f"contact mode vs oracle: n_seg {n_seg}, populations {c1}/{c2}, {steps} steps of {dt}")
+    w, net, ctx, tab = _film(kmc, nw, seed=29)
+    cap = 8000
+    ctx.contact_init(n_seg, c1, c2, cap)
+    p0 = ctx.contact_particles()
+    pop, cur, live = ctx.contact_run(dt, steps // 2)
+    pop2, cur2, live2 = ctx.contact_run(dt, steps - steps // 2)   # the run continues
+    pop, cur, live = np.vstack([pop, pop2]), np.vstack([cur, cur2]), np.concatenate([live, live2])
+    onet = orc.Net(net, w.cutoff, 0.4 * NM)
+    onet.set_table(*tab)
+    ref = orc.contact(net, onet.csr(0), 29, n_seg, c1, c2, cap, dt, steps)
+    area, ymin, dy, T = ctx.contact_sections()
+    assert np.array_equal(area, ref["area"])
+    assert T == pytest.approx(steps * dt, rel=1e-12)
+    assert np.array_equal(pop, ref["population"])
+    assert np.array_equal(cur, ref["crossings"])
+    assert np.array_equal(live, ref["n_live"])
+    p = ctx.contact_particles()
+    assert np.array_equal(p["id"], ref["id"]) and np.array_equal(p["site"], ref["site"])
+    assert np.array_equal(p["hops"], ref["hops"])
+    np.testing.assert_allclose(p["t_next"], ref["t_next"], rtol=1e-12)
+    # bookkeeping: a step's population counts the particles that entered it
+    assert pop[0].sum() == p0["id"].size
+    assert np.array_equal(pop[1:].sum(1), live[:-1])
+    assert int(p["hops"].sum()) > 0 and np.any(cur != 0)
+    say(f"  live {live[-1]}, hops {int(p['hops'].sum())}, crossings {int(np.abs(cur).sum())}")
+    ctx.close()
+
+
+def test_contact_errors(kmc, nw):
+    w, net, ctx, tab = _film(kmc, nw, seed=5)
+    with pytest.raises(kmc.KmcError) as e:
+        ctx.contact_nseg = 4
+        ctx.contact_run(1e-8, 1)
+    assert e.value.code == kmc.ERR_STATE
+    with pytest.raises(kmc.KmcError) as e:
+        ctx.contact_init(1, 10, 0, 100)
+    assert e.value.code == kmc.ERR_ARG
+    with pytest.raises(kmc.KmcError) as e:
+        ctx.contact_init(4, 1000, 0, 500)              # capacity below the initial profile
+    assert e.value.code == kmc.ERR_CAPACITY
+    ctx.contact_init(4, 100, 0, 300)
+    with pytest.raises(kmc.KmcError, match="max_hops"):
+        ctx.contact_run(1e-6, 1, max_hops_per_step=2)  # long step, tiny hop cap
+    ctx.close()
+
+
+def test_fick_known_diffusion(kmc):
+    # simple cubic lattice, spacing a, cutoff 1.2 a (nearest neighbours only, the face
+    # diagonals at 1.41 a excluded), rate k to each: along y the walk has D = k a^2
+    a, k = 10 * NM, 1e12
+    nx, ny, nz = 4, 60, 4
+    g = np.stack(np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij"), -1)
+    pos = g.reshape(-1, 3).astype(float) * a
+    ori = np.tile([0.0, 1.0, 0.0], (pos.shape[0], 1))
+    ctx = kmc.Context(kmc.Config(cutoff=1.2 * a, seed=11))
+    ctx.set_sites(pos[:, 0], pos[:, 1], pos[:, 2], ori[:, 0], ori[:, 1], ori[:, 2])
+    ctx.set_rate_table([0.0], [0.0], [0.0], [0.0], np.full(1, k))
+    ctx.build_cells()
+    ctx.build_rates(kmc.RATE_TABLE)
+    assert ctx.nnz == 6 * nx * ny * nz - 2 * (ny * nz + nx * nz + nx * ny)
+    ctx.contact_init(6, 1000, 0, 20000)
+    dt = 1e-12                                         # ~2 hops per particle and step
+    ctx.contact_run(dt, 6000)                          # to the steady state (L^2 / D = 3.5e-9 s)
+    pop, cur, live = ctx.contact_run(dt, 8000)
+    area, ymin, dy, _ = ctx.contact_sections()
+    D, J, grad, n = kmc.fick_diffusion(pop, cur, area, ymin, dy, dt, interior_fit=True)
+    D_ends = kmc.fick_diffusion(pop, cur, area, ymin, dy, dt)[0]
+    say(f"  Fick D = {D:.4e} (interior fit), {D_ends:.4e} (end sections); k a^2 = {k * a * a:.4e}")
+    say(f"  density profile {np.array2string(n, precision=3)}")
+    # measured -3.6 % (fit) and +16 % (end sections): ~2 net crossings per step over 8,000
+    # steps leave ~1.5 % noise, the step-wise reset of the contacts a few percent of bias, and
+    # the end-section gradient of the reference's profiler more (its contacts bend the profile)
+    assert abs(D / (k * a * a) - 1) < 0.08
+    assert abs(D_ends / (k * a * a) - 1) < 0.25
+    ctx.close()
