@@ -103,9 +103,10 @@ def test_fick_known_diffusion(kmc):
     D_ends = kmc.fick_diffusion(pop, cur, area, ymin, dy, dt)[0]
     say(f"  Fick D = {D:.4e} (interior fit), {D_ends:.4e} (end sections); k a^2 = {k * a * a:.4e}")
     say(f"  density profile {np.array2string(n, precision=3)}")
-    # measured -3.6 % (fit) and +16 % (end sections): ~2 net crossings per step over 8,000
-    # steps leave ~1.5 % noise, the step-wise reset of the contacts a few percent of bias, and
-    # the end-section gradient of the reference's profiler more (its contacts bend the profile)
+    # measured -3.6 % (fit) and +16 % (end sections): the sections are 59 a / 6 long but hold
+    # 10 layers, so the gradient over section centres is 1.7 % steep; ~2 net crossings per step
+    # over 8,000 steps leave ~1.5 % noise; the step-wise reset of the contacts bends the profile
+    # next to them, which the end-section gradient of the reference's profiler takes in
     assert abs(D / (k * a * a) - 1) < 0.08
     assert abs(D_ends / (k * a * a) - 1) < 0.25
     ctx.close()
