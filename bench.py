@@ -353,8 +353,8 @@ def main():
         log(f"copied {n * 4} bytes x 3")
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: the KMC path shards excitons across GPUs by design, "
-              "but this build runs and reports a single MI355X only (--gpus 1).", file=sys.stderr)
+        print("bench.py measures one GPU: the KMC path would shard excitons across GPUs, but "
+              "it runs and reports a single MI355X only (--gpus 1).", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
