@@ -1739,6 +1739,9 @@ int kmc_contact_run(kmc_ctx* c, double dt, int64_t n_steps, uint64_t max_hops_pe
   KMC_TRY(dalloc(c, c->c_live_out, (size_t)n_steps));
   KMC_HIP(c, hipMemsetAsync(c->c_pop_out.p, 0, c->c_pop_out.n * sizeof(int32_t), c->stream));
   KMC_HIP(c, hipMemsetAsync(c->c_cur_out.p, 0, c->c_cur_out.n * sizeof(int32_t), c->stream));
+  // errors are this call's: a hop cap reached leaves every particle consistent (the one that
+  // stopped continues in the next step), so a retry with a larger cap may go on
+  KMC_HIP(c, hipMemsetAsync(c->c_err.p, 0, sizeof(int32_t), c->stream));
   for (int64_t k = 0; k < n_steps; ++k) {
     // monte_carlo::step, then save_metrics (main.cpp:95-97): every particle to the step's end
     contact_args a = contact_args_of(c);

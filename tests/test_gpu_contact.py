@@ -77,6 +77,10 @@ def test_contact_errors(kmc, nw):
     ctx.contact_init(4, 100, 0, 300)
     with pytest.raises(kmc.KmcError, match="max_hops"):
         ctx.contact_run(1e-6, 1, max_hops_per_step=2)  # long step, tiny hop cap
+    # the next call starts clean; the particles stopped by the cap catch up first (up to
+    # ~1e6 hops each over the microsecond they are behind), hence the large cap
+    pop, cur, live = ctx.contact_run(1e-12, 2, max_hops_per_step=1 << 30)
+    assert pop.shape == (2, 4) and live[-1] > 0
     ctx.close()
 
 
