@@ -131,7 +131,7 @@ struct kmc_ctx {
   bool c_on = false;
   int32_t c_nseg = 0;
   int64_t c_pop[2] = {0, 0}, c_cap = 0;
-  double c_ymin = 0.0, c_dy = 0.0, c_time = 0.0;
+  double c_ymin = 0.0, c_dy = 0.0, c_time = 0.0, c_vel = 0.0;
   double c_rm_lo[2] = {0, 0}, c_rm_hi[2] = {0, 0};  // repopulate ranges of the two contacts
   std::vector<double> c_area;
   int64_t c_list_off[2] = {0, 0}, c_list_n[2] = {0, 0};
@@ -140,6 +140,8 @@ struct kmc_ctx {
   dbuf<double> c_t[2];
   dbuf<uint64_t> c_hops[2];
   dbuf<uint32_t> c_id[2];
+  dbuf<double> c_y[2], c_soff[2], c_tlast[2];
+  dbuf<uint8_t> c_head[2];
   int c_buf = 0;                          // the buffer holding the live particles
   dbuf<uint32_t> c_keep;
   dbuf<int64_t> c_off, c_scan_tmp, c_live, c_live_out;
@@ -450,6 +452,7 @@ int kmc_destroy(kmc_ctx* c) {
   dfree(c->c_live_out); dfree(c->c_next); dfree(c->c_err); dfree(c->c_pop_out); dfree(c->c_cur_out);
   for (int b = 0; b < 2; ++b) {
     dfree(c->c_site[b]); dfree(c->c_t[b]); dfree(c->c_hops[b]); dfree(c->c_id[b]);
+    dfree(c->c_y[b]); dfree(c->c_soff[b]); dfree(c->c_tlast[b]); dfree(c->c_head[b]);
   }
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -1572,7 +1575,8 @@ int kmc_ensemble_autotune(kmc_ensemble* m, int64_t trial_hops, int32_t n_cand, c
 namespace {
 
 contact_slots contact_buf(kmc_ctx* c, int b) {
-  return contact_slots{c->c_site[b].p, c->c_t[b].p, c->c_hops[b].p, c->c_id[b].p};
+  return contact_slots{c->c_site[b].p, c->c_t[b].p,    c->c_hops[b].p, c->c_id[b].p,
+                       c->c_y[b].p,    c->c_head[b].p, c->c_soff[b].p, c->c_tlast[b].p};
 }
 
 contact_args contact_args_of(kmc_ctx* c) {
@@ -1588,6 +1592,10 @@ contact_args contact_args_of(kmc_ctx* c) {
   a.ymin = c->c_ymin;
   a.dy = c->c_dy;
   a.err = c->c_err.p;
+  if (c->c_vel > 0.0) {
+    a.fv = view_of(c);
+    a.fv.velocity = c->c_vel;
+  }
   return a;
 }
 
@@ -1601,6 +1609,9 @@ int contact_check_err(kmc_ctx* c, const char* who) {
     return fail(c, KMC_ERR_CAPACITY, std::string(who) + ": more particles than the capacity");
   if (err & CONTACT_ERR_IDS)
     return fail(c, KMC_ERR_CAPACITY, std::string(who) + ": more than 2^32 particles injected");
+  if (err & CONTACT_ERR_FLY)
+    return fail(c, KMC_ERR_CAPACITY, std::string(who) + ": a flight passed more than 2^22 sites "
+                                         "within one step (shorten dt)");
   return KMC_OK;
 }
 
@@ -1614,13 +1625,18 @@ int kmc_contact_init(kmc_ctx* c, int32_t n_seg, int64_t c1_pop, int64_t c2_pop, 
   if (!c) return fail(nullptr, KMC_ERR_ARG, "kmc_contact_init: null context");
   if (c->csr_state != 2 || !c->d_blocks.p)
     return fail(c, KMC_ERR_STATE, "kmc_contact_init: CSR not complete");
-  if (c->velocity != 0.0)
-    return fail(c, KMC_ERR_STATE, "kmc_contact_init: the contact mode runs with velocity 0 only");
   if (n_seg < 2 || n_seg > CONTACT_MAX_SEG)
     return fail(c, KMC_ERR_ARG, "kmc_contact_init: need 2 <= n_seg <= 1024");
   if (c1_pop < 0 || c2_pop < 0 || capacity <= 0 || capacity > (int64_t)0x7FFFFFFF)
     return fail(c, KMC_ERR_ARG, "kmc_contact_init: need populations >= 0, 0 < capacity < 2^31");
   KMC_HIP(c, hipSetDevice(c->cfg.device));
+  if (c->velocity > 0.0) {
+    KMC_TRY(check_links(c, who));
+    for (uint8_t f : c->h_flags)
+      if (f & KMC_SITE_DRAIN)
+        return fail(c, KMC_ERR_STATE, "kmc_contact_init: with velocity > 0 a flight would "
+                                      "re-inject at Kubo drain sites; clear KMC_SITE_DRAIN");
+  }
   const std::vector<double>& X = c->h_pos[0];
   const std::vector<double>& Y = c->h_pos[1];
   const std::vector<double>& Z = c->h_pos[2];
@@ -1680,6 +1696,7 @@ int kmc_contact_init(kmc_ctx* c, int32_t n_seg, int64_t c1_pop, int64_t c2_pop, 
   c->c_ymin = ymin;
   c->c_dy = dy;
   c->c_time = 0.0;
+  c->c_vel = c->velocity;
   // repopulate_contacts (monte_carlo.h:443-455): [ymin, ymin + dy] and [ymin + (n-1) dy, ymax]
   c->c_rm_lo[0] = ymin;
   c->c_rm_hi[0] = ymin + dy;
@@ -1698,6 +1715,10 @@ int kmc_contact_init(kmc_ctx* c, int32_t n_seg, int64_t c1_pop, int64_t c2_pop, 
     KMC_TRY(dalloc(c, c->c_t[b], capacity));
     KMC_TRY(dalloc(c, c->c_hops[b], capacity));
     KMC_TRY(dalloc(c, c->c_id[b], capacity));
+    KMC_TRY(dalloc(c, c->c_y[b], capacity));
+    KMC_TRY(dalloc(c, c->c_head[b], capacity));
+    KMC_TRY(dalloc(c, c->c_soff[b], capacity));
+    KMC_TRY(dalloc(c, c->c_tlast[b], capacity));
   }
   KMC_TRY(dalloc(c, c->c_keep, capacity));
   KMC_TRY(dalloc(c, c->c_off, capacity + 1));
@@ -1749,7 +1770,7 @@ int kmc_contact_run(kmc_ctx* c, double dt, int64_t n_steps, uint64_t max_hops_pe
     a.max_hops = max_hops_per_step;
     a.pop = c->c_pop_out.p + k * S;
     a.cur = c->c_cur_out.p + k * (S - 1);
-    KMC_HIP(c, launch_contact_step(a, c->stream));
+    KMC_HIP(c, launch_contact_step(a, c->c_vel > 0.0, c->stream));
     c->c_time = a.t_end;  // _time += dt (monte_carlo.h:354)
     // repopulate_contacts (monte_carlo.h:443-491): contact 1, then contact n_seg; the kept
     // particles keep their order, the new ones follow them
@@ -1796,14 +1817,14 @@ int kmc_contact_sections(const kmc_ctx* c, double* area, double* ymin, double* d
 }
 
 int kmc_contact_particles(kmc_ctx* c, int64_t capacity, int64_t* n, uint32_t* id, int32_t* site,
-                          double* t_next, uint64_t* hops) {
+                          double* t_next, uint64_t* hops, double* y) {
   if (!c || !n) return fail(c, KMC_ERR_ARG, "kmc_contact_particles: null argument");
   if (!c->c_on) return fail(c, KMC_ERR_STATE, "kmc_contact_particles: call kmc_contact_init first");
   KMC_HIP(c, hipSetDevice(c->cfg.device));
   int64_t live = 0;
   KMC_HIP(c, hipMemcpy(&live, c->c_live.p, sizeof(live), hipMemcpyDeviceToHost));
   *n = live;
-  if (!id && !site && !t_next && !hops) return KMC_OK;  // size query
+  if (!id && !site && !t_next && !hops && !y) return KMC_OK;  // size query
   if (capacity < live) return fail(c, KMC_ERR_CAPACITY, "kmc_contact_particles: capacity below n");
   const int b = c->c_buf;
   if (live > 0) {
@@ -1811,6 +1832,7 @@ int kmc_contact_particles(kmc_ctx* c, int64_t capacity, int64_t* n, uint32_t* id
     if (site) KMC_HIP(c, hipMemcpy(site, c->c_site[b].p, live * sizeof(int32_t), hipMemcpyDeviceToHost));
     if (t_next) KMC_HIP(c, hipMemcpy(t_next, c->c_t[b].p, live * sizeof(double), hipMemcpyDeviceToHost));
     if (hops) KMC_HIP(c, hipMemcpy(hops, c->c_hops[b].p, live * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    if (y) KMC_HIP(c, hipMemcpy(y, c->c_y[b].p, live * sizeof(double), hipMemcpyDeviceToHost));
   }
   return KMC_OK;
 }

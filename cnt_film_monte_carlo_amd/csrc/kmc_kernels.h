@@ -65,12 +65,18 @@ struct net_view {
 
 // Contact (steady-state) mode: particle slots [0, *n_live) of one buffer.
 constexpr int CONTACT_MAX_SEG = 1024;
-enum : int32_t { CONTACT_ERR_HOPS = 1, CONTACT_ERR_CAPACITY = 2, CONTACT_ERR_IDS = 4 };
+enum : int32_t {
+  CONTACT_ERR_HOPS = 1, CONTACT_ERR_CAPACITY = 2, CONTACT_ERR_IDS = 4, CONTACT_ERR_FLY = 8
+};
 struct contact_slots {
   int32_t* site;
   double* t_next;
   uint64_t* hops;   // hops done by the particle (its draw counter)
   uint32_t* id;     // the particle's draw key, unique over the run
+  double* y;        // FP64 y of the particle (its site's, plus the flight offset when v > 0)
+  uint8_t* heading; // v > 0: 1 = toward `right`
+  double* s_off;    // v > 0: distance travelled beyond the site toward the next one
+  double* t_last;   // v > 0: time up to which the flight is taken
 };
 struct contact_args {
   const uint4* blocks;
@@ -87,6 +93,7 @@ struct contact_args {
   int32_t* pop;             // [n_seg] particles per section after the step
   int32_t* cur;             // [n_seg - 1] net crossings of each interface during the step
   int32_t* err;             // CONTACT_ERR_* bits
+  net_view fv;              // v > 0: links, periods, velocity and blocks for the flights
 };
 
 struct trace_view {         // trace [n_trace][tr_max] (single network only)
@@ -178,7 +185,7 @@ hipError_t launch_hop_ens(const ens_args& a, bool count, bool sample, hipStream_
 hipError_t launch_ens_keys(int64_t n, const net_view* nets, const uint16_t* exc_net,
                            const int64_t* ebase, uint64_t* keys, uint32_t* ids, hipStream_t st);
 hipError_t launch_reduce(const reduce_args& a, hipStream_t st);
-hipError_t launch_contact_step(const contact_args& a, hipStream_t st);
+hipError_t launch_contact_step(const contact_args& a, bool fly, hipStream_t st);
 hipError_t launch_contact_keep(const contact_args& a, double y_lo, double y_hi, uint32_t* keep,
                                hipStream_t st);
 hipError_t launch_contact_compact(int64_t capacity, const contact_slots& src,

@@ -1273,6 +1273,10 @@ __device__ __forceinline__ int contact_section(double y, double ymin, double dy,
   return i < 0 ? 0 : (i < n_seg ? i : n_seg - 1);
 }
 
+// particle::step (src/monte_carlo/particle.cpp:57-80) over one step.  FLY (v > 0): the flight
+// up to each due hop, the hop from the site reached, then the flight to the step's end, which
+// is taken (particle::fly(dt), particle.cpp:77) and leaves the particle between sites.
+template <bool FLY>
 __global__ __launch_bounds__(256) void k_contact_step(contact_args a) {
 #pragma clang fp contract(off)
   __shared__ int32_t s_pop[CONTACT_MAX_SEG], s_cur[CONTACT_MAX_SEG];
@@ -1283,13 +1287,36 @@ __global__ __launch_bounds__(256) void k_contact_step(contact_args a) {
     const uint32_t id = a.s.id[e];
     double t = a.s.t_next[e];
     uint64_t h = a.s.hops[e];
-    const int32_t site0 = a.s.site[e];
-    const uint32_t L0 = (uint32_t)a.roff[site0];
+    const uint32_t L0 = (uint32_t)a.roff[a.s.site[e]];
     uint32_t R = row_word(L0, a.blocks[(size_t)ROW_UNITS * L0].x);
     row_regs row = load_row(a.blocks, R);
-    // particle::step (src/monte_carlo/particle.cpp:57-80): every hop due by the end of the step
+    const double y_old = a.s.y[e];
+    double tl = FLY ? a.s.t_last[e] : 0.0;
+    flight_state g{};
+    if (FLY)
+      g = flight_state{R, row.site, row.x, row.y, row.z, a.s.s_off[e], (uint32_t)a.s.heading[e],
+                       0.0, 0.0, 0.0, 0u, 0ull};
     uint64_t made = 0;
-    while (t <= a.t_end && !trapped(row.n, row.gamma)) {
+    while (t <= a.t_end) {
+      if (FLY) {
+        if (!fly(a.fv, g, t - tl, id, h)) {
+          atomicOr(a.err, CONTACT_ERR_FLY);
+          break;
+        }
+        if (g.R != R) {
+          R = g.R;
+          row = load_row(a.blocks, R);
+        }
+        if (trapped(row.n, row.gamma)) {
+          // no rates: the particle stays on its tube and flies on (scatterer.cpp:14-15)
+          tl = t;
+          t = INFINITY;
+          ++h;
+          break;
+        }
+      } else if (trapped(row.n, row.gamma)) {
+        break;
+      }
       if (made == a.max_hops) {
         atomicOr(a.err, CONTACT_ERR_HOPS);
         break;
@@ -1300,14 +1327,32 @@ __global__ __launch_bounds__(256) void k_contact_step(contact_args a) {
       R = target_of(a.blocks, R, row, idx) & ~EDGE_DRAIN_BIT;
       row = load_row(a.blocks, R);
       const uint32_t r = nonzero_draw(w.v[1], w.v[3], a.seed, id, h, STREAM_HOP_REDRAW);
+      if (FLY) {
+        g = flight_state{R, row.site, row.x, row.y, row.z, 0.0, g.heading, 0.0, 0.0, 0.0, 0u, 0ull};
+        tl = t;
+      }
       t = t + dwell_time(row.n, row.gamma, r);
       ++h;
       ++made;
     }
-    a.s.site[e] = (int32_t)row.site;
+    double y_new;
+    if (FLY) {
+      if (!fly(a.fv, g, a.t_end - tl, id, h)) atomicOr(a.err, CONTACT_ERR_FLY);
+      tl = a.t_end;
+      double ox, oy, oz;
+      fly_offset(a.fv, g, ox, oy, oz);
+      y_new = a.py[g.site] + oy;
+      a.s.site[e] = (int32_t)g.site;
+      a.s.heading[e] = (uint8_t)g.heading;
+      a.s.s_off[e] = g.s;
+      a.s.t_last[e] = tl;
+    } else {
+      y_new = a.py[row.site];
+      a.s.site[e] = (int32_t)row.site;
+    }
     a.s.t_next[e] = t;
     a.s.hops[e] = h;
-    const double y_old = a.py[site0], y_new = a.py[row.site];
+    a.s.y[e] = y_new;
     const int i_new = contact_section(y_new, a.ymin, a.dy, a.n_seg);
     atomicAdd(&s_pop[i_new], 1);
     // save_currents (monte_carlo.h:598-626): interface k at ymin + dy (k + 1); +1 for a crossing
@@ -1338,7 +1383,7 @@ __global__ __launch_bounds__(256) void k_contact_keep(contact_args a, double y_l
   if (e >= a.capacity) return;
   uint32_t k = 0;
   if (e < *a.n_live) {
-    const double y = a.py[a.s.site[e]];
+    const double y = a.s.y[e];
     k = (y >= y_lo && y <= y_hi) ? 0u : 1u;
   }
   keep[e] = k;
@@ -1356,11 +1401,15 @@ __global__ __launch_bounds__(256) void k_contact_compact(int64_t capacity, conta
   dst.t_next[o] = src.t_next[e];
   dst.hops[o] = src.hops[e];
   dst.id[o] = src.id[e];
+  dst.y[o] = src.y[e];
+  dst.heading[o] = src.heading[e];
+  dst.s_off[o] = src.s_off[e];
+  dst.t_last[o] = src.t_last[e];
 }
 
 // `count` new particles in slots *base + k on sites of `sites`, as the particle constructor
 // makes them (particle.h:49-52): a random site of the list (rand() % size), the first dwell
-// from the site (ff_time).  Ids continue from *next_id.
+// from the site (ff_time), then the heading (rand() % 2).  Ids continue from *next_id.
 __global__ __launch_bounds__(256) void k_contact_inject(contact_args a, const int64_t* base,
                                                         int64_t count,
                                                         const int32_t* __restrict__ sites,
@@ -1389,6 +1438,10 @@ __global__ __launch_bounds__(256) void k_contact_inject(contact_args a, const in
   a.s.t_next[slot] = t_now + dwell_time(hh.x, __hiloint2double((int)hh.w, (int)hh.z), r);
   a.s.hops[slot] = 0;
   a.s.id[slot] = id;
+  a.s.y[slot] = a.py[site];
+  a.s.heading[slot] = (uint8_t)(r31(w.v[2]) & 1u);
+  a.s.s_off[slot] = 0.0;
+  a.s.t_last[slot] = t_now;
 }
 
 __global__ void k_contact_finish(const int64_t* base, int64_t count, int64_t capacity,
@@ -1612,10 +1665,13 @@ hipError_t launch_sum_counts(int64_t n, int64_t stride, const uint32_t* copies, 
   return hipGetLastError();
 }
 
-hipError_t launch_contact_step(const contact_args& a, hipStream_t st) {
+hipError_t launch_contact_step(const contact_args& a, bool fly, hipStream_t st) {
   if (a.capacity == 0) return hipSuccess;
   if (a.n_seg < 1 || a.n_seg > CONTACT_MAX_SEG) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_contact_step, grid_for(a.capacity), dim3(256), 0, st, a);
+  if (fly)
+    hipLaunchKernelGGL(k_contact_step<true>, grid_for(a.capacity), dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL(k_contact_step<false>, grid_for(a.capacity), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 

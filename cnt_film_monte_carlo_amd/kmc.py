@@ -116,7 +116,7 @@ _SIGS = {
     "kmc_contact_run": (C.c_int, [_P, C.c_double, C.c_int64, C.c_uint64, _I32, _I32, _I64]),
     "kmc_contact_sections": (C.c_int, [_P, _D, _D, _D, _D]),
     "kmc_contact_particles": (C.c_int, [_P, C.c_int64, _I64, C.POINTER(C.c_uint32), _I32, _D,
-                                        C.POINTER(C.c_uint64)]),
+                                        C.POINTER(C.c_uint64), _D]),
     "kmc_autotune_schedule": (C.c_int, [_P, C.c_int64, C.c_int32, _I64, _I64, _D]),
     "kmc_ensemble_create": (C.c_int, [C.c_int32, C.POINTER(_P), C.POINTER(_P)]),
     "kmc_ensemble_destroy": (C.c_int, [_P]),
@@ -408,7 +408,7 @@ class Context:
         self._ck(self.lib.kmc_get_flight(self.h, _ptr(off, _D), _ptr(hd, _U8), _ptr(tl, _D)))
         return dict(offset=off, heading=hd, t_last=tl)
 
-    # ---- contact (steady-state) mode (monte_carlo.h:157-195, 443-684), velocity 0 ----
+    # ---- contact (steady-state) mode (monte_carlo.h:157-195, 443-684) ----
     def contact_init(self, n_seg: int, c1_pop: int, c2_pop: int, capacity: int):
         """Sections, contact site lists, areas and the initial linear profile; see
         kmc_contact_init."""
@@ -435,18 +435,20 @@ class Context:
         return area, v[0].value, v[1].value, v[2].value
 
     def contact_particles(self) -> dict:
-        """The live particles in slot order: id, site, t_next, hops."""
+        """The live particles in slot order: id, site, t_next, hops, y."""
         n = C.c_int64()
-        self._ck(self.lib.kmc_contact_particles(self.h, 0, C.byref(n), None, None, None, None))
+        self._ck(self.lib.kmc_contact_particles(self.h, 0, C.byref(n), None, None, None, None,
+                                                None))
         N = n.value
         pid = np.zeros(N, np.uint32)
         site = np.zeros(N, np.int32)
         t = np.zeros(N)
         hops = np.zeros(N, np.uint64)
+        y = np.zeros(N)
         self._ck(self.lib.kmc_contact_particles(
             self.h, N, C.byref(n), pid.ctypes.data_as(C.POINTER(C.c_uint32)), _ptr(site, _I32),
-            _ptr(t, _D), hops.ctypes.data_as(C.POINTER(C.c_uint64))))
-        return dict(id=pid, site=site, t_next=t, hops=hops)
+            _ptr(t, _D), hops.ctypes.data_as(C.POINTER(C.c_uint64)), _ptr(y, _D)))
+        return dict(id=pid, site=site, t_next=t, hops=hops, y=y)
 
     def get_excitons(self):
         E = self.n_exc

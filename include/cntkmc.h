@@ -160,7 +160,7 @@ int kmc_set_velocity(kmc_ctx* ctx, double velocity);
  * along that segment.  NULL = skip. */
 int kmc_get_flight(kmc_ctx* ctx, double* offset, uint8_t* heading, double* t_last);
 
-/* ---- contact (steady-state) mode, velocity 0 ---------------------------------------------
+/* ---- contact (steady-state) mode ---------------------------------------------------------
  * The reference's second workflow: init(), step(), save_metrics() and repopulate_contacts()
  * (src/monte_carlo/monte_carlo.h:157-195, 274-316, 343-355, 443-684; driven by main.cpp:91-99).
  * The y range of the sites' bounding box is cut into n_seg sections; section 0 is a contact
@@ -170,13 +170,17 @@ int kmc_get_flight(kmc_ctx* ctx, double* offset, uint8_t* heading, double* t_las
  * bounds).  kmc_contact_init makes the initial linear profile of create_particles
  * (round(c1_pop + i (c2_pop - c1_pop) / (n_seg - 1)) particles on sites with y in
  * [ymin + i dy, ymin + i dy + dy)) and the section areas of get_area.  Needs a complete CSR;
- * capacity bounds the live particles.  Draws are keyed by a particle id (unique over the run)
- * and the particle's hop count; the reference's rand() order is not reproduced. */
+ * capacity bounds the live particles.  With a velocity set (kmc_set_velocity, before this
+ * call) the particles drift along their tubes as in particle::fly (particle.cpp:9-54): up to
+ * each hop, and to the end of every step, where the flight is taken; a particle's y is then
+ * its site's plus the offset along the tube, and the sites may carry no KMC_SITE_DRAIN flag.
+ * Draws are keyed by a particle id (unique over the run) and the particle's hop count; the
+ * reference's rand() order is not reproduced. */
 int kmc_contact_init(kmc_ctx* ctx, int32_t n_seg, int64_t c1_pop, int64_t c2_pop,
                      int64_t capacity);
 /* n_steps steps of dt.  In each, every particle makes the hops due by the step's end
  * (particle::step; more than max_hops_per_step hops of one particle in one step is an error,
- * KMC_ERR_CAPACITY); then population[step][n_seg] receives the particles per section
+ * KMC_ERR_CAPACITY, as is a flight passing more than 2^22 sites); then population[step][n_seg] receives the particles per section
  * (save_population_profile, section = (int)((y - ymin) / dy) clamped) and
  * crossings[step][n_seg - 1] the net crossings of the interfaces ymin + (k + 1) dy between
  * the particles' positions at the start and the end of the step (save_currents: +1 upward,
@@ -189,10 +193,10 @@ int kmc_contact_run(kmc_ctx* ctx, double dt, int64_t n_steps, uint64_t max_hops_
  * simulated time; any output may be NULL. */
 int kmc_contact_sections(const kmc_ctx* ctx, double* area, double* ymin, double* dy,
                          double* time);
-/* The live particles in slot order: id, site, time of the next hop and hops made.  With every
- * array NULL only *n is set. */
+/* The live particles in slot order: id, site, time of the next hop, hops made and y (FP64).
+ * With every array NULL only *n is set. */
 int kmc_contact_particles(kmc_ctx* ctx, int64_t capacity, int64_t* n, uint32_t* id,
-                          int32_t* site, double* t_next, uint64_t* hops);
+                          int32_t* site, double* t_next, uint64_t* hops, double* y);
 /* Autotuner of that schedule (no reference counterpart): runs trial_hops hops from the current
  * state once per candidate resort_hops value (counts into scratch, no samples), restores the
  * state, installs the fastest candidate.  best and hops_per_s[n_cand] may be NULL. */

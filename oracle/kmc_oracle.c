@@ -948,10 +948,14 @@ typedef struct {
   int32_t site;
   double t_next;
   uint64_t hops;
+  double y;         /* FP64 y: the site's, plus the flight offset when v > 0 */
+  unsigned heading; /* v > 0: the particle's heading, drawn at creation (particle.h:51) */
+  double off;       /* v > 0: distance beyond the site toward the next one */
+  double t_last;    /* v > 0: time up to which the flight is taken */
 } orc_cparticle;
 
 static int orc_new_particle(uint64_t seed, uint32_t id, const int32_t* list, int64_t n_list,
-                            const int64_t* row_ptr, const double* cum, double t_now,
+                            const int64_t* row_ptr, const double* cum, const double* y, double t_now,
                             orc_cparticle* p) {
   uint32_t w[4], r;
   int32_t s;
@@ -963,15 +967,24 @@ static int orc_new_particle(uint64_t seed, uint32_t id, const int32_t* list, int
   p->site = s;
   p->t_next = t_now + orc_dwell(cum + row_ptr[s], row_ptr[s + 1] - row_ptr[s], r);
   p->hops = 0;
+  p->y = y[s];
+  p->heading = (w[2] >> 1) & 1u;
+  p->off = 0.0;
+  p->t_last = t_now;
   return 0;
 }
 
-/* Returns 0, -1 for a bad argument or an empty list, -2 when the particles exceed capacity. */
+/* velocity > 0 adds particle::fly (particle.cpp:9-54) with the links left / right over the FP32
+ * positions pos32 [n][3] (per_len: periods, 0 = open): up to each hop, and to the end of every
+ * step, where the flight is taken.  Returns 0, -1 for a bad argument or an empty list, -2 when
+ * the particles exceed capacity, -3 when a flight passes more than ORC_FLY_MAX_PASSES sites. */
 int orc_contact(int64_t n_sites, const double* x, const double* y, const double* z,
                 const int64_t* row_ptr, const int32_t* col, const double* cum, uint64_t seed,
                 int n_seg, int64_t c1_pop, int64_t c2_pop, int64_t capacity, double dt,
-                int64_t n_steps, int32_t* pop, int32_t* cur, int64_t* n_live, double* area,
-                int64_t* n_final, uint32_t* p_id, int32_t* p_site, double* p_t, uint64_t* p_hops) {
+                int64_t n_steps, double velocity, const int32_t* left, const int32_t* right,
+                const float* pos32, const double* per_len, int32_t* pop, int32_t* cur,
+                int64_t* n_live, double* area, int64_t* n_final, uint32_t* p_id, int32_t* p_site,
+                double* p_t, uint64_t* p_hops, double* p_y) {
   double lo[3], hi[3], ymin, ymax, dy, dp, T = 0.0;
   double *xmn, *xmx, *zmn, *zmx;
   int32_t *c1, *cn, *sec;
@@ -979,7 +992,21 @@ int orc_contact(int64_t n_sites, const double* x, const double* y, const double*
   uint32_t next_id = 0;
   orc_cparticle *P, *Q;
   int a, side, ret = 0;
+  orc_run_args ra;
+  uint8_t* no_flags;
   if (n_sites <= 0 || n_seg < 2 || capacity <= 0) return -1;
+  if (velocity > 0.0 && (!left || !right || !pos32 || !per_len)) return -1;
+  /* the flight's view of the network: links, FP32 positions, periods; no drain sites */
+  memset(&ra, 0, sizeof(ra));
+  no_flags = calloc((size_t)n_sites, 1);
+  ra.n_sites = n_sites;
+  ra.pos = pos32;
+  ra.flags = no_flags;
+  ra.seed = seed;
+  ra.velocity = velocity;
+  ra.left = left;
+  ra.right = right;
+  for (a = 0; a < 3; ++a) ra.per_len[a] = per_len ? per_len[a] : 0.0;
   /* find_simulation_domain (monte_carlo.h:328-340) */
   for (a = 0; a < 3; ++a) {
     const double* v = a == 0 ? x : (a == 1 ? y : z);
@@ -1027,7 +1054,7 @@ int orc_contact(int64_t n_sites, const double* x, const double* y, const double*
       if (y1 <= y[s] && y[s] < y2) sec[ns++] = (int32_t)s;
     for (k = 0; k < want && ret == 0; ++k) {
       if (np_ >= capacity) { ret = -2; break; }
-      if (orc_new_particle(seed, next_id++, sec, ns, row_ptr, cum, 0.0, &P[np_++]) != 0) ret = -1;
+      if (orc_new_particle(seed, next_id++, sec, ns, row_ptr, cum, y, 0.0, &P[np_++]) != 0) ret = -1;
     }
   }
   for (step = 0; step < n_steps && ret == 0; ++step) {
@@ -1039,23 +1066,62 @@ int orc_contact(int64_t n_sites, const double* x, const double* y, const double*
     /* monte_carlo::step -> particle::step (particle.cpp:57-80) */
     for (k = 0; k < np_; ++k) {
       orc_cparticle* p = &P[k];
-      const double y_old = y[p->site];
+      const double y_old = p->y;
       double y_new;
       int j, q;
+      orc_flight f;
+      f.s = p->site;
+      f.off = p->off;
+      f.heading = p->heading;
+      f.d[0] = f.d[1] = f.d[2] = 0.0;
+      f.nd = 0;
+      f.hs = 0;
       while (p->t_next <= t_end) {
-        const int64_t b = row_ptr[p->site], n = row_ptr[p->site + 1] - b;
+        int64_t b, n;
         uint32_t w[4], r;
         double dice;
-        if (n == 0 || !(cum[b + n - 1] > 0.0)) break;          /* no rates: stays */
+        if (velocity > 0.0) {
+          /* the flight up to the hop; the hop is from the site reached */
+          if (!orc_fly(&ra, &f, p->t_next - p->t_last, p->id, p->hops)) { ret = -3; break; }
+          p->site = f.s;
+        }
+        b = row_ptr[p->site];
+        n = row_ptr[p->site + 1] - b;
+        if (n == 0 || !(cum[b + n - 1] > 0.0)) {                /* no rates: stays */
+          if (velocity > 0.0) {
+            p->t_last = p->t_next;
+            p->t_next = INFINITY;
+            p->hops++;
+          }
+          break;
+        }
         draws(seed, p->id, p->hops, 0u, w);
         dice = (cum[b + n - 1] * (double)(w[0] >> 1)) / ORC_RAND_MAX;
         p->site = col[b + orc_select(cum + b, (unsigned)n, dice)];
         r = dwell_draw(seed, p->id, p->hops, w[1], w[3], 2u);
+        if (velocity > 0.0) {
+          f.s = p->site;
+          f.off = 0.0;
+          p->t_last = p->t_next;
+        }
         p->t_next = p->t_next + orc_dwell(cum + row_ptr[p->site],
                                           row_ptr[p->site + 1] - row_ptr[p->site], r);
         p->hops++;
       }
-      y_new = y[p->site];
+      if (velocity > 0.0) {
+        double o[3];
+        /* the rest of the step, taken (particle::fly(dt), particle.cpp:77) */
+        if (!orc_fly(&ra, &f, t_end - p->t_last, p->id, p->hops)) ret = -3;
+        p->t_last = t_end;
+        p->site = f.s;
+        p->off = f.off;
+        p->heading = f.heading;
+        orc_fly_offset(&ra, &f, o);
+        y_new = y[p->site] + o[1];
+      } else {
+        y_new = y[p->site];
+      }
+      p->y = y_new;
       /* save_population_profile (monte_carlo.h:565-571) */
       j = (int)((y_new - ymin) / dy);
       j = j < 0 ? 0 : (j < n_seg ? j : n_seg - 1);
@@ -1076,13 +1142,13 @@ int orc_contact(int64_t n_sites, const double* x, const double* y, const double*
       const int64_t want = side == 0 ? c1_pop : c2_pop;
       int64_t nk = 0;
       for (k = 0; k < np_; ++k) {
-        const double yy = y[P[k].site];
+        const double yy = P[k].y;
         if (!(yy >= ylo && yy <= yhi)) Q[nk++] = P[k];
       }
       for (k = 0; k < want; ++k) {
         if (nk >= capacity) { ret = -2; break; }
         if (orc_new_particle(seed, next_id++, side == 0 ? c1 : cn, side == 0 ? n1 : nn, row_ptr,
-                             cum, T, &Q[nk++]) != 0) { ret = -1; break; }
+                             cum, y, T, &Q[nk++]) != 0) { ret = -1; break; }
       }
       memcpy(P, Q, sizeof(orc_cparticle) * nk);
       np_ = nk;
@@ -1095,8 +1161,9 @@ int orc_contact(int64_t n_sites, const double* x, const double* y, const double*
     p_site[k] = P[k].site;
     p_t[k] = P[k].t_next;
     p_hops[k] = P[k].hops;
+    p_y[k] = P[k].y;
   }
-  free(c1); free(cn); free(sec); free(P); free(Q);
+  free(c1); free(cn); free(sec); free(P); free(Q); free(no_flags);
   return ret;
 }
 

@@ -79,7 +79,8 @@ def library() -> C.CDLL:
             "orc_max_threads": (C.c_int, []),
             "orc_contact": (C.c_int, [C.c_int64, _D, _D, _D, _I64, _I32, _D, C.c_uint64, C.c_int,
                                       C.c_int64, C.c_int64, C.c_int64, C.c_double, C.c_int64,
-                                      _I32, _I32, _I64, _D, _I64, _U32, _I32, _D, _U64]),
+                                      C.c_double, _I32, _I32, _F, _D,
+                                      _I32, _I32, _I64, _D, _I64, _U32, _I32, _D, _U64, _D]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)
@@ -359,12 +360,22 @@ def first_order(donor: dict, acceptor: dict, temperature: float, broadening: flo
 
 
 def contact(net: dict, csr: dict, seed: int, n_seg: int, c1_pop: int, c2_pop: int,
-            capacity: int, dt: float, n_steps: int) -> dict:
+            capacity: int, dt: float, n_steps: int, velocity: float = 0.0) -> dict:
     """Contact (steady-state) mode over the oracle CSR (orc_contact: init, step, save_metrics,
     repopulate_contacts of src/monte_carlo/monte_carlo.h, written literally): per step the
     population [n_steps, n_seg], the interface crossings [n_steps, n_seg - 1] and the live count;
-    the section areas; the final particles (id, site, t_next, hops) in list order."""
+    the section areas; the final particles (id, site, t_next, hops, y) in list order.  velocity > 0
+    adds the drift along the tubes (particle::fly) over net's left / right links, the FP32
+    csr["site_pos"] and the periods csr["period"]."""
     lib = library()
+    velocity = float(velocity)
+    if velocity > 0.0:
+        left, right = _c(net["left"], np.int32), _c(net["right"], np.int32)
+        spos = _c(csr["site_pos"], np.float32)
+        per = _c(np.asarray(csr.get("period", (0.0, 0.0, 0.0)), np.float64), np.float64)
+        fly = (velocity, _p(left, _I32), _p(right, _I32), _p(spos, _F), _p(per, _D))
+    else:
+        fly = (0.0, None, None, None, None)
     pos = [_c(net[k], np.float64) for k in ("x", "y", "z")]
     row = _c(csr["row_ptr"], np.int64)
     col = _c(csr["col"], np.int32)
@@ -378,14 +389,15 @@ def contact(net: dict, csr: dict, seed: int, n_seg: int, c1_pop: int, c2_pop: in
     site = np.zeros(capacity, np.int32)
     t = np.zeros(capacity)
     hops = np.zeros(capacity, np.uint64)
+    yy = np.zeros(capacity)
     rc = lib.orc_contact(pos[0].size, *[_p(v, _D) for v in pos], _p(row, _I64), _p(col, _I32),
-                         _p(cum, _D), seed, n_seg, c1_pop, c2_pop, capacity, dt, n_steps,
+                         _p(cum, _D), seed, n_seg, c1_pop, c2_pop, capacity, dt, n_steps, *fly,
                          _p(pop, _I32), _p(cur, _I32), _p(live, _I64), _p(area, _D),
                          _p(n_final, _I64), _p(pid, _U32), _p(site, _I32), _p(t, _D),
-                         _p(hops, _U64))
+                         _p(hops, _U64), _p(yy, _D))
     if rc != 0:
         raise RuntimeError(f"orc_contact failed ({rc})")
     n = int(n_final[0])
     return dict(population=pop, crossings=cur, n_live=live, area=area, id=pid[:n],
-                site=site[:n], t_next=t[:n], hops=hops[:n])
+                site=site[:n], t_next=t[:n], hops=hops[:n], y=yy[:n])
 

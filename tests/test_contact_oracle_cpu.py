@@ -58,3 +58,54 @@ def test_fick_known_diffusion_oracle(orc, kmc):
     # ny / n_seg = 6 layers each, so the section centres the gradient uses are 3.4 % closer than
     # the layers they average; the rest is the step-wise contact reset and noise
     assert abs(D / (k * a * a) - 1) < 0.1
+
+
+def test_drift_vanishing_velocity_is_hopping(orc, kmc):
+    # v -> 0: no flight reaches the next site, so the run is the hopping run with y unchanged
+    a, k = 10 * NM, 1e12
+    nx, ny, nz = 3, 30, 3
+    net, csr = _lattice(orc, nx, ny, nz, a, k)
+    _tubes_along_y(net, ny, nz)
+    r0 = orc.contact(net, csr, 5, 5, 120, 10, 5000, 2e-12, 60)
+    r1 = orc.contact(net, csr, 5, 5, 120, 10, 5000, 2e-12, 60, velocity=1e-30)
+    for key in ("population", "crossings", "n_live", "id", "site", "hops", "t_next", "y"):
+        assert np.array_equal(r0[key], r1[key]), key
+
+
+def _tubes_along_y(net, ny, nz):
+    # every (x, z) column a tube along y: right = the next site up, left = the one down
+    idx = np.arange(net["n"])
+    iy = (idx // nz) % ny
+    net["right"] = np.where(iy < ny - 1, idx + nz, -1).astype(np.int32)
+    net["left"] = np.where(iy > 0, idx - nz, -1).astype(np.int32)
+
+
+def test_drift_pure_flight_closed_form(orc):
+    """No rates: every particle only flies (particle::fly, particle.cpp:9-54) along its straight
+    tube at v, turning round at the tube ends, so y(t) = fold(y0 + s v t) with fold the
+    reflection into [0, L].  A tiny first step gives each particle's start y0 and heading s; a
+    second run from the same seed, one step of 2.3 a / v, must put it at fold(y0 + 2.3 s a)."""
+    a, v = 10 * NM, 1.0
+    nx, ny, nz = 3, 40, 3
+    net, csr = _lattice(orc, nx, ny, nz, a, 0.0)
+    _tubes_along_y(net, ny, nz)
+    L = (ny - 1) * a
+    tiny = orc.contact(net, csr, 9, 4, 50, 50, 2000, 0.01 * a / v, 1, velocity=v)
+    off = tiny["y"] - net["y"][tiny["site"]]
+    assert np.all(np.isinf(tiny["t_next"]))
+    n0 = int(tiny["population"][0].sum())                    # the initial particles: ids < n0
+    first = tiny["id"] < n0
+    assert np.allclose(np.abs(off[first]), 0.01 * a, rtol=1e-5)
+    s = np.sign(off[first])
+    assert 0.3 < np.mean(s > 0) < 0.7                         # both headings drawn
+    y0 = dict(zip(tiny["id"][first].tolist(), zip(net["y"][tiny["site"][first]], s)))
+    far = orc.contact(net, csr, 9, 4, 50, 50, 2000, 2.3 * a / v, 1, velocity=v)
+    checked = 0
+    for pid, y in zip(far["id"].tolist(), far["y"]):
+        if pid >= n0 or pid not in y0:
+            continue
+        u = (y0[pid][0] + y0[pid][1] * 2.3 * a) % (2 * L)
+        want = L - abs(u - L)
+        assert abs(y - want) < 1e-4 * a, (pid, y, want)
+        checked += 1
+    assert checked > 40

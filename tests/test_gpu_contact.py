@@ -16,11 +16,18 @@ def say(*a):
     print("[gpu-contact]", *a, file=sys.stderr, flush=True)
 
 
-def _film(kmc, nw, seed):
+def _film(kmc, nw, seed, velocity=0.0, open_tubes=False):
     # the 5k-site film with open axes (transport along y) and no Kubo flags
     w = nw.film_small()
     net = nw.build_network(w)
     net["flags"] = np.zeros(net["n"], np.uint8)
+    if open_tubes:
+        # the tubes wrap around the film's y period; with open axes that link is a ~500 nm
+        # segment across the film, so the drift tests cut it and the tubes end at the edges
+        for side, other in (("left", "right"), ("right", "left")):
+            lk = net[side]
+            far = (lk >= 0) & (np.abs(net["y"][np.maximum(lk, 0)] - net["y"]) > 0.25 * np.ptp(net["y"]))
+            net[side] = np.where(far, -1, lk).astype(np.int32)
     ctx = kmc.Context(kmc.Config(cutoff=w.cutoff, min_distance=0.4 * NM, seed=seed))
     ctx.set_sites(net["x"], net["y"], net["z"], net["ox"], net["oy"], net["oz"], net["left"],
                   net["right"], net["chirality"], net["flags"])
@@ -28,6 +35,8 @@ def _film(kmc, nw, seed):
     ctx.set_rate_table(*tab)
     ctx.build_cells()
     ctx.build_rates(kmc.RATE_TABLE)
+    if velocity > 0.0:
+        ctx.set_velocity(velocity)
     return w, net, ctx, tab
 
 
@@ -59,6 +68,55 @@ def test_contact_matches_oracle(kmc, orc, nw, n_seg, c1, c2, dt, steps):
     assert np.array_equal(pop[1:].sum(1), live[:-1])
     assert int(p["hops"].sum()) > 0 and np.any(cur != 0)
     say(f"  live {live[-1]}, hops {int(p['hops'].sum())}, crossings {int(np.abs(cur).sum())}")
+    ctx.close()
+
+
+@pytest.mark.parametrize("v", [1.0, 10.0])
+def test_contact_drift_matches_oracle(kmc, orc, nw, v):
+    """v > 0: particle::fly (particle.cpp:9-54) to each hop and to every step's end, the
+    particle's y = its site's y plus the offset along the segment toward the next site."""
+    n_seg, c1, c2, dt, steps = 10, 400, 40, 2e-8, 80
+    say(f"contact mode with drift v={v} m/s vs oracle: {steps} steps of {dt}")
+    w, net, ctx, tab = _film(kmc, nw, seed=31, velocity=v, open_tubes=True)
+    cap = 8000
+    ctx.contact_init(n_seg, c1, c2, cap)
+    pop, cur, live = ctx.contact_run(dt, steps // 2)
+    pop2, cur2, live2 = ctx.contact_run(dt, steps - steps // 2)
+    pop, cur, live = np.vstack([pop, pop2]), np.vstack([cur, cur2]), np.concatenate([live, live2])
+    onet = orc.Net(net, w.cutoff, 0.4 * NM)
+    onet.set_table(*tab)
+    ref = orc.contact(net, onet.csr(0), 31, n_seg, c1, c2, cap, dt, steps, velocity=v)
+    assert np.array_equal(pop, ref["population"])
+    assert np.array_equal(cur, ref["crossings"])
+    assert np.array_equal(live, ref["n_live"])
+    p = ctx.contact_particles()
+    assert np.array_equal(p["id"], ref["id"]) and np.array_equal(p["site"], ref["site"])
+    assert np.array_equal(p["hops"], ref["hops"])
+    # times (and so the flights' lengths) agree to rounding: the dwell times' logarithms are
+    # the device's and libm's, equal to within an ulp or two
+    np.testing.assert_allclose(p["y"], ref["y"], rtol=1e-12, atol=0)
+    np.testing.assert_allclose(p["t_next"], ref["t_next"], rtol=1e-12)
+    off = p["y"] - net["y"][p["site"]]
+    assert np.mean(off != 0) > 0.5 and np.abs(off).max() <= 10.01 * NM   # between sites
+    say(f"  live {live[-1]}, hops {int(p['hops'].sum())}, crossings {int(np.abs(cur).sum())}")
+    ctx.close()
+
+
+def test_contact_drift_refuses_drain_sites(kmc, nw):
+    say("contact mode with drift refuses Kubo drain flags")
+    w, net, ctx, tab = _film(kmc, nw, seed=3)
+    ctx.close()
+    net["flags"][:10] = kmc.SITE_DRAIN
+    ctx = kmc.Context(kmc.Config(cutoff=w.cutoff, min_distance=0.4 * NM, seed=3))
+    ctx.set_sites(net["x"], net["y"], net["z"], net["ox"], net["oy"], net["oz"], net["left"],
+                  net["right"], net["chirality"], net["flags"])
+    ctx.set_rate_table(*tab)
+    ctx.build_cells()
+    ctx.build_rates(kmc.RATE_TABLE)
+    ctx.contact_init(4, 100, 0, 300)                 # v = 0: drain flags play no part
+    ctx.set_velocity(1.0)
+    with pytest.raises(RuntimeError, match="KMC_SITE_DRAIN"):
+        ctx.contact_init(4, 100, 0, 300)
     ctx.close()
 
 
