@@ -1261,7 +1261,7 @@ hipError_t launch_reduce(const reduce_args& a, hipStream_t st) {
 // launch wrappers for (a) and (b)
 // ---------------------------------------------------------------------------------------------
 // =============================================================================================
-// Contact (steady-state) mode with v = 0: monte_carlo::step, save_population_profile,
+// Contact (steady-state) mode: monte_carlo::step, save_population_profile,
 // save_currents and repopulate_contacts (src/monte_carlo/monte_carlo.h:343-355, 443-516, 525-630).
 // A particle's draws are keyed by its id (unique over the run) and its own hop count, so slots
 // can be compacted and reused without any draw repeating.
