@@ -101,7 +101,17 @@ def rate_tables(w, exact_tables=True):
     grid = kmc.default_grids()
     sur = S.RateSurrogate.load()
     cls = [S.CHIRALITIES.index(c) for c in w.chiralities]
-    models = {c: X.CntModel(*c, 10) for c in w.chiralities}
+    # the CNT exciton models: v(q) and PI(q) on the GPU, timed beside the all-host build
+    t0 = time.perf_counter()
+    for c in w.chiralities:
+        X.CntModel(*c, 10).close()
+    host_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    models = {c: X.CntModel(*c, 10, device=0) for c in w.chiralities}
+    gpu_s = time.perf_counter() - t0
+    log(f"CNT models {w.chiralities}: {gpu_s * 1e3:.1f} ms with the GPU sums, "
+        f"{host_s * 1e3:.1f} ms on the host cores")
+    cnt_s = dict(gpu=gpu_s, host=host_s, host_cores=len(os.sched_getaffinity(0)))
     trs = []
     for ci in w.chiralities:
         for cj in w.chiralities:
@@ -133,7 +143,7 @@ def rate_tables(w, exact_tables=True):
         log(f"tables: surrogate {mlp_s * 1e3:.1f} ms, {fallback}/{mlp.size} nodes exact")
         return dict(grid=grid, exact=None, mlp=mlp, exact_s=None, mlp_s=mlp_s, fallback=fallback,
                     nodes=int(mlp.size), err_median=None, err_p99=None, err_over=None,
-                    surrogate=sur, transfers=trs, models=models)
+                    surrogate=sur, transfers=trs, models=models, cnt_s=cnt_s)
     err = S.relative_error(np.log10(np.maximum(mlp, 1e-300)), np.log10(np.maximum(exact, 1e-300)))
     log(f"tables: exact {exact_s:.3f} s, surrogate {mlp_s * 1e3:.1f} ms ({exact_s / mlp_s:.0f}x), "
         f"{fallback}/{exact.size} nodes exact, error median {np.median(err):.4f} p99 "
@@ -141,7 +151,7 @@ def rate_tables(w, exact_tables=True):
     return dict(grid=grid, exact=exact, mlp=mlp, exact_s=exact_s, mlp_s=mlp_s, fallback=fallback,
                 nodes=int(exact.size), err_median=float(np.median(err)),
                 err_p99=float(np.quantile(err, 0.99)), err_over=float(np.mean(err > 0.05)),
-                surrogate=sur, transfers=trs, models=models)
+                surrogate=sur, transfers=trs, models=models, cnt_s=cnt_s)
 
 
 def exact_table_run(w, net, tabs, E, args, t_sample_max, D):
@@ -492,6 +502,10 @@ def main():
                     "inference plus exact recomputation of flagged nodes; wall time incl. "
                     "host<->device copies",
         },
+        "cnt_model_build_s": dict(tabs["cnt_s"], note=(
+            "exciton dispersion of the film's chiralities (calculate_exciton_dispersion): the "
+            "Ohno transform v(q) and polarization PI(q) on the GPU vs the whole build on the "
+            "host cores; setup, outside the timed region")),
         "csr_build_s": build_s,
         "diffusion_m2_per_s": D,
         "diffusion_ci95": ci,

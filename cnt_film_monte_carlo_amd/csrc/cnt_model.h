@@ -70,12 +70,20 @@ struct cnt_model {
   double length_in_meter() const;   // length * |t_vec| (cnt.h:318-320)
 
   // Build everything (calculate_exciton_dispersion, cnt.cpp:1056-1081).  threads <= 0: all.
-  // Returns an empty string on success, else an error message.
-  std::string build(int n_, int m_, int length_, int threads);
+  // device >= 0 computes v(q) and PI(q) on that GPU, the rest on the host.  Returns an empty
+  // string on success, else an error message.
+  std::string build(int n_, int m_, int length_, int threads, int device = -1);
   // The A2 singlet kernel (kernel_11 + kernel_12 + 2 kernel_exchange) at one ik_cm_idx,
   // column-major nk_relev x nk_relev (for tests of the eigensolver).
   std::vector<cplx> singlet_kernel(int ik_cm_idx) const;
 };
+
+// v(q) [nq][nmu_q][4] and PI(q) [nq][nmu_q] of m (its bands, wavefunctions and q grid) on the
+// GPU `device`, from the relative atom positions rel [4][natom][2] and the Ohno denominators
+// den [4][natom]; norm = 2 Nu ncells.  Returns an empty string on success.
+std::string cnt_sums_on_device(int device, const cnt_model& m, int natom, const double* rel,
+                               const double* den, double norm, std::vector<cplx>& vq,
+                               std::vector<double>& PI);
 
 }  // namespace cntkmc
 

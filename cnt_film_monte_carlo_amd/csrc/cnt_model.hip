@@ -201,7 +201,7 @@ void hermitian_eig(int n, const std::vector<cplx>& a_in, std::vector<double>& ev
 double cnt_model::Au() const { return std::fabs(a1.x * a2.y - a1.y * a2.x); }
 double cnt_model::length_in_meter() const { return double(length) * norm2v(t_vec); }
 
-std::string cnt_model::build(int n_, int m_, int length_, int threads) {
+std::string cnt_model::build(int n_, int m_, int length_, int threads, int device) {
   using namespace phys;
   n = n_;
   m = m_;
@@ -407,8 +407,22 @@ std::string cnt_model::build(int n_, int m_, int length_, int threads) {
       for (size_t k = 0; k < rel[s].size(); k++)
         den[s][k] = Upp / std::sqrt(coeff * (std::pow(rel[s][k].x, 2) + std::pow(rel[s][k].y, 2)) + 1);
     }
-    vq.assign((size_t)nq * nmu_q * 4, cplx(0, 0));
     const double norm = double(2 * Nu * ncells);
+    if (device >= 0) {
+      // v(q) and PI(q) on the GPU (cnt_gpu.hip): positions [4][natom][2], denominators [4][natom]
+      const int natom = (int)rel[0].size();
+      std::vector<double> rel_flat((size_t)4 * natom * 2), den_flat((size_t)4 * natom);
+      for (int s = 0; s < 4; s++)
+        for (int k = 0; k < natom; k++) {
+          rel_flat[((size_t)s * natom + k) * 2] = rel[s][k].x;
+          rel_flat[((size_t)s * natom + k) * 2 + 1] = rel[s][k].y;
+          den_flat[(size_t)s * natom + k] = den[s][k];
+        }
+      const std::string e =
+          cnt_sums_on_device(device, *this, natom, rel_flat.data(), den_flat.data(), norm, vq, PI);
+      if (!e.empty()) return e;
+    } else {
+    vq.assign((size_t)nq * nmu_q * 4, cplx(0, 0));
     parallel_for(nq, threads, [&](int iqi) {
       const int iq = iqi - (nk - 1);
       for (int mi = 0; mi < nmu_q; mi++) {
@@ -422,13 +436,15 @@ std::string cnt_model::build(int n_, int m_, int length_, int threads) {
         }
       }
     });
+    }
   }
 
   // ---- calculate_polarization (cnt.cpp:634-732) and calculate_dielectric (cnt.cpp:735-780)
-  PI.assign((size_t)nq * nmu_q, 0.0);
   eps.assign((size_t)nq * nmu_q, 0.0);
   {
     const int nkM = nk_K1 * M;
+    if (device < 0) {
+    PI.assign((size_t)nq * nmu_q, 0.0);
     parallel_for(nq, threads, [&](int iqi) {
       const int iq = iqi - (nk - 1);
       for (int mi = 0; mi < nmu_q; mi++) {
@@ -459,6 +475,7 @@ std::string cnt_model::build(int n_, int m_, int length_, int threads) {
         PI[(size_t)iqi * nmu_q + mi] = 2 * acc;
       }
     });
+    }
     for (size_t i = 0; i < PI.size(); i++) {
       const cplx* v = &vq[i * 4];
       const double mean = (v[0].real() + v[1].real() + v[2].real() + v[3].real()) / 4.0;

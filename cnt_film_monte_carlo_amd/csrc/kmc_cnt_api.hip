@@ -35,6 +35,21 @@ int kmc_cnt_create(int32_t n, int32_t m, int32_t length_cells, int32_t threads, 
   return KMC_OK;
 }
 
+int kmc_cnt_create_on_device(int32_t n, int32_t m, int32_t length_cells, int32_t threads,
+                             int32_t device, kmc_cnt** out) {
+  if (!out) return cfail(KMC_ERR_ARG, "kmc_cnt_create_on_device: null output");
+  *out = nullptr;
+  if (device < 0) return cfail(KMC_ERR_ARG, "kmc_cnt_create_on_device: need device >= 0");
+  kmc_cnt* c = new kmc_cnt();
+  const std::string err = c->m.build(n, m, length_cells, threads, device);
+  if (!err.empty()) {
+    delete c;
+    return cfail(err.rfind("cnt_model on the GPU", 0) == 0 ? KMC_ERR_HIP : KMC_ERR_ARG, err);
+  }
+  *out = c;
+  return KMC_OK;
+}
+
 int kmc_cnt_destroy(kmc_cnt* c) {
   delete c;
   return KMC_OK;

@@ -36,6 +36,8 @@ _I32 = C.POINTER(C.c_int32)
 _I64 = C.POINTER(C.c_int64)
 _SIGS = {
     "kmc_cnt_create": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(_P)]),
+    "kmc_cnt_create_on_device": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                           C.c_int32, C.POINTER(_P)]),
     "kmc_cnt_destroy": (C.c_int, [_P]),
     "kmc_cnt_last_error": (C.c_char_p, []),
     "kmc_cnt_get_info": (C.c_int, [_P, C.POINTER(_CntInfo)]),
@@ -93,12 +95,17 @@ def hermitian_eig(a: np.ndarray):
 
 
 class CntModel:
-    """(n, m) tube of `length` unit cells: A2 singlet exciton dispersion and wavefunctions."""
+    """(n, m) tube of `length` unit cells: A2 singlet exciton dispersion and wavefunctions.
+    device (a GPU index) takes the Coulomb transform v(q) and the polarization PI(q), the bulk
+    of the build, on that GPU (kmc_cnt_create_on_device); None builds on the host only."""
 
-    def __init__(self, n: int, m: int, length: int = 10, threads: int = 0):
+    def __init__(self, n: int, m: int, length: int = 10, threads: int = 0, device=None):
         lib = library()
         h = _P()
-        rc = lib.kmc_cnt_create(n, m, length, threads, C.byref(h))
+        if device is None:
+            rc = lib.kmc_cnt_create(n, m, length, threads, C.byref(h))
+        else:
+            rc = lib.kmc_cnt_create_on_device(n, m, length, threads, int(device), C.byref(h))
         if rc < 0:
             raise kmc.KmcError(rc, lib.kmc_cnt_last_error().decode())
         self.h = h

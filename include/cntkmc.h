@@ -344,6 +344,14 @@ typedef struct kmc_cnt_info {
 
 /* Build the model of a (n,m) tube of length_cells unit cells (threads <= 0: all host cores). */
 int kmc_cnt_create(int32_t n, int32_t m, int32_t length_cells, int32_t threads, kmc_cnt** out);
+/* The same model with its two large sums on the GPU `device`: the Ohno-potential transform v(q)
+ * (calculate_vq, cnt.cpp:520-631) and the polarization PI(q) (calculate_polarization,
+ * cnt.cpp:634-732), ~95 % of the host build's time; the rest runs on `threads` host cores.  The
+ * sums are taken in another order and with the device sincos / hypot, so v(q), PI(q) and what
+ * follows from them agree with kmc_cnt_create to rounding, not bitwise.  KMC_ERR_HIP when the
+ * device cannot be used. */
+int kmc_cnt_create_on_device(int32_t n, int32_t m, int32_t length_cells, int32_t threads,
+                             int32_t device, kmc_cnt** out);
 int kmc_cnt_destroy(kmc_cnt* c);
 const char* kmc_cnt_last_error(void);
 int kmc_cnt_get_info(const kmc_cnt* c, kmc_cnt_info* info);

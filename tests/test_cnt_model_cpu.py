@@ -131,3 +131,24 @@ def test_temperature_widens_the_window(models):
     lo = X.Transfer(m, m, 150.0).info.n_donor_states
     hi = X.Transfer(m, m, 400.0).info.n_donor_states
     assert lo <= hi and hi >= 7
+
+
+@pytest.mark.parametrize("n,m", [(4, 2), (7, 5)])
+def test_interaction_sums_vs_numpy(models, n, m):
+    # v(q) and PI(q) (cnt.cpp:520-732) against the oracle's numpy restatement over the model's
+    # atoms and bands: numpy's exp and pairwise sums, another order than the host loop.  Errors
+    # are measured against each array's scale: PI(q = 0) is a sum of exactly cancelling
+    # overlaps (|<v(k)|c(k)>| = 0), ~1e-32 of the scale, where only the absolute error means much
+    from oracle import cnt_oracle as co
+    md = models[(n, m)]
+    i = md.info
+    pa, pb, _ = md.atoms()
+    e, wf = md.electrons()
+    vq, PI, eps = md.interaction()
+    ov = co.vq(i, pa, pb)
+    op = co.polarization(i, e, wf)
+    scale = np.abs(ov).max(axis=(0, 1))
+    assert np.all(np.abs(vq - ov).max(axis=(0, 1)) < 1e-13 * scale)
+    assert np.abs(PI - op).max() < 1e-13 * op.max()
+    q0 = i.nk - 1
+    assert op[q0, i.n_mu - 1] < 1e-25 * op.max()           # q = 0: no interband overlap
