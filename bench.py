@@ -280,6 +280,63 @@ def ensemble_leg(sur, n_networks=256, hops=10_000, plain_subset=16, bins=16):
                     "D compared on the plain-run subset"}
 
 
+def contact_leg(w, net, tabs, n_seg=20, c1_pop=100_000, dwell_mult=100.0, warmup=2, steps=20):
+    """The reference's second workflow on the same film: contact (steady-state) mode with the
+    transport along y, n_seg sections, c1_pop particles held in the first contact and the last
+    emptied each step (monte_carlo.h:443-491), from the initial linear profile.  A step is
+    dwell_mult mean dwell times, so a particle makes ~dwell_mult hops per step.  Open axes: the
+    contacts are the film's two y ends."""
+    import numpy as np
+
+    from cnt_film_monte_carlo_amd import kmc
+
+    flags = np.zeros(net["n"], np.uint8)
+    ctx = kmc.Context(kmc.Config(cutoff=w.cutoff, min_distance=0.4e-9, seed=4242))
+    ctx.set_sites(net["x"], net["y"], net["z"], net["ox"], net["oy"], net["oz"], net["left"],
+                  net["right"], net["chirality"], flags)
+    th, z, a1, a2 = tabs["grid"]
+    ctx.set_rate_table(th, z, a1, a2, tabs["mlp"])
+    ctx.build_cells()
+    ctx.build_rates(kmc.RATE_TABLE)
+    csr = ctx.get_csr()
+    deg = np.diff(csr["row_ptr"])
+    gamma = csr["cum"][csr["row_ptr"][1:][deg > 0] - 1]
+    dt = dwell_mult * float(np.mean(1.0 / gamma))
+    capacity = 4 * n_seg * c1_pop // 2
+    ctx.contact_init(n_seg, c1_pop, 0, capacity)
+    ctx.contact_run(dt, warmup)
+    h0, ms0 = ctx.contact_stats()
+    ctx.sync()
+    t0 = time.perf_counter()
+    pop, cur, live = ctx.contact_run(dt, steps)
+    wall = time.perf_counter() - t0
+    h1, ms1 = ctx.contact_stats()
+    ctx.close()
+    hops = h1 - h0
+    kern_s = (ms1 - ms0) / 1e3
+    particle_steps = int(pop.sum())
+    # algorithmic bytes: a hop reads the row header and the cumulative rates (24 + 8 d, d the
+    # mean out-degree: the visit weights of contact mode are not counted) and a particle step
+    # reads its slot (site, t_next, hops, id, y: 32 B) and writes it back (28 B)
+    bpe = 24.0 + 8.0 * float(deg.mean())
+    achieved = (hops * bpe + particle_steps * 60.0) / kern_s / 1e9
+    log(f"contact mode: {steps} steps of {dt:.3e} s, {particle_steps / steps:.0f} particles, "
+        f"{hops:.3e} hops in {wall:.3f} s ({hops / wall:.3e} hops/s; step kernels "
+        f"{kern_s * 1e3:.1f} ms, {hops / kern_s:.3e} hops/s)")
+    return {
+        "sites": int(net["n"]), "n_seg": n_seg, "c1_pop": c1_pop, "c2_pop": 0,
+        "dt_s": dt, "steps": steps, "warmup": warmup,
+        "particles_mean": particle_steps / steps, "hops": int(hops), "seconds": wall,
+        "hops_per_s": hops / wall, "step_kernel_ms": kern_s * 1e3,
+        "step_kernel_hops_per_s": hops / kern_s,
+        "roofline": {"kernel": "k_contact_step", "bound": "hbm", "achieved": achieved,
+                     "peak": 8000.0, "unit": "GB/s", "frac": achieved / 8000.0,
+                     "bytes_per_hop": bpe, "bytes_per_particle_step": 60.0},
+        "note": "whole steps (particle steps + repopulation + per-step outputs) timed by the "
+                "wall clock; the step kernels by HIP events on the context stream",
+    }
+
+
 def surrogate_roofline(sur, n=1 << 22, reps=5):
     """Fused BF16 inference kernel on n pair geometries: algorithmic FLOP / kernel time."""
     import numpy as np
@@ -348,6 +405,8 @@ def main():
                     help="skip the exact table build and the exact-table comparison run")
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the 256-network ensemble leg")
+    ap.add_argument("--no-contact", action="store_true",
+                    help="skip the contact (steady-state) mode leg")
     ap.add_argument("--bins", type=int, default=64)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes (counter calibration)")
@@ -443,6 +502,7 @@ def main():
         Dx, cix, d_rel = exact_table_run(w, net, tabs, E, args, t_sample_max, D)
     sur_roof = surrogate_roofline(tabs["surrogate"])
     ens = None if args.no_ensemble else ensemble_leg(tabs["surrogate"], n_networks=n_networks)
+    contact = None if args.no_contact else contact_leg(w, net, tabs)
 
 
     # HBM bytes per hop-kernel launch measured with rocprofv3 counters on this workload
@@ -529,6 +589,7 @@ def main():
         },
         "roofline_surrogate": sur_roof,
         "ensemble": ens,
+        "contact_mode": contact,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -147,7 +147,17 @@ struct kmc_ctx {
   dbuf<int64_t> c_off, c_scan_tmp, c_live, c_live_out;
   dbuf<uint64_t> c_next;
   dbuf<int32_t> c_err, c_pop_out, c_cur_out;
+  dbuf<uint64_t> c_made;                  // hops made since kmc_contact_init
+  dbuf<uint32_t> c_skeys, c_sids;         // [2 capacity] slot sort by row: keys, slot ids
+  dbuf<uint8_t> c_sort_tmp;
+  double c_step_ms = 0.0;                 // device time of the step kernels since then
+  std::vector<hipEvent_t> c_ev;           // CONTACT_EV_PAIRS (start, stop) pairs, reused
+  std::vector<char> c_ev_busy;            // pair recorded and not yet read
 };
+
+// step-kernel timing: event pairs reused round robin; a pair is read before it is recorded
+// again, so the host waits only on a step CONTACT_EV_PAIRS steps back
+constexpr int CONTACT_EV_PAIRS = 64;
 
 namespace {
 
@@ -450,6 +460,9 @@ int kmc_destroy(kmc_ctx* c) {
   dfree(c->d_reduced); dfree(c->r_drain); dfree(c->tr_site); dfree(c->tr_time); dfree(c->tr_n);
   dfree(c->c_sites); dfree(c->c_keep); dfree(c->c_off); dfree(c->c_scan_tmp); dfree(c->c_live);
   dfree(c->c_live_out); dfree(c->c_next); dfree(c->c_err); dfree(c->c_pop_out); dfree(c->c_cur_out);
+  dfree(c->c_made); dfree(c->c_skeys); dfree(c->c_sids); dfree(c->c_sort_tmp);
+  for (hipEvent_t e : c->c_ev)
+    if (e) (void)hipEventDestroy(e);
   for (int b = 0; b < 2; ++b) {
     dfree(c->c_site[b]); dfree(c->c_t[b]); dfree(c->c_hops[b]); dfree(c->c_id[b]);
     dfree(c->c_y[b]); dfree(c->c_soff[b]); dfree(c->c_tlast[b]); dfree(c->c_head[b]);
@@ -1592,11 +1605,23 @@ contact_args contact_args_of(kmc_ctx* c) {
   a.ymin = c->c_ymin;
   a.dy = c->c_dy;
   a.err = c->c_err.p;
+  a.hops_made = reinterpret_cast<unsigned long long*>(c->c_made.p);
   if (c->c_vel > 0.0) {
     a.fv = view_of(c);
     a.fv.velocity = c->c_vel;
   }
   return a;
+}
+
+// Adds the elapsed time of event pair q to c_step_ms if it holds an unread step.
+int contact_read_pair(kmc_ctx* c, int q) {
+  if (!c->c_ev_busy[q]) return KMC_OK;
+  float ms = 0.0f;
+  KMC_HIP(c, hipEventSynchronize(c->c_ev[2 * q + 1]));
+  KMC_HIP(c, hipEventElapsedTime(&ms, c->c_ev[2 * q], c->c_ev[2 * q + 1]));
+  c->c_step_ms += (double)ms;
+  c->c_ev_busy[q] = 0;
+  return KMC_OK;
 }
 
 int contact_check_err(kmc_ctx* c, const char* who) {
@@ -1726,6 +1751,14 @@ int kmc_contact_init(kmc_ctx* c, int32_t n_seg, int64_t c1_pop, int64_t c2_pop, 
   KMC_TRY(dalloc(c, c->c_live, 1));
   KMC_TRY(dalloc(c, c->c_next, 1));
   KMC_TRY(dalloc(c, c->c_err, 1));
+  KMC_TRY(dalloc(c, c->c_made, 1));
+  KMC_HIP(c, hipMemsetAsync(c->c_made.p, 0, sizeof(uint64_t), c->stream));
+  c->c_step_ms = 0.0;
+  if (c->c_ev.empty()) {
+    c->c_ev.assign(2 * CONTACT_EV_PAIRS, nullptr);
+    for (hipEvent_t& e : c->c_ev) KMC_HIP(c, hipEventCreate(&e));
+  }
+  c->c_ev_busy.assign(CONTACT_EV_PAIRS, 0);
   KMC_HIP(c, hipMemsetAsync(c->c_live.p, 0, sizeof(int64_t), c->stream));
   KMC_HIP(c, hipMemsetAsync(c->c_next.p, 0, sizeof(uint64_t), c->stream));
   KMC_HIP(c, hipMemsetAsync(c->c_err.p, 0, sizeof(int32_t), c->stream));
@@ -1763,6 +1796,20 @@ int kmc_contact_run(kmc_ctx* c, double dt, int64_t n_steps, uint64_t max_hops_pe
   // errors are this call's: a hop cap reached leaves every particle consistent (the one that
   // stopped continues in the next step), so a retry with a larger cap may go on
   KMC_HIP(c, hipMemsetAsync(c->c_err.p, 0, sizeof(int32_t), c->stream));
+  // each step's threads take the live slots in the order of their sites' rows (radix sort of
+  // row lines; dead slots key `lines` and sort last): draws are keyed by particle id and hop
+  // count, so the order changes no result
+  const int64_t cap = c->c_cap;
+  const uint32_t lines = (uint32_t)(c->n_units / ROW_UNITS);
+  int key_bits = 1;
+  while (key_bits < 32 && (lines >> key_bits) != 0) ++key_bits;
+  KMC_TRY(dalloc(c, c->c_skeys, 2 * (size_t)cap));
+  KMC_TRY(dalloc(c, c->c_sids, 2 * (size_t)cap));
+  size_t sort_need = 0;
+  KMC_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, sort_need, c->c_skeys.p,
+                                                c->c_skeys.p + cap, c->c_sids.p, c->c_sids.p + cap,
+                                                (int)cap, 0, key_bits, c->stream));
+  KMC_TRY(dalloc(c, c->c_sort_tmp, std::max<size_t>(sort_need, 1)));
   for (int64_t k = 0; k < n_steps; ++k) {
     // monte_carlo::step, then save_metrics (main.cpp:95-97): every particle to the step's end
     contact_args a = contact_args_of(c);
@@ -1770,7 +1817,20 @@ int kmc_contact_run(kmc_ctx* c, double dt, int64_t n_steps, uint64_t max_hops_pe
     a.max_hops = max_hops_per_step;
     a.pop = c->c_pop_out.p + k * S;
     a.cur = c->c_cur_out.p + k * (S - 1);
+    KMC_HIP(c, launch_contact_keys(cap, c->c_live.p, c->c_site[c->c_buf].p, c->d_roff.p, lines,
+                                   c->c_skeys.p, c->c_sids.p, c->stream));
+    size_t have = c->c_sort_tmp.n;
+    KMC_HIP(c, hipcub::DeviceRadixSort::SortPairs(c->c_sort_tmp.p, have, c->c_skeys.p,
+                                                  c->c_skeys.p + cap, c->c_sids.p,
+                                                  c->c_sids.p + cap, (int)cap, 0, key_bits,
+                                                  c->stream));
+    a.perm = c->c_sids.p + cap;
+    const int q = (int)(k % CONTACT_EV_PAIRS);
+    KMC_TRY(contact_read_pair(c, q));
+    KMC_HIP(c, hipEventRecord(c->c_ev[2 * q], c->stream));
     KMC_HIP(c, launch_contact_step(a, c->c_vel > 0.0, c->stream));
+    KMC_HIP(c, hipEventRecord(c->c_ev[2 * q + 1], c->stream));
+    c->c_ev_busy[q] = 1;
     c->c_time = a.t_end;  // _time += dt (monte_carlo.h:354)
     // repopulate_contacts (monte_carlo.h:443-491): contact 1, then contact n_seg; the kept
     // particles keep their order, the new ones follow them
@@ -1792,6 +1852,7 @@ int kmc_contact_run(kmc_ctx* c, double dt, int64_t n_steps, uint64_t max_hops_pe
                               hipMemcpyDeviceToDevice, c->stream));
   }
   KMC_HIP(c, hipStreamSynchronize(c->stream));
+  for (int q = 0; q < CONTACT_EV_PAIRS; ++q) KMC_TRY(contact_read_pair(c, q));
   KMC_TRY(contact_check_err(c, who));
   if (population)
     KMC_HIP(c, hipMemcpy(population, c->c_pop_out.p, c->c_pop_out.n * sizeof(int32_t),
@@ -1802,6 +1863,18 @@ int kmc_contact_run(kmc_ctx* c, double dt, int64_t n_steps, uint64_t max_hops_pe
   if (n_live)
     KMC_HIP(c, hipMemcpy(n_live, c->c_live_out.p, (size_t)n_steps * sizeof(int64_t),
                          hipMemcpyDeviceToHost));
+  return KMC_OK;
+}
+
+int kmc_contact_stats(kmc_ctx* c, uint64_t* hops, double* step_kernel_ms) {
+  if (!c) return fail(nullptr, KMC_ERR_ARG, "kmc_contact_stats: null context");
+  if (!c->c_on) return fail(c, KMC_ERR_STATE, "kmc_contact_stats: call kmc_contact_init first");
+  KMC_HIP(c, hipSetDevice(c->cfg.device));
+  if (hops) {
+    KMC_HIP(c, hipStreamSynchronize(c->stream));
+    KMC_HIP(c, hipMemcpy(hops, c->c_made.p, sizeof(uint64_t), hipMemcpyDeviceToHost));
+  }
+  if (step_kernel_ms) *step_kernel_ms = c->c_step_ms;
   return KMC_OK;
 }
 

@@ -1280,10 +1280,14 @@ template <bool FLY>
 __global__ __launch_bounds__(256) void k_contact_step(contact_args a) {
 #pragma clang fp contract(off)
   __shared__ int32_t s_pop[CONTACT_MAX_SEG], s_cur[CONTACT_MAX_SEG];
+  __shared__ unsigned long long s_made;
   for (int i = threadIdx.x; i < a.n_seg; i += blockDim.x) s_pop[i] = s_cur[i] = 0;
+  if (threadIdx.x == 0) s_made = 0ull;
   __syncthreads();
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e < *a.n_live) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < *a.n_live) {
+    // slots in the order of their sites' rows: neighbouring lanes read neighbouring rows
+    const int64_t e = a.perm ? (int64_t)a.perm[j] : j;
     const uint32_t id = a.s.id[e];
     double t = a.s.t_next[e];
     uint64_t h = a.s.hops[e];
@@ -1353,6 +1357,7 @@ __global__ __launch_bounds__(256) void k_contact_step(contact_args a) {
     a.s.t_next[e] = t;
     a.s.hops[e] = h;
     a.s.y[e] = y_new;
+    if (made) atomicAdd(&s_made, (unsigned long long)made);
     const int i_new = contact_section(y_new, a.ymin, a.dy, a.n_seg);
     atomicAdd(&s_pop[i_new], 1);
     // save_currents (monte_carlo.h:598-626): interface k at ymin + dy (k + 1); +1 for a crossing
@@ -1374,6 +1379,18 @@ __global__ __launch_bounds__(256) void k_contact_step(contact_args a) {
     if (s_pop[i]) atomicAdd(a.pop + i, s_pop[i]);
     if (i + 1 < a.n_seg && s_cur[i]) atomicAdd(a.cur + i, s_cur[i]);
   }
+  if (threadIdx.x == 0 && s_made) atomicAdd(a.hops_made, s_made);
+}
+
+__global__ __launch_bounds__(256) void k_contact_keys(int64_t capacity, const int64_t* n_live,
+                                                      const int32_t* __restrict__ site,
+                                                      const int64_t* __restrict__ roff,
+                                                      uint32_t dead, uint32_t* __restrict__ keys,
+                                                      uint32_t* __restrict__ ids) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= capacity) return;
+  keys[e] = e < *n_live ? (uint32_t)roff[site[e]] : dead;
+  ids[e] = (uint32_t)e;
 }
 
 // repopulate (monte_carlo.h:461-474): a particle whose y lies in [y_lo, y_hi] leaves
@@ -1672,6 +1689,15 @@ hipError_t launch_contact_step(const contact_args& a, bool fly, hipStream_t st) 
     hipLaunchKernelGGL(k_contact_step<true>, grid_for(a.capacity), dim3(256), 0, st, a);
   else
     hipLaunchKernelGGL(k_contact_step<false>, grid_for(a.capacity), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_contact_keys(int64_t capacity, const int64_t* n_live, const int32_t* site,
+                               const int64_t* roff, uint32_t dead, uint32_t* keys, uint32_t* ids,
+                               hipStream_t st) {
+  if (capacity == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_contact_keys, grid_for(capacity), dim3(256), 0, st, capacity, n_live, site,
+                     roff, dead, keys, ids);
   return hipGetLastError();
 }
 

@@ -115,6 +115,7 @@ _SIGS = {
     "kmc_contact_init": (C.c_int, [_P, C.c_int32, C.c_int64, C.c_int64, C.c_int64]),
     "kmc_contact_run": (C.c_int, [_P, C.c_double, C.c_int64, C.c_uint64, _I32, _I32, _I64]),
     "kmc_contact_sections": (C.c_int, [_P, _D, _D, _D, _D]),
+    "kmc_contact_stats": (C.c_int, [_P, C.POINTER(C.c_uint64), _D]),
     "kmc_contact_particles": (C.c_int, [_P, C.c_int64, _I64, C.POINTER(C.c_uint32), _I32, _D,
                                         C.POINTER(C.c_uint64), _D]),
     "kmc_autotune_schedule": (C.c_int, [_P, C.c_int64, C.c_int32, _I64, _I64, _D]),
@@ -426,6 +427,13 @@ class Context:
         self._ck(self.lib.kmc_contact_run(self.h, float(dt), int(n_steps), int(max_hops_per_step),
                                           _ptr(pop, _I32), _ptr(cur, _I32), _ptr(live, _I64)))
         return pop, cur, live
+
+    def contact_stats(self):
+        """(hops made by all particles, device ms of the step kernels) since contact_init."""
+        h = C.c_uint64()
+        ms = C.c_double()
+        self._ck(self.lib.kmc_contact_stats(self.h, C.byref(h), C.byref(ms)))
+        return int(h.value), float(ms.value)
 
     def contact_sections(self):
         """(area [n_seg], ymin, dy, time)."""

@@ -191,6 +191,12 @@ int kmc_contact_run(kmc_ctx* ctx, double dt, int64_t n_steps, uint64_t max_hops_
                     int32_t* population, int32_t* crossings, int64_t* n_live);
 /* area[n_seg] (get_area: x extent times z extent of the section's sites), ymin, dy and the
  * simulated time; any output may be NULL. */
+/* Since kmc_contact_init: the hops made by all particles (also those since removed at the
+ * contacts) and the device time of the step kernels (particle::step of every particle, HIP
+ * events around each step's launch), without the repopulation.  No counterpart in the
+ * reference, which times nothing; a throughput measurement of the contact mode.  Either
+ * output may be NULL. */
+int kmc_contact_stats(kmc_ctx* ctx, uint64_t* hops, double* step_kernel_ms);
 int kmc_contact_sections(const kmc_ctx* ctx, double* area, double* ymin, double* dy,
                          double* time);
 /* The live particles in slot order: id, site, time of the next hop, hops made and y (FP64).

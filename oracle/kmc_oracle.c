@@ -984,7 +984,7 @@ int orc_contact(int64_t n_sites, const double* x, const double* y, const double*
                 int64_t n_steps, double velocity, const int32_t* left, const int32_t* right,
                 const float* pos32, const double* per_len, int32_t* pop, int32_t* cur,
                 int64_t* n_live, double* area, int64_t* n_final, uint32_t* p_id, int32_t* p_site,
-                double* p_t, uint64_t* p_hops, double* p_y) {
+                double* p_t, uint64_t* p_hops, double* p_y, uint64_t* hops_made) {
   double lo[3], hi[3], ymin, ymax, dy, dp, T = 0.0;
   double *xmn, *xmx, *zmn, *zmx;
   int32_t *c1, *cn, *sec;
@@ -994,6 +994,7 @@ int orc_contact(int64_t n_sites, const double* x, const double* y, const double*
   int a, side, ret = 0;
   orc_run_args ra;
   uint8_t* no_flags;
+  uint64_t made_total = 0; /* hops made by all particles, removed ones included */
   if (n_sites <= 0 || n_seg < 2 || capacity <= 0) return -1;
   if (velocity > 0.0 && (!left || !right || !pos32 || !per_len)) return -1;
   /* the flight's view of the network: links, FP32 positions, periods; no drain sites */
@@ -1107,6 +1108,7 @@ int orc_contact(int64_t n_sites, const double* x, const double* y, const double*
         p->t_next = p->t_next + orc_dwell(cum + row_ptr[p->site],
                                           row_ptr[p->site + 1] - row_ptr[p->site], r);
         p->hops++;
+        made_total++;
       }
       if (velocity > 0.0) {
         double o[3];
@@ -1164,6 +1166,7 @@ int orc_contact(int64_t n_sites, const double* x, const double* y, const double*
     p_y[k] = P[k].y;
   }
   free(c1); free(cn); free(sec); free(P); free(Q); free(no_flags);
+  if (hops_made) *hops_made = made_total;
   return ret;
 }
 

@@ -80,7 +80,8 @@ def library() -> C.CDLL:
             "orc_contact": (C.c_int, [C.c_int64, _D, _D, _D, _I64, _I32, _D, C.c_uint64, C.c_int,
                                       C.c_int64, C.c_int64, C.c_int64, C.c_double, C.c_int64,
                                       C.c_double, _I32, _I32, _F, _D,
-                                      _I32, _I32, _I64, _D, _I64, _U32, _I32, _D, _U64, _D]),
+                                      _I32, _I32, _I64, _D, _I64, _U32, _I32, _D, _U64, _D,
+                                      _U64]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)
@@ -390,14 +391,15 @@ def contact(net: dict, csr: dict, seed: int, n_seg: int, c1_pop: int, c2_pop: in
     t = np.zeros(capacity)
     hops = np.zeros(capacity, np.uint64)
     yy = np.zeros(capacity)
+    made = np.zeros(1, np.uint64)
     rc = lib.orc_contact(pos[0].size, *[_p(v, _D) for v in pos], _p(row, _I64), _p(col, _I32),
                          _p(cum, _D), seed, n_seg, c1_pop, c2_pop, capacity, dt, n_steps, *fly,
                          _p(pop, _I32), _p(cur, _I32), _p(live, _I64), _p(area, _D),
                          _p(n_final, _I64), _p(pid, _U32), _p(site, _I32), _p(t, _D),
-                         _p(hops, _U64), _p(yy, _D))
+                         _p(hops, _U64), _p(yy, _D), _p(made, _U64))
     if rc != 0:
         raise RuntimeError(f"orc_contact failed ({rc})")
     n = int(n_final[0])
     return dict(population=pop, crossings=cur, n_live=live, area=area, id=pid[:n],
-                site=site[:n], t_next=t[:n], hops=hops[:n], y=yy[:n])
+                site=site[:n], t_next=t[:n], hops=hops[:n], y=yy[:n], hops_made=int(made[0]))
 

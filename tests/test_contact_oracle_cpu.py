@@ -42,6 +42,7 @@ def test_areas_profile_and_conservation(orc):
     assert np.array_equal(pop[1:].sum(1), live[:-1])
     assert np.array_equal(live, pop[:, 1:-1].sum(1) + c1 + c2)
     assert np.all(np.diff(np.sort(r["id"])) > 0)              # ids stay unique
+    assert r["hops_made"] >= int(r["hops"].sum()) > 0         # removed particles' hops too
     assert int(r["id"].max()) < sum(want) + 300 * (c1 + c2)
 
 

@@ -63,6 +63,9 @@ def test_contact_matches_oracle(kmc, orc, nw, n_seg, c1, c2, dt, steps):
     assert np.array_equal(p["id"], ref["id"]) and np.array_equal(p["site"], ref["site"])
     assert np.array_equal(p["hops"], ref["hops"])
     np.testing.assert_allclose(p["t_next"], ref["t_next"], rtol=1e-12)
+    # every hop made, those of particles since removed included, and the step kernels' time
+    made, step_ms = ctx.contact_stats()
+    assert made == ref["hops_made"] and step_ms > 0.0
     # bookkeeping: a step's population counts the particles that entered it
     assert pop[0].sum() == p0["id"].size
     assert np.array_equal(pop[1:].sum(1), live[:-1])
@@ -96,6 +99,7 @@ def test_contact_drift_matches_oracle(kmc, orc, nw, v):
     # the device's and libm's, equal to within an ulp or two
     np.testing.assert_allclose(p["y"], ref["y"], rtol=1e-12, atol=0)
     np.testing.assert_allclose(p["t_next"], ref["t_next"], rtol=1e-12)
+    assert ctx.contact_stats()[0] == ref["hops_made"]
     off = p["y"] - net["y"][p["site"]]
     assert np.mean(off != 0) > 0.5 and np.abs(off).max() <= 10.01 * NM   # between sites
     say(f"  live {live[-1]}, hops {int(p['hops'].sum())}, crossings {int(np.abs(cur).sum())}")
