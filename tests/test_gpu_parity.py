@@ -266,4 +266,25 @@ def test_large_film_csr_and_hops(kmc, orc, nw):
     ox.run(H)
     _compare_state(ctx.get_excitons(), ox, "large film")
     assert np.array_equal(ctx.get_hop_counts(), ox.site_count)
+    # the full exciton count (10^6): trajectories are keyed by exciton id, so the first 10^4
+    # of the full run are the subset's, bit for bit; the run split in two (with the exciton
+    # sorts between sub-launches) equals one call; drains + in flight account for every exciton
+    say("  10^6 excitons x 5000 hops: first 10^4 vs the oracle, split vs whole")
+    Ef = 1_000_000
+    ctx.set_hop_schedule(625)
+    ctx.inject(Ef)
+    ctx.run(H)
+    whole = ctx.get_excitons()
+    counts_whole = ctx.get_hop_counts()
+    sub = {k: v[:E] for k, v in whole.items()}
+    _compare_state(sub, ox, "large film, first 10^4 of 10^6")
+    ctx.inject(Ef)
+    ctx.run(H // 2)
+    ctx.run(H - H // 2)
+    split = ctx.get_excitons()
+    for k in ("site", "hops", "hash", "drains", "disp"):
+        assert np.array_equal(split[k], whole[k]), k
+    assert np.array_equal(split["t_next"], whole["t_next"])
+    # every hop counted once at its source site (the counts restart at each injection)
+    assert int(counts_whole.sum()) == int(whole["hops"].sum()) > 0.99 * Ef * H
     ctx.close()
