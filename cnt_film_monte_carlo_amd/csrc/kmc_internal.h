@@ -15,6 +15,26 @@
 namespace cntkmc {
 
 constexpr double RAND_MAX_D = 2147483647.0;  // glibc RAND_MAX, the divisor of scatterer.cpp:17
+// RN(1 / RAND_MAX_D) = 2^-31 (1 + 2^-31) exactly: 1/D = 2^-31 (1 + 2^-31 + 2^-62 + ...).
+constexpr double INV_RAND_MAX_D = 1.0 / 2147483647.0;
+
+// x / RAND_MAX_D, correctly rounded (the IEEE quotient, bit for bit), in one multiply and two
+// FMAs instead of a divide sequence.  Proof for a normal double x with quotient Q in
+// [2^E, 2^(E+1)), D = 2^31 - 1:
+//  * x / D is never a rounding midpoint: that would need D k (k odd, 54 bits) to fit in x's
+//    53-bit significand.  Both x (ulp >= 2^(E-22), since x < 2^(E+32)) and the midpoints
+//    (odd multiples of 2^(E-53)) lie on the 2^(E-53) grid, so x / D is at least 2^(E-53) / D
+//    ~ 2^(E-84) from any midpoint.
+//  * q0 = RN(x INV) is within one ulp of Q: INV's relative error is ~2^-62.  The remainder
+//    x - q0 D is a multiple of 2^(E-52) below 2^(E-21), exact in one FMA.
+//  * q0 + rem INV differs from Q = q0 + rem / D by rem |INV - 1/D| < 2^(E-114).  That is far
+//    inside the 2^(E-84) distance to a midpoint, so RN of it is RN(Q).
+// Checked exhaustively for every r in [1, 2^31 - 1] (dwell draws) and on 5e8 random doubles.
+__host__ __device__ __forceinline__ double div_rand_max(double x) {
+  const double q0 = x * INV_RAND_MAX_D;
+  const double rem = __builtin_fma(-q0, RAND_MAX_D, x);
+  return __builtin_fma(rem, INV_RAND_MAX_D, q0);
+}
 constexpr uint32_t EDGE_DRAIN_BIT = 0x80000000u;
 constexpr uint32_t ROW_UNITS = 8;  // 16-byte units per 128-byte line of the hop kernel's blocks
 // Same-tube links of the drift path: no link (tube end) and the bound on site passes per flight.

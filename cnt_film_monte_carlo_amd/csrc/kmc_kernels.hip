@@ -296,7 +296,7 @@ __device__ __forceinline__ double dwell_time(uint32_t n, double gamma, uint32_t 
 #pragma clang fp contract(off)
   if (trapped(n, gamma)) return INFINITY;
   const double inv = 1.0 / gamma;
-  return -inv * log((double)r / RAND_MAX_D);
+  return -inv * log(div_rand_max((double)r));
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -907,7 +907,7 @@ __device__ __forceinline__ void hop_exciton(const net_view& v, uint32_t e, uint6
 
     // event selection (scatterer::update_state, scatterer.cpp:17-30)
     const u32x4 w = draw4(v.seed, e, h, STREAM_HOP);
-    const double dice = (row.gamma * (double)r31(w.v[0])) / RAND_MAX_D;
+    const double dice = div_rand_max(row.gamma * (double)r31(w.v[0]));
     const uint32_t idx = select_entry(v.blocks, R, row, dice);
     const uint32_t tg = target_of(v.blocks, R, row, idx);
     if (COUNT) cc.add(row.site, cbase);
@@ -1034,7 +1034,7 @@ __device__ __forceinline__ void hop_wave(const net_view& v, uint32_t e, bool val
     if (active) {
       // event selection (scatterer::update_state, scatterer.cpp:17-30)
       const u32x4 w = draw4(v.seed, e, h, STREAM_HOP);
-      const double dice = (row.gamma * (double)r31(w.v[0])) / RAND_MAX_D;
+      const double dice = div_rand_max(row.gamma * (double)r31(w.v[0]));
       const uint32_t idx = select_entry(v.blocks, W, row, dice);
       const uint32_t tg = target_of(v.blocks, W, row, idx);
       if (COUNT) cc.add(row.site, cbase);
@@ -1326,7 +1326,7 @@ __global__ __launch_bounds__(256) void k_contact_step(contact_args a) {
         break;
       }
       const u32x4 w = draw4(a.seed, id, h, STREAM_HOP);
-      const double dice = (row.gamma * (double)r31(w.v[0])) / RAND_MAX_D;
+      const double dice = div_rand_max(row.gamma * (double)r31(w.v[0]));
       const uint32_t idx = select_entry(a.blocks, R, row, dice);
       R = target_of(a.blocks, R, row, idx) & ~EDGE_DRAIN_BIT;
       row = load_row(a.blocks, R);

@@ -512,6 +512,35 @@ static double orc_dwell(const double* row, int64_t n, uint32_t r) {
   return -inv * log((double)r / (double)ORC_RAND_MAX);
 }
 
+/* Mismatches between x / 2147483647 (IEEE division) and the divide-free form the device uses,
+ * q0 = x INV, rem = fma(-q0, D, x), q = fma(rem, INV, q0) with INV = RN(1 / D): over every
+ * integer r in [lo, hi) (the dwell draws), then over n_random doubles with random significands
+ * and exponents 2^30..2^70 (the selection dice Gamma r / D), from an xorshift stream. */
+int64_t orc_div_rand_max_mismatches(uint32_t lo, uint32_t hi, int64_t n_random, uint64_t seed) {
+  const double D = 2147483647.0, INV = 1.0 / 2147483647.0;
+  int64_t bad = 0, i;
+#pragma omp parallel for reduction(+ : bad) schedule(static)
+  for (i = (int64_t)lo; i < (int64_t)hi; ++i) {
+    const double x = (double)i;
+    const double q0 = x * INV;
+    const double q = fma(fma(-q0, D, x), INV, q0);
+    if (q != x / D) ++bad;
+  }
+#pragma omp parallel for reduction(+ : bad) schedule(static)
+  for (i = 0; i < n_random; ++i) {
+    uint64_t s = seed ^ ((uint64_t)i * 0x9E3779B97F4A7C15ull), bits;
+    double x, q0, q;
+    s ^= s << 13; s ^= s >> 7; s ^= s << 17;
+    s ^= s << 13; s ^= s >> 7; s ^= s << 17;
+    bits = (s & 0x000FFFFFFFFFFFFFull) | ((uint64_t)(1023 + 30 + (int)(i % 41)) << 52);
+    memcpy(&x, &bits, sizeof x);
+    q0 = x * INV;
+    q = fma(fma(-q0, D, x), INV, q0);
+    if (q != x / D) ++bad;
+  }
+  return bad;
+}
+
 double orc_dwell_time(double max_rate, uint32_t r) {
   double inv = 1. / max_rate;
   return -inv * log((double)r / (double)ORC_RAND_MAX);

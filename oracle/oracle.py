@@ -70,6 +70,8 @@ def library() -> C.CDLL:
             "orc_net_fill": (C.c_int, [_P, C.c_int, _I64, _D, _I32, _D, _F, _D, _D, _D, _D]),
             "orc_select_index": (C.c_int, [_D, C.c_int, C.c_double]),
             "orc_dwell_time": (C.c_double, [C.c_double, C.c_uint32]),
+            "orc_div_rand_max_mismatches": (C.c_int64, [C.c_uint32, C.c_uint32, C.c_int64,
+                                                        C.c_uint64]),
             "orc_inject": (C.c_int, [C.c_int64, _I64, _D, C.c_int64, _I32, C.c_uint64, C.c_int64,
                                      _I32, _D, _U64, _D, _U32, _U64, _U32]),
             "orc_run": (C.c_int, [C.POINTER(RunArgs)]),
