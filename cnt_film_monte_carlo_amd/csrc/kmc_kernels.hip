@@ -1273,6 +1273,28 @@ __device__ __forceinline__ int contact_section(double y, double ymin, double dy,
   return i < 0 ? 0 : (i < n_seg ? i : n_seg - 1);
 }
 
+// The population and crossing counts of one particle's step (save_population_profile,
+// save_currents: monte_carlo.h:565-571, 598-626) into the workgroup's LDS counters.  Interface k
+// lies at ymin + dy (k + 1); +1 for a crossing upward (old < y_k <= new), -1 downward.
+// Interfaces more than one section away from both positions cannot lie between them, so only
+// those near the two sections are compared.
+__device__ __forceinline__ void contact_tally(const contact_args& a, double y_old, double y_new,
+                                              int32_t* s_pop, int32_t* s_cur) {
+#pragma clang fp contract(off)
+  const int i_new = contact_section(y_new, a.ymin, a.dy, a.n_seg);
+  atomicAdd(&s_pop[i_new], 1);
+  if (y_old != y_new) {
+    const int i_old = contact_section(y_old, a.ymin, a.dy, a.n_seg);
+    const int k_lo = max(0, min(i_old, i_new) - 1);
+    const int k_hi = min(a.n_seg - 2, max(i_old, i_new) + 1);
+    for (int k = k_lo; k <= k_hi; ++k) {
+      const double yk = a.ymin + a.dy * (double)(k + 1);
+      if (y_old < yk && y_new >= yk) atomicAdd(&s_cur[k], 1);
+      else if (y_old >= yk && y_new < yk) atomicAdd(&s_cur[k], -1);
+    }
+  }
+}
+
 // particle::step (src/monte_carlo/particle.cpp:57-80) over one step.  FLY (v > 0): the flight
 // up to each due hop, the hop from the site reached, then the flight to the step's end, which
 // is taken (particle::fly(dt), particle.cpp:77) and leaves the particle between sites.
@@ -1358,21 +1380,7 @@ __global__ __launch_bounds__(256) void k_contact_step(contact_args a) {
     a.s.hops[e] = h;
     a.s.y[e] = y_new;
     if (made) atomicAdd(&s_made, (unsigned long long)made);
-    const int i_new = contact_section(y_new, a.ymin, a.dy, a.n_seg);
-    atomicAdd(&s_pop[i_new], 1);
-    // save_currents (monte_carlo.h:598-626): interface k at ymin + dy (k + 1); +1 for a crossing
-    // upward (old < y_k <= new), -1 downward.  Interfaces more than one section away from both
-    // positions cannot lie between them, so only those near the two sections are compared.
-    if (y_old != y_new) {
-      const int i_old = contact_section(y_old, a.ymin, a.dy, a.n_seg);
-      const int k_lo = max(0, min(i_old, i_new) - 1);
-      const int k_hi = min(a.n_seg - 2, max(i_old, i_new) + 1);
-      for (int k = k_lo; k <= k_hi; ++k) {
-        const double yk = a.ymin + a.dy * (double)(k + 1);
-        if (y_old < yk && y_new >= yk) atomicAdd(&s_cur[k], 1);
-        else if (y_old >= yk && y_new < yk) atomicAdd(&s_cur[k], -1);
-      }
-    }
+    contact_tally(a, y_old, y_new, s_pop, s_cur);
   }
   __syncthreads();
   for (int i = threadIdx.x; i < a.n_seg; i += blockDim.x) {
@@ -1391,6 +1399,95 @@ __global__ __launch_bounds__(256) void k_contact_keys(int64_t capacity, const in
   if (e >= capacity) return;
   keys[e] = e < *n_live ? (uint32_t)roff[site[e]] : dead;
   ids[e] = (uint32_t)e;
+}
+
+// particle::step of every live particle for v = 0, built like the Kubo kernel's wave loop around
+// coop_load_row: every lane of a wave takes part in every row load, and the wave loops until no
+// lane has a hop due by the step's end (a lane whose particle has stopped keeps loading its
+// row).  Each particle's arithmetic and its order are those of k_contact_step's hop loop.  The live
+// slots, sorted by row, are dealt to the XCDs in eighths (from the device-side live count), so
+// each XCD's L2 serves one compact region of the film.  The grid is a multiple of 8 workgroups
+// covering the capacity.
+__global__ __launch_bounds__(256, 4) void k_contact_wave(contact_args a) {
+#pragma clang fp contract(off)
+  __shared__ uint4 rows[4 * 64 * ROW_UNITS];  // 8 KB of staged rows per wave
+  __shared__ unsigned long long s_made;
+  // the section counters are sized by the launch (2 n_seg words), so that four workgroups fit
+  // a CU's LDS beside the staged rows
+  extern __shared__ int32_t s_counts[];
+  int32_t* s_pop = s_counts;
+  int32_t* s_cur = s_counts + a.n_seg;
+  for (int i = threadIdx.x; i < a.n_seg; i += blockDim.x) s_pop[i] = s_cur[i] = 0;
+  if (threadIdx.x == 0) s_made = 0ull;
+  __syncthreads();
+  const int64_t n_live = *a.n_live;
+  const uint32_t groups = (uint32_t)((n_live + 255) / 256);
+  const uint32_t per_xcd = (groups + KMC_COUNT_COPIES - 1) / KMC_COUNT_COPIES;
+  const uint32_t b = blockIdx.x;
+  const int64_t j = (b / KMC_COUNT_COPIES < per_xcd)
+                        ? ((int64_t)((b % KMC_COUNT_COPIES) * per_xcd + b / KMC_COUNT_COPIES) * 256 +
+                           threadIdx.x)
+                        : INT64_MAX;
+  const bool valid = j < n_live;
+  const uint32_t lane = threadIdx.x & 63u;
+  if (__ballot(valid) != 0) {
+    uint4* lds_wave = rows + 64u * ROW_UNITS * __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t e = valid ? (a.perm ? (int64_t)a.perm[j] : j) : 0;
+    uint32_t id = 0, W = 0, r_dw = 0;
+    double t = INFINITY, y_old = 0.0;
+    uint64_t h = 0, made = 0;
+    if (valid) {
+      id = a.s.id[e];
+      t = a.s.t_next[e];
+      h = a.s.hops[e];
+      y_old = a.s.y[e];
+      const uint32_t L0 = (uint32_t)a.roff[a.s.site[e]];
+      W = row_word(L0, a.blocks[(size_t)ROW_UNITS * L0].x);
+    }
+    bool active = valid, pend = false;
+    row_regs row;
+    while (true) {
+      row = coop_load_row(a.blocks, W, lds_wave, lane);
+      if (pend) {
+        t = t + dwell_time(row.n, row.gamma, r_dw);  // dwell at the new site
+        pend = false;
+      }
+      if (active) {
+        if (!(t <= a.t_end) || trapped(row.n, row.gamma)) {
+          active = false;
+        } else if (made == a.max_hops) {
+          atomicOr(a.err, CONTACT_ERR_HOPS);
+          active = false;
+        }
+      }
+      if (__ballot(active) == 0) break;
+      if (active) {
+        const u32x4 w = draw4(a.seed, id, h, STREAM_HOP);
+        const double dice = div_rand_max(row.gamma * (double)r31(w.v[0]));
+        const uint32_t idx = select_entry(a.blocks, W, row, dice);
+        W = target_of(a.blocks, W, row, idx) & ~EDGE_DRAIN_BIT;
+        r_dw = nonzero_draw(w.v[1], w.v[3], a.seed, id, h, STREAM_HOP_REDRAW);
+        pend = true;
+        ++h;
+        ++made;
+      }
+    }
+    if (valid) {
+      const double y_new = a.py[row.site];
+      a.s.site[e] = (int32_t)row.site;
+      a.s.t_next[e] = t;
+      a.s.hops[e] = h;
+      a.s.y[e] = y_new;
+      if (made) atomicAdd(&s_made, (unsigned long long)made);
+      contact_tally(a, y_old, y_new, s_pop, s_cur);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < a.n_seg; i += blockDim.x) {
+    if (s_pop[i]) atomicAdd(a.pop + i, s_pop[i]);
+    if (i + 1 < a.n_seg && s_cur[i]) atomicAdd(a.cur + i, s_cur[i]);
+  }
+  if (threadIdx.x == 0 && s_made) atomicAdd(a.hops_made, s_made);
 }
 
 // repopulate (monte_carlo.h:461-474): a particle whose y lies in [y_lo, y_hi] leaves
@@ -1688,7 +1785,10 @@ hipError_t launch_contact_step(const contact_args& a, bool fly, hipStream_t st) 
   if (fly)
     hipLaunchKernelGGL(k_contact_step<true>, grid_for(a.capacity), dim3(256), 0, st, a);
   else
-    hipLaunchKernelGGL(k_contact_step<false>, grid_for(a.capacity), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_contact_wave,
+                       dim3((unsigned)((a.capacity + 256 * KMC_COUNT_COPIES - 1) /
+                                       (256 * KMC_COUNT_COPIES) * KMC_COUNT_COPIES)),
+                       dim3(256), (size_t)2 * a.n_seg * sizeof(int32_t), st, a);
   return hipGetLastError();
 }
 
