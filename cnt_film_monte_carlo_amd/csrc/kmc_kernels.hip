@@ -910,7 +910,7 @@ __device__ __forceinline__ void hop_exciton(const net_view& v, uint32_t e, uint6
     const double dice = div_rand_max(row.gamma * (double)r31(w.v[0]));
     const uint32_t idx = select_entry(v.blocks, R, row, dice);
     const uint32_t tg = target_of(v.blocks, R, row, idx);
-    if (COUNT) cc.add(row.site, cbase);
+    if (COUNT) cc.add(count_slot(row.site, v.count_stride), cbase);
     const float sx = row.x, sy = row.y, sz = row.z;
     const double t_hop = t;
 
@@ -1037,7 +1037,7 @@ __device__ __forceinline__ void hop_wave(const net_view& v, uint32_t e, bool val
       const double dice = div_rand_max(row.gamma * (double)r31(w.v[0]));
       const uint32_t idx = select_entry(v.blocks, W, row, dice);
       const uint32_t tg = target_of(v.blocks, W, row, idx);
-      if (COUNT) cc.add(row.site, cbase);
+      if (COUNT) cc.add(count_slot(row.site, v.count_stride), cbase);
       const uint32_t r = nonzero_draw(w.v[1], w.v[3], v.seed, e, h, STREAM_HOP_REDRAW);
       t_hop = t;
       if (tg & EDGE_DRAIN_BIT) {
@@ -1206,8 +1206,9 @@ __global__ __launch_bounds__(256) void k_sum_counts(int64_t n, int64_t stride,
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   uint64_t s = 0;
+  const uint32_t slot = count_slot((uint32_t)i, stride);
 #pragma unroll
-  for (int x = 0; x < KMC_COUNT_COPIES; ++x) s += copies[(size_t)x * stride + i];
+  for (int x = 0; x < KMC_COUNT_COPIES; ++x) s += copies[(size_t)x * stride + slot];
   out[i] = s;
 }
 
