@@ -83,8 +83,8 @@ def cpu_baseline(net, w, tables, csr, seconds=12.0):
         "unit": "hops/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"{E} excitons x {H} hops on the same 624,739-site network, neighbours and "
-                   f"rates recomputed per hop (scatterer.cpp:12), OpenMP over excitons"),
+        "sample": (f"{E} excitons x {H} hops on the same {net['n']:,}-site network, neighbours "
+                   f"and rates recomputed per hop (scatterer.cpp:12), OpenMP over excitons"),
         "cached_csr_value": cached,
         "cached_csr_sample": f"{4 * E} excitons x {Hc} hops over the GPU-built CSR",
     }
@@ -329,7 +329,7 @@ def contact_leg(w, net, tabs, n_seg=20, c1_pop=100_000, dwell_mult=100.0, warmup
         "particles_mean": particle_steps / steps, "hops": int(hops), "seconds": wall,
         "hops_per_s": hops / wall, "step_kernel_ms": kern_s * 1e3,
         "step_kernel_hops_per_s": hops / kern_s,
-        "roofline": {"kernel": "k_contact_step", "bound": "hbm", "achieved": achieved,
+        "roofline": {"kernel": "k_contact_wave", "bound": "hbm", "achieved": achieved,
                      "peak": 8000.0, "unit": "GB/s", "frac": achieved / 8000.0,
                      "bytes_per_hop": bpe, "bytes_per_particle_step": 60.0},
         "note": "whole steps (particle steps + repopulation + per-step outputs) timed by the "
