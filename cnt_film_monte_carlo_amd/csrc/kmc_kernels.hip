@@ -1296,11 +1296,11 @@ __device__ __forceinline__ void contact_tally(const contact_args& a, double y_ol
   }
 }
 
-// particle::step (src/monte_carlo/particle.cpp:57-80) over one step.  FLY (v > 0): the flight
-// up to each due hop, the hop from the site reached, then the flight to the step's end, which
-// is taken (particle::fly(dt), particle.cpp:77) and leaves the particle between sites.
-template <bool FLY>
-__global__ __launch_bounds__(256) void k_contact_step(contact_args a) {
+// particle::step (src/monte_carlo/particle.cpp:57-80) over one step with drift (v > 0), one
+// thread per particle: the flight up to each due hop, the hop from the site reached, then the
+// flight to the step's end, which is taken (particle::fly(dt), particle.cpp:77) and leaves the
+// particle between sites.
+__global__ __launch_bounds__(256) void k_contact_fly(contact_args a) {
 #pragma clang fp contract(off)
   __shared__ int32_t s_pop[CONTACT_MAX_SEG], s_cur[CONTACT_MAX_SEG];
   __shared__ unsigned long long s_made;
@@ -1318,30 +1318,24 @@ __global__ __launch_bounds__(256) void k_contact_step(contact_args a) {
     uint32_t R = row_word(L0, a.blocks[(size_t)ROW_UNITS * L0].x);
     row_regs row = load_row(a.blocks, R);
     const double y_old = a.s.y[e];
-    double tl = FLY ? a.s.t_last[e] : 0.0;
-    flight_state g{};
-    if (FLY)
-      g = flight_state{R, row.site, row.x, row.y, row.z, a.s.s_off[e], (uint32_t)a.s.heading[e],
-                       0.0, 0.0, 0.0, 0u, 0ull};
+    double tl = a.s.t_last[e];
+    flight_state g{R, row.site, row.x, row.y, row.z, a.s.s_off[e], (uint32_t)a.s.heading[e],
+                   0.0, 0.0, 0.0, 0u, 0ull};
     uint64_t made = 0;
     while (t <= a.t_end) {
-      if (FLY) {
-        if (!fly(a.fv, g, t - tl, id, h)) {
-          atomicOr(a.err, CONTACT_ERR_FLY);
-          break;
-        }
-        if (g.R != R) {
-          R = g.R;
-          row = load_row(a.blocks, R);
-        }
-        if (trapped(row.n, row.gamma)) {
-          // no rates: the particle stays on its tube and flies on (scatterer.cpp:14-15)
-          tl = t;
-          t = INFINITY;
-          ++h;
-          break;
-        }
-      } else if (trapped(row.n, row.gamma)) {
+      if (!fly(a.fv, g, t - tl, id, h)) {
+        atomicOr(a.err, CONTACT_ERR_FLY);
+        break;
+      }
+      if (g.R != R) {
+        R = g.R;
+        row = load_row(a.blocks, R);
+      }
+      if (trapped(row.n, row.gamma)) {
+        // no rates: the particle stays on its tube and flies on (scatterer.cpp:14-15)
+        tl = t;
+        t = INFINITY;
+        ++h;
         break;
       }
       if (made == a.max_hops) {
@@ -1354,29 +1348,21 @@ __global__ __launch_bounds__(256) void k_contact_step(contact_args a) {
       R = target_of(a.blocks, R, row, idx) & ~EDGE_DRAIN_BIT;
       row = load_row(a.blocks, R);
       const uint32_t r = nonzero_draw(w.v[1], w.v[3], a.seed, id, h, STREAM_HOP_REDRAW);
-      if (FLY) {
-        g = flight_state{R, row.site, row.x, row.y, row.z, 0.0, g.heading, 0.0, 0.0, 0.0, 0u, 0ull};
-        tl = t;
-      }
+      g = flight_state{R, row.site, row.x, row.y, row.z, 0.0, g.heading, 0.0, 0.0, 0.0, 0u, 0ull};
+      tl = t;
       t = t + dwell_time(row.n, row.gamma, r);
       ++h;
       ++made;
     }
-    double y_new;
-    if (FLY) {
-      if (!fly(a.fv, g, a.t_end - tl, id, h)) atomicOr(a.err, CONTACT_ERR_FLY);
-      tl = a.t_end;
-      double ox, oy, oz;
-      fly_offset(a.fv, g, ox, oy, oz);
-      y_new = a.py[g.site] + oy;
-      a.s.site[e] = (int32_t)g.site;
-      a.s.heading[e] = (uint8_t)g.heading;
-      a.s.s_off[e] = g.s;
-      a.s.t_last[e] = tl;
-    } else {
-      y_new = a.py[row.site];
-      a.s.site[e] = (int32_t)row.site;
-    }
+    if (!fly(a.fv, g, a.t_end - tl, id, h)) atomicOr(a.err, CONTACT_ERR_FLY);
+    tl = a.t_end;
+    double ox, oy, oz;
+    fly_offset(a.fv, g, ox, oy, oz);
+    const double y_new = a.py[g.site] + oy;
+    a.s.site[e] = (int32_t)g.site;
+    a.s.heading[e] = (uint8_t)g.heading;
+    a.s.s_off[e] = g.s;
+    a.s.t_last[e] = tl;
     a.s.t_next[e] = t;
     a.s.hops[e] = h;
     a.s.y[e] = y_new;
@@ -1405,7 +1391,7 @@ __global__ __launch_bounds__(256) void k_contact_keys(int64_t capacity, const in
 // particle::step of every live particle for v = 0, built like the Kubo kernel's wave loop around
 // coop_load_row: every lane of a wave takes part in every row load, and the wave loops until no
 // lane has a hop due by the step's end (a lane whose particle has stopped keeps loading its
-// row).  Each particle's arithmetic and its order are those of k_contact_step's hop loop.  The live
+// row).  Each particle's arithmetic and its order are those of the oracle's step.  The live
 // slots, sorted by row, are dealt to the XCDs in eighths (from the device-side live count), so
 // each XCD's L2 serves one compact region of the film.  The grid is a multiple of 8 workgroups
 // covering the capacity.
@@ -1784,7 +1770,7 @@ hipError_t launch_contact_step(const contact_args& a, bool fly, hipStream_t st) 
   if (a.capacity == 0) return hipSuccess;
   if (a.n_seg < 1 || a.n_seg > CONTACT_MAX_SEG) return hipErrorInvalidValue;
   if (fly)
-    hipLaunchKernelGGL(k_contact_step<true>, grid_for(a.capacity), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_contact_fly, grid_for(a.capacity), dim3(256), 0, st, a);
   else
     hipLaunchKernelGGL(k_contact_wave,
                        dim3((unsigned)((a.capacity + 256 * KMC_COUNT_COPIES - 1) /
