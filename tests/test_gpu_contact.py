@@ -176,3 +176,47 @@ def test_fick_known_diffusion(kmc):
     assert abs(D / (k * a * a) - 1) < 0.08
     assert abs(D_ends / (k * a * a) - 1) < 0.25
     ctx.close()
+
+
+def test_contact_full_size_split_equals_whole(kmc, nw):
+    """The 625k-site film at the bench's contact workload (20 sections, 100,000 particles held in
+    contact 1, ~1 M live): a run split in two calls equals one call bit for bit (populations,
+    crossings, live counts, every particle), the population of each step counts the particles
+    that entered it, and every hop is counted."""
+    say("contact mode, 625k-site film: 6 steps whole vs 3 + 3")
+    w = nw.film_large()
+    net = nw.build_network(w)
+    net["flags"] = np.zeros(net["n"], np.uint8)
+    tab = nw.forster_tables(3)
+
+    def make():
+        ctx = kmc.Context(kmc.Config(cutoff=w.cutoff, min_distance=0.4 * NM, seed=77))
+        ctx.set_sites(net["x"], net["y"], net["z"], net["ox"], net["oy"], net["oz"], net["left"],
+                      net["right"], net["chirality"], net["flags"])
+        ctx.set_rate_table(*tab)
+        ctx.build_cells()
+        ctx.build_rates(kmc.RATE_TABLE)
+        ctx.contact_init(20, 100_000, 0, 4_000_000)
+        return ctx
+
+    g = make().get_csr()
+    gam = g["cum"][g["row_ptr"][1:][np.diff(g["row_ptr"]) > 0] - 1]
+    dt = 20.0 * float(np.mean(1.0 / gam))
+    a = make()
+    pa, ca, la = a.contact_run(dt, 6)
+    b = make()
+    p1, c1, l1 = b.contact_run(dt, 3)
+    p2, c2, l2 = b.contact_run(dt, 3)
+    assert np.array_equal(pa, np.vstack([p1, p2]))
+    assert np.array_equal(ca, np.vstack([c1, c2]))
+    assert np.array_equal(la, np.concatenate([l1, l2]))
+    qa, qb = a.contact_particles(), b.contact_particles()
+    for k in ("id", "site", "hops", "t_next", "y"):
+        assert np.array_equal(qa[k], qb[k]), k
+    assert np.array_equal(pa[1:].sum(1), la[:-1])
+    assert np.array_equal(la, pa[:, 1:-1].sum(1) + 100_000)
+    ha, hb = a.contact_stats()[0], b.contact_stats()[0]
+    assert ha == hb and ha >= int(qa["hops"].sum()) > 0
+    say(f"  live {la[-1]}, hops {ha}")
+    a.close()
+    b.close()
