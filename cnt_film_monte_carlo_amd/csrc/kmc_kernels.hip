@@ -12,6 +12,8 @@
 // it bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "kmc_internal.h"
 #include "kmc_kernels.h"
 #include "philox.h"
@@ -967,10 +969,15 @@ __device__ __forceinline__ void hop_exciton(const net_view& v, uint32_t e, uint6
 // target row is in registers (displacement, visit hash, dwell at the new site), which keeps every
 // exciton's arithmetic and its order identical to hop_exciton.  A drain target's header and
 // position are gathered by the lanes concerned, and their next row load is the source site's.
-template <bool COUNT, bool SAMPLE, bool TRACE>
+// WINDOW: departures from the block lines [win_base, win_base + COUNT_WINDOW) are counted in the
+// workgroup's LDS window `win` (flushed by the kernel), the rest through the register LRU.
+constexpr uint32_t COUNT_WINDOW = 2048;
+
+template <bool COUNT, bool SAMPLE, bool TRACE, bool WINDOW = false>
 __device__ __forceinline__ void hop_wave(const net_view& v, uint32_t e, bool valid,
                                          uint64_t hop_budget, double t_max, const trace_view& tv,
-                                         uint4* lds_wave, uint32_t lane, uint4* lds_victim) {
+                                         uint4* lds_wave, uint32_t lane, uint4* lds_victim,
+                                         uint32_t* win = nullptr, uint32_t win_base = 0u) {
 #pragma clang fp contract(off)
   double t = INFINITY, dx = 0.0, dy = 0.0, dz = 0.0;
   uint64_t h = 0, hs = 0;
@@ -996,8 +1003,8 @@ __device__ __forceinline__ void hop_wave(const net_view& v, uint32_t e, bool val
   uint32_t r_dw = 0;           // its dwell draw
   float px = 0.f, py = 0.f, pz = 0.f;  // position of the site it left
   double t_hop = 0.0;
-  count_cache_lds cc;
-  if (COUNT) cc.init(lds_victim);
+  typename std::conditional<WINDOW, count_cache, count_cache_lds>::type cc;
+  if constexpr (COUNT && !WINDOW) cc.init(lds_victim);
   uint32_t* cbase = COUNT ? v.site_count + (size_t)xcd_id() * v.count_stride : nullptr;
   row_regs row;
 
@@ -1037,7 +1044,11 @@ __device__ __forceinline__ void hop_wave(const net_view& v, uint32_t e, bool val
       const double dice = div_rand_max(row.gamma * (double)r31(w.v[0]));
       const uint32_t idx = select_entry(v.blocks, W, row, dice);
       const uint32_t tg = target_of(v.blocks, W, row, idx);
-      if (COUNT) cc.add(count_slot(row.site, v.count_stride), cbase);
+      if (COUNT) {
+        const uint32_t off = (W & TGT_LINE_MASK) - win_base;
+        if (WINDOW && off < COUNT_WINDOW) atomicAdd(win + off, 1u);
+        else cc.add(count_slot(row.site, v.count_stride), cbase);
+      }
       const uint32_t r = nonzero_draw(w.v[1], w.v[3], v.seed, e, h, STREAM_HOP_REDRAW);
       t_hop = t;
       if (tg & EDGE_DRAIN_BIT) {
@@ -1098,15 +1109,43 @@ __device__ __forceinline__ uint32_t xcd_position() {
 template <bool COUNT, bool SAMPLE, bool TRACE>
 __global__ __launch_bounds__(256, 4) void k_hop(hop_args a) {
   __shared__ uint4 rows[4 * 64 * ROW_UNITS];  // 8 KB of staged rows per wave
-  __shared__ uint4 victims[2 * 256];           // 32 bytes of count victims per lane
+  __shared__ uint32_t win[COUNT_WINDOW];       // 8 KB: the workgroup's departure-count window
   const uint32_t pos = xcd_position();
   const uint32_t lane = threadIdx.x & 63u;
-  if (__builtin_amdgcn_readfirstlane(pos - lane) >= a.v.x.n) return;  // the whole wave is past the end
+  const bool wave_live = __builtin_amdgcn_readfirstlane(pos - lane) < a.v.x.n;
   const bool valid = pos < a.v.x.n;
   const uint32_t e = valid ? (a.perm ? a.perm[pos] : pos) : 0u;
   uint4* lds_wave = rows + 64u * ROW_UNITS * __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  hop_wave<COUNT, SAMPLE, TRACE>(a.v, e, valid, a.hop_budget, a.t_max, a.tv, lds_wave, lane,
-                                 victims + 2u * threadIdx.x);
+  if constexpr (!COUNT) {
+    if (wave_live)
+      hop_wave<COUNT, SAMPLE, TRACE>(a.v, e, valid, a.hop_budget, a.t_max, a.tv, lds_wave, lane,
+                                     nullptr);
+  } else {
+    // The workgroup's excitons are consecutive in the order sorted by block line, and their
+    // hops in one sub-launch stay mostly among nearby blocks: departures from the COUNT_WINDOW
+    // lines around the first one's line are summed in LDS and flushed once, per line, at the end.
+    for (uint32_t i = threadIdx.x; i < COUNT_WINDOW; i += blockDim.x) win[i] = 0u;
+    const uint32_t p0 = pos - threadIdx.x;
+    uint32_t base = 0u;
+    if (p0 < a.v.x.n) {
+      const uint32_t e0 = a.perm ? a.perm[p0] : p0;
+      const uint32_t L0 = (uint32_t)a.v.roff[a.v.x.site[e0]];
+      base = L0 > COUNT_WINDOW / 2 ? L0 - COUNT_WINDOW / 2 : 0u;
+    }
+    __syncthreads();
+    if (wave_live)
+      hop_wave<COUNT, SAMPLE, TRACE, true>(a.v, e, valid, a.hop_budget, a.t_max, a.tv, lds_wave,
+                                           lane, nullptr, win, base);
+    __syncthreads();
+    uint32_t* cbase = a.v.site_count + (size_t)xcd_id() * a.v.count_stride;
+    for (uint32_t i = threadIdx.x; i < COUNT_WINDOW; i += blockDim.x) {
+      const uint32_t c = win[i];
+      if (c) {
+        const uint32_t site = a.v.blocks[(size_t)ROW_UNITS * (base + i)].y;
+        atomicAdd(cbase + count_slot(site, a.v.count_stride), c);
+      }
+    }
+  }
 }
 
 // The same with drift along the tubes (v > 0).
