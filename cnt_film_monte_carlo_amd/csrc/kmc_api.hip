@@ -1442,7 +1442,7 @@ int kmc_ensemble_create(int32_t n_networks, kmc_ctx* const* networks, kmc_ensemb
       return efail(KMC_ERR_STATE, "kmc_ensemble_create: every member needs its CSR and excitons");
     pbase[k] = np;
     ebase[k] = ne;
-    np += (c->n_exc + 63) / 64 * 64;
+    np += (c->n_exc + 255) / 256 * 256;  // network ranges start on workgroup boundaries
     ne += c->n_exc;
   }
   if (ne >= ((int64_t)1 << 31) || np >= ((int64_t)1 << 32))
@@ -1450,7 +1450,7 @@ int kmc_ensemble_create(int32_t n_networks, kmc_ctx* const* networks, kmc_ensemb
   std::vector<uint16_t> wave_net(np / 64), exc_net(ne);
   for (int k = 0; k < n_networks; ++k) {
     const int64_t E = networks[k]->n_exc;
-    for (int64_t w = pbase[k] / 64; w < (pbase[k] + (E + 63) / 64 * 64) / 64; ++w) wave_net[w] = (uint16_t)k;
+    for (int64_t w = pbase[k] / 64; w < (pbase[k] + (E + 255) / 256 * 256) / 64; ++w) wave_net[w] = (uint16_t)k;
     for (int64_t e = 0; e < E; ++e) exc_net[ebase[k] + e] = (uint16_t)k;
   }
   kmc_ensemble* m = new kmc_ensemble();

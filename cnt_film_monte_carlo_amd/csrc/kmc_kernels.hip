@@ -1158,26 +1158,55 @@ __global__ __launch_bounds__(256) void k_hop_fly(hop_args a) {
 }
 
 // Several networks in one launch.  Network k's sorted excitons occupy positions
-// [pbase[k], pbase[k] + n_k) of a position space whose network ranges start on wave boundaries, so
-// every wave belongs to one network and reads that network's view with scalar loads; sorted
-// entries of network k sit at [ebase[k], ebase[k] + n_k) of perm, which holds ensemble-wide
-// exciton numbers ebase[k] + local id.
+// [pbase[k], pbase[k] + n_k) of a position space whose network ranges start on workgroup
+// boundaries, so every workgroup belongs to one network and reads that network's view with
+// scalar loads; sorted entries of network k sit at [ebase[k], ebase[k] + n_k) of perm, which
+// holds ensemble-wide exciton numbers ebase[k] + local id.  Departures are counted as in k_hop:
+// an LDS window over the block lines around the workgroup's first exciton, flushed into that
+// network's counts.
 template <bool COUNT, bool SAMPLE>
 __global__ __launch_bounds__(256, 4) void k_hop_ens(ens_args a) {
-  const uint32_t pos = xcd_position();
-  if (pos >= a.n_positions) return;
   __shared__ uint4 rows[4 * 64 * ROW_UNITS];  // 8 KB of staged rows per wave
-  __shared__ uint4 victims[2 * 256];           // 32 bytes of count victims per lane
+  __shared__ uint32_t win[COUNT_WINDOW];       // 8 KB: the workgroup's departure-count window
+  const uint32_t pos = xcd_position();
+  const uint32_t p0 = pos - threadIdx.x;       // the workgroup's first position
+  if (p0 >= a.n_positions) return;             // the whole workgroup is past the end
   const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t net = __builtin_amdgcn_readfirstlane(a.wave_net[pos >> 6]);
+  const uint32_t net = __builtin_amdgcn_readfirstlane(a.wave_net[p0 >> 6]);
   const net_view v = a.nets[net];
   const int64_t q = (int64_t)pos - a.pbase[net];
   const bool valid = q < v.x.n;
+  const bool wave_live = __builtin_amdgcn_readfirstlane((uint32_t)(q - lane < v.x.n));
   const int64_t eb = a.ebase[net];
   const uint32_t e = valid ? (uint32_t)((int64_t)a.perm[eb + q] - eb) : 0u;
   uint4* lds_wave = rows + 64u * ROW_UNITS * __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  hop_wave<COUNT, SAMPLE, false>(v, e, valid, a.hop_budget, a.t_max, trace_view{}, lds_wave, lane,
-                                 victims + 2u * threadIdx.x);
+  if constexpr (!COUNT) {
+    if (wave_live)
+      hop_wave<COUNT, SAMPLE, false>(v, e, valid, a.hop_budget, a.t_max, trace_view{}, lds_wave,
+                                     lane, nullptr);
+  } else {
+    for (uint32_t i = threadIdx.x; i < COUNT_WINDOW; i += blockDim.x) win[i] = 0u;
+    const int64_t q0 = (int64_t)p0 - a.pbase[net];
+    uint32_t base = 0u;
+    if (q0 < v.x.n) {
+      const uint32_t e0 = (uint32_t)((int64_t)a.perm[eb + q0] - eb);
+      const uint32_t L0 = (uint32_t)v.roff[v.x.site[e0]];
+      base = L0 > COUNT_WINDOW / 2 ? L0 - COUNT_WINDOW / 2 : 0u;
+    }
+    __syncthreads();
+    if (wave_live)
+      hop_wave<COUNT, SAMPLE, false, true>(v, e, valid, a.hop_budget, a.t_max, trace_view{},
+                                           lds_wave, lane, nullptr, win, base);
+    __syncthreads();
+    uint32_t* cbase = v.site_count + (size_t)xcd_id() * v.count_stride;
+    for (uint32_t i = threadIdx.x; i < COUNT_WINDOW; i += blockDim.x) {
+      const uint32_t c = win[i];
+      if (c) {
+        const uint32_t site = v.blocks[(size_t)ROW_UNITS * (base + i)].y;
+        atomicAdd(cbase + count_slot(site, v.count_stride), c);
+      }
+    }
+  }
 }
 
 hipError_t launch_hop_ens(const ens_args& a, bool count, bool sample, hipStream_t st) {
