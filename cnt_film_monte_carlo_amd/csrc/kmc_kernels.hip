@@ -12,8 +12,6 @@
 // it bit for bit.
 #include <hip/hip_runtime.h>
 
-#include <type_traits>
-
 #include "kmc_internal.h"
 #include "kmc_kernels.h"
 #include "philox.h"
@@ -601,73 +599,6 @@ struct count_cache {
   }
 };
 
-// count_cache extended by a victim cache of four more (site, count) entries per lane in LDS
-// (private to the lane, 32 bytes): together an eight-entry LRU.  A site leaving the registers
-// goes to the front of the victim entries; a site found there moves back with its count; only
-// the entry pushed out of the victim cache becomes a device atomic.  On the 625k-site film this
-// cuts the atomics from 0.41 to 0.27 per hop.
-struct count_cache_lds {
-  uint32_t s0 = 0xFFFFFFFFu, s1 = 0xFFFFFFFFu, s2 = 0xFFFFFFFFu, s3 = 0xFFFFFFFFu;
-  uint32_t n0 = 0, n1 = 0, n2 = 0, n3 = 0;
-  uint4* vic;  // vic[0] = victim sites (most recent first), vic[1] = their counts
-
-  __device__ __forceinline__ void init(uint4* lane_area) {
-    vic = lane_area;
-    vic[0] = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
-    vic[1] = make_uint4(0u, 0u, 0u, 0u);
-  }
-  __device__ __forceinline__ void add(uint32_t s, uint32_t* __restrict__ base) {
-    // register entries: selects instead of branches (lanes of a wave hit different entries); the
-    // entries above the hit move down one, a miss moves all four and (s3, n3) to the victims
-    const bool h0 = s == s0, h1 = s == s1, h2 = s == s2, h3 = s == s3;
-    uint32_t c = h0 ? n0 : h1 ? n1 : h2 ? n2 : n3;
-    const bool m1 = !h0, m2 = m1 && !h1, m3 = m2 && !h2;
-    const uint32_t out_s = s3, out_n = n3;
-    s3 = m3 ? s2 : s3;
-    n3 = m3 ? n2 : n3;
-    s2 = m2 ? s1 : s2;
-    n2 = m2 ? n1 : n2;
-    s1 = m1 ? s0 : s1;
-    n1 = m1 ? n0 : n1;
-    if (m3 && !h3) {
-      // (out_s, out_n) leaves the registers for the front of the victim entries; s comes back
-      // from them with its count if it is there, else the oldest victim entry goes to the device
-      uint4 vs = vic[0], vc = vic[1];
-      c = 0;
-      if (vs.x == s) {
-        c = vc.x;
-        vs.x = out_s; vc.x = out_n;
-      } else if (vs.y == s) {
-        c = vc.y;
-        vs.y = vs.x; vc.y = vc.x; vs.x = out_s; vc.x = out_n;
-      } else if (vs.z == s) {
-        c = vc.z;
-        vs.z = vs.y; vc.z = vc.y; vs.y = vs.x; vc.y = vc.x; vs.x = out_s; vc.x = out_n;
-      } else {
-        if (vs.w == s) c = vc.w;
-        else if (vc.w) atomicAdd(base + vs.w, vc.w);
-        vs.w = vs.z; vc.w = vc.z; vs.z = vs.y; vc.z = vc.y; vs.y = vs.x; vc.y = vc.x;
-        vs.x = out_s; vc.x = out_n;
-      }
-      vic[0] = vs;
-      vic[1] = vc;
-    }
-    s0 = s;
-    n0 = c + 1u;
-  }
-  __device__ __forceinline__ void flush(uint32_t* __restrict__ base) {
-    if (n0) atomicAdd(base + s0, n0);
-    if (n1) atomicAdd(base + s1, n1);
-    if (n2) atomicAdd(base + s2, n2);
-    if (n3) atomicAdd(base + s3, n3);
-    const uint4 vs = vic[0], vc = vic[1];
-    if (vc.x) atomicAdd(base + vs.x, vc.x);
-    if (vc.y) atomicAdd(base + vs.y, vc.y);
-    if (vc.z) atomicAdd(base + vs.z, vc.z);
-    if (vc.w) atomicAdd(base + vs.w, vc.w);
-  }
-};
-
 // Hardware id of the XCD running this wave (gfx950: 8 XCDs).  Only picks which private copy
 // of the count array a wave updates; correctness does not depend on it (device-scope atomics).
 __device__ __forceinline__ uint32_t xcd_id() {
@@ -969,15 +900,16 @@ __device__ __forceinline__ void hop_exciton(const net_view& v, uint32_t e, uint6
 // target row is in registers (displacement, visit hash, dwell at the new site), which keeps every
 // exciton's arithmetic and its order identical to hop_exciton.  A drain target's header and
 // position are gathered by the lanes concerned, and their next row load is the source site's.
-// WINDOW: departures from the block lines [win_base, win_base + COUNT_WINDOW) are counted in the
-// workgroup's LDS window `win` (flushed by the kernel), the rest through the register LRU.
+// COUNT: departures from the block lines [win_base, win_base + COUNT_WINDOW) are counted in the
+// workgroup's LDS window `win` (zeroed and flushed by the kernel), the rest through a
+// four-entry register LRU into the XCD's copy of the counts.
 constexpr uint32_t COUNT_WINDOW = 2048;
 
-template <bool COUNT, bool SAMPLE, bool TRACE, bool WINDOW = false>
+template <bool COUNT, bool SAMPLE, bool TRACE>
 __device__ __forceinline__ void hop_wave(const net_view& v, uint32_t e, bool valid,
                                          uint64_t hop_budget, double t_max, const trace_view& tv,
-                                         uint4* lds_wave, uint32_t lane, uint4* lds_victim,
-                                         uint32_t* win = nullptr, uint32_t win_base = 0u) {
+                                         uint4* lds_wave, uint32_t lane, uint32_t* win,
+                                         uint32_t win_base) {
 #pragma clang fp contract(off)
   double t = INFINITY, dx = 0.0, dy = 0.0, dz = 0.0;
   uint64_t h = 0, hs = 0;
@@ -1003,8 +935,7 @@ __device__ __forceinline__ void hop_wave(const net_view& v, uint32_t e, bool val
   uint32_t r_dw = 0;           // its dwell draw
   float px = 0.f, py = 0.f, pz = 0.f;  // position of the site it left
   double t_hop = 0.0;
-  typename std::conditional<WINDOW, count_cache, count_cache_lds>::type cc;
-  if constexpr (COUNT && !WINDOW) cc.init(lds_victim);
+  count_cache cc;
   uint32_t* cbase = COUNT ? v.site_count + (size_t)xcd_id() * v.count_stride : nullptr;
   row_regs row;
 
@@ -1046,7 +977,7 @@ __device__ __forceinline__ void hop_wave(const net_view& v, uint32_t e, bool val
       const uint32_t tg = target_of(v.blocks, W, row, idx);
       if (COUNT) {
         const uint32_t off = (W & TGT_LINE_MASK) - win_base;
-        if (WINDOW && off < COUNT_WINDOW) atomicAdd(win + off, 1u);
+        if (off < COUNT_WINDOW) atomicAdd(win + off, 1u);
         else cc.add(count_slot(row.site, v.count_stride), cbase);
       }
       const uint32_t r = nonzero_draw(w.v[1], w.v[3], v.seed, e, h, STREAM_HOP_REDRAW);
@@ -1119,7 +1050,7 @@ __global__ __launch_bounds__(256, 4) void k_hop(hop_args a) {
   if constexpr (!COUNT) {
     if (wave_live)
       hop_wave<COUNT, SAMPLE, TRACE>(a.v, e, valid, a.hop_budget, a.t_max, a.tv, lds_wave, lane,
-                                     nullptr);
+                                     nullptr, 0u);
   } else {
     // The workgroup's excitons are consecutive in the order sorted by block line, and their
     // hops in one sub-launch stay mostly among nearby blocks: departures from the COUNT_WINDOW
@@ -1134,8 +1065,8 @@ __global__ __launch_bounds__(256, 4) void k_hop(hop_args a) {
     }
     __syncthreads();
     if (wave_live)
-      hop_wave<COUNT, SAMPLE, TRACE, true>(a.v, e, valid, a.hop_budget, a.t_max, a.tv, lds_wave,
-                                           lane, nullptr, win, base);
+      hop_wave<COUNT, SAMPLE, TRACE>(a.v, e, valid, a.hop_budget, a.t_max, a.tv, lds_wave, lane,
+                                     win, base);
     __syncthreads();
     uint32_t* cbase = a.v.site_count + (size_t)xcd_id() * a.v.count_stride;
     for (uint32_t i = threadIdx.x; i < COUNT_WINDOW; i += blockDim.x) {
@@ -1183,7 +1114,7 @@ __global__ __launch_bounds__(256, 4) void k_hop_ens(ens_args a) {
   if constexpr (!COUNT) {
     if (wave_live)
       hop_wave<COUNT, SAMPLE, false>(v, e, valid, a.hop_budget, a.t_max, trace_view{}, lds_wave,
-                                     lane, nullptr);
+                                     lane, nullptr, 0u);
   } else {
     for (uint32_t i = threadIdx.x; i < COUNT_WINDOW; i += blockDim.x) win[i] = 0u;
     const int64_t q0 = (int64_t)p0 - a.pbase[net];
@@ -1195,8 +1126,8 @@ __global__ __launch_bounds__(256, 4) void k_hop_ens(ens_args a) {
     }
     __syncthreads();
     if (wave_live)
-      hop_wave<COUNT, SAMPLE, false, true>(v, e, valid, a.hop_budget, a.t_max, trace_view{},
-                                           lds_wave, lane, nullptr, win, base);
+      hop_wave<COUNT, SAMPLE, false>(v, e, valid, a.hop_budget, a.t_max, trace_view{}, lds_wave,
+                                     lane, win, base);
     __syncthreads();
     uint32_t* cbase = v.site_count + (size_t)xcd_id() * v.count_stride;
     for (uint32_t i = threadIdx.x; i < COUNT_WINDOW; i += blockDim.x) {
