@@ -64,11 +64,18 @@ def cpu_baseline(net, w, tables, csr, seconds=12.0):
     t0 = time.perf_counter()
     pilot.run(200, net=onet)
     rate = E * 200 / (time.perf_counter() - t0)
-    H = int(max(50, min(20000, rate * seconds / E)))
+    H = int(max(50, min(200_000, rate * seconds / E)))
     ex = orc.Excitons(csr, net["flags"], seed=99, n_excitons=E)
     t0 = time.perf_counter()
     ex.run(H, net=onet)
     dt = time.perf_counter() - t0
+    if dt < seconds / 3 and H < 200_000:
+        # the short pilot underestimates the steady rate: size the sample again from this run
+        H = int(max(50, min(200_000, H * seconds / dt)))
+        ex = orc.Excitons(csr, net["flags"], seed=99, n_excitons=E)
+        t0 = time.perf_counter()
+        ex.run(H, net=onet)
+        dt = time.perf_counter() - t0
     faithful = E * H / dt
     log(f"cpu faithful: {E}x{H} hops in {dt:.2f} s = {faithful:.3e} hops/s on {threads} threads")
     Hc = 2000
