@@ -112,8 +112,11 @@ struct kmc_ctx {
   dbuf<uint8_t> x_heading;
   dbuf<double> x_soff, x_tlast;
   dbuf<uint32_t> d_site_count;  // KMC_COUNT_COPIES copies, count_stride apart
-  dbuf<uint64_t> d_counts64;
+  dbuf<uint64_t> d_counts64;    // running 64-bit totals the copies are folded into
   int64_t count_stride = 0;
+  uint64_t count_pending = 0;   // bound on any copy's count since the last fold
+  dbuf<uint32_t> d_live;        // excitons able to hop after the last launch
+  int64_t run_launches = 0;     // hop launches of the last kmc_run
 
   // MSD records
   int32_t n_bins = 0;
@@ -270,6 +273,7 @@ net_view view_of(kmc_ctx* c) {
   v.velocity = c->exc_velocity;
   v.links = c->d_links.p;
   v.fly_err = c->d_fly_err.p;
+  v.live = c->d_live.p;
   return v;
 }
 
@@ -987,6 +991,12 @@ int kmc_inject(kmc_ctx* c, int64_t n_excitons) {
   KMC_TRY(dalloc(c, c->d_site_count, (size_t)KMC_COUNT_COPIES * c->count_stride));
   KMC_HIP(c, hipMemsetAsync(c->d_site_count.p, 0, c->d_site_count.n * sizeof(uint32_t),
                             c->stream));
+  KMC_TRY(dalloc(c, c->d_counts64, c->n_sites));
+  KMC_HIP(c, hipMemsetAsync(c->d_counts64.p, 0, c->n_sites * sizeof(uint64_t), c->stream));
+  c->count_pending = 0;
+  KMC_TRY(dalloc(c, c->d_live, 1));
+  KMC_HIP(c, hipMemsetAsync(c->d_live.p, 0, sizeof(uint32_t), c->stream));
+  c->run_launches = 0;
   c->n_exc = E;
   c->n_bins = 0;
   c->n_trace = 0;
@@ -1043,12 +1053,29 @@ int kmc_set_trace(kmc_ctx* c, int64_t n_trace, int64_t max_hops) {
   return KMC_OK;
 }
 
+// Departure counts: each sub-launch adds at most E x (its hops) to any slot of a 32-bit copy,
+// so launches are capped at UINT32_MAX / E hops while counting, and the copies are folded into
+// the 64-bit totals before a launch that could take `pending` past UINT32_MAX.
+constexpr uint64_t COUNT_COPY_MAX = 0xFFFFFFFFull;
+
+// Sub-launches between two reads of the device's live-exciton count (a host synchronisation):
+// a run stops early once no exciton can hop (all past t_max or trapped).
+constexpr int64_t LIVE_CHECK_EVERY = 8;
+
+static int fold_counts(kmc_ctx* c, hipStream_t st) {
+  KMC_HIP(c, launch_fold_counts(c->n_sites, c->count_stride, c->d_site_count.p,
+                                c->d_counts64.p, st));
+  c->count_pending = 0;
+  return KMC_OK;
+}
+
 // The hop budget runs as sub-launches of at most `chunk` hops (0: one launch); before each, the
 // excitons are sorted by the block of their current site (radix sort of 32-bit keys) so that
 // neighbouring threads and each XCD's L2 share rows.  Draws are keyed by exciton and hop, so the
-// split and the order leave every trajectory unchanged.
+// split and the order leave every trajectory unchanged.  With `fold` the counts go to the
+// context's own copies (folded into its totals as needed); without, to a scratch array.
 static int run_hops(kmc_ctx* c, hop_args a, int64_t hops, int64_t chunk, bool count, bool sample,
-                    bool trace) {
+                    bool trace, bool fold) {
   const int64_t E = c->n_exc;
   const bool fly = c->exc_velocity > 0.0;
   if (fly) KMC_HIP(c, hipMemsetAsync(c->d_fly_err.p, 0, sizeof(int32_t), c->stream));
@@ -1063,12 +1090,18 @@ static int run_hops(kmc_ctx* c, hop_args a, int64_t hops, int64_t chunk, bool co
     c->sort_tmp_bytes = need;
   }
   a.perm = c->d_ids.p + E;
-  const int64_t step = chunk > 0 ? chunk : std::max<int64_t>(hops, 1);
-  int64_t done = 0;
+  int64_t step = chunk > 0 ? chunk : std::max<int64_t>(hops, 1);
+  if (count) step = std::min<int64_t>(step, (int64_t)(COUNT_COPY_MAX / (uint64_t)E));
+  int64_t done = 0, launches = 0;
   do {
     const int64_t sub = std::min(step, hops - done);
+    if (count && fold) {
+      const uint64_t add = (uint64_t)E * (uint64_t)sub;
+      if (c->count_pending + add > COUNT_COPY_MAX) KMC_TRY(fold_counts(c, c->stream));
+      c->count_pending += add;
+    }
     KMC_HIP(c, launch_exciton_keys(E, c->x_site.p, c->d_roff.p, c->d_keys.p, c->d_ids.p,
-                                   c->stream));
+                                   c->d_live.p, c->stream));
     size_t have = c->sort_tmp_bytes;
     KMC_HIP(c, hipcub::DeviceRadixSort::SortPairs(c->d_sort_tmp.p, have, c->d_keys.p,
                                                   c->d_keys.p + E, c->d_ids.p, c->d_ids.p + E,
@@ -1076,7 +1109,14 @@ static int run_hops(kmc_ctx* c, hop_args a, int64_t hops, int64_t chunk, bool co
     a.hop_budget = (uint64_t)sub;
     KMC_HIP(c, launch_hop(a, count, sample, trace, fly, c->stream));
     done += sub;
+    ++launches;
+    if (done < hops && launches % LIVE_CHECK_EVERY == 0) {
+      uint32_t live = 0;
+      KMC_TRY(copy_out(c, &live, c->d_live.p, sizeof(uint32_t)));
+      if (live == 0u) break;
+    }
   } while (done < hops);
+  c->run_launches = launches;
   if (fly) {
     int32_t flag = 0;
     KMC_TRY(copy_out(c, &flag, c->d_fly_err.p, sizeof(int32_t)));
@@ -1135,7 +1175,7 @@ int kmc_run(kmc_ctx* c, int64_t hops_per_exciton, double t_max, int32_t count_si
   a.tv.tr_n = c->tr_n.p;
   KMC_HIP(c, hipEventRecord(c->ev0, c->stream));
   KMC_TRY(run_hops(c, a, hops_per_exciton, c->resort_hops, count_sites != 0, c->n_bins > 0,
-                   c->n_trace > 0 && c->tr_max > 0));
+                   c->n_trace > 0 && c->tr_max > 0, true));
   KMC_HIP(c, hipEventRecord(c->ev1, c->stream));
   if (kernel_ms) {
     KMC_HIP(c, hipEventSynchronize(c->ev1));
@@ -1172,10 +1212,21 @@ int kmc_get_hop_counts(kmc_ctx* c, uint64_t* counts) {
   if (!c || !counts) return fail(c, KMC_ERR_ARG, "kmc_get_hop_counts: null argument");
   if (c->n_exc == 0) return fail(c, KMC_ERR_STATE, "kmc_get_hop_counts: no excitons");
   KMC_HIP(c, hipSetDevice(c->cfg.device));
-  KMC_TRY(dalloc(c, c->d_counts64, c->n_sites));
-  KMC_HIP(c, launch_sum_counts(c->n_sites, c->count_stride, c->d_site_count.p, c->d_counts64.p,
-                               c->stream));
+  if (c->count_pending) KMC_TRY(fold_counts(c, c->stream));
   KMC_TRY(copy_out(c, counts, c->d_counts64.p, c->n_sites * sizeof(uint64_t)));
+  return KMC_OK;
+}
+
+int kmc_get_run_stats(kmc_ctx* c, int64_t* launches, int64_t* live_excitons) {
+  if (!c) return fail(nullptr, KMC_ERR_ARG, "kmc_get_run_stats: null context");
+  if (c->n_exc == 0) return fail(c, KMC_ERR_STATE, "kmc_get_run_stats: no excitons");
+  KMC_HIP(c, hipSetDevice(c->cfg.device));
+  if (launches) *launches = c->run_launches;
+  if (live_excitons) {
+    uint32_t live = 0;
+    KMC_TRY(copy_out(c, &live, c->d_live.p, sizeof(uint32_t)));
+    *live_excitons = c->run_launches > 0 ? (int64_t)live : -1;
+  }
   return KMC_OK;
 }
 
@@ -1291,6 +1342,8 @@ struct kmc_ensemble {
   size_t sort_bytes = 0;
   int end_bit = 33;
   int64_t resort_hops = 625;
+  uint32_t* d_live = nullptr;    // excitons of all members able to hop after the last launch
+  int64_t max_exc = 0;           // the largest member (bounds a launch's count per slot)
 };
 
 namespace {
@@ -1321,6 +1374,7 @@ int ens_upload(kmc_ensemble* m, F modify) {
                                   "networks with kmc_run)");
     ENS_HIP(hipStreamSynchronize(c->stream));
     m->views[k] = view_of(c);
+    m->views[k].live = m->d_live;
     modify(k, m->views[k]);
   }
   ENS_HIP(hipMemcpyAsync(m->d_nets, m->views.data(), m->views.size() * sizeof(net_view),
@@ -1328,8 +1382,10 @@ int ens_upload(kmc_ensemble* m, F modify) {
   return KMC_OK;
 }
 
+// As run_hops, for all members in one launch per sub-launch.  With `fold` the counts go to the
+// members' own copies, folded into their totals as needed; without, to the views' scratch.
 int ens_run_hops(kmc_ensemble* m, int64_t hops, int64_t chunk, bool count, bool sample,
-                 double t_max) {
+                 double t_max, bool fold) {
   ens_args a{};
   a.nets = m->d_nets;
   a.wave_net = m->d_wave_net;
@@ -1338,12 +1394,24 @@ int ens_run_hops(kmc_ensemble* m, int64_t hops, int64_t chunk, bool count, bool 
   a.perm = m->d_ids + m->n_total;
   a.n_positions = (uint64_t)m->n_positions;
   a.t_max = t_max;
-  const int64_t step = chunk > 0 ? chunk : std::max<int64_t>(hops, 1);
-  int64_t done = 0;
+  int64_t step = chunk > 0 ? chunk : std::max<int64_t>(hops, 1);
+  if (count) step = std::min<int64_t>(step, (int64_t)(COUNT_COPY_MAX / (uint64_t)m->max_exc));
+  int64_t done = 0, launches = 0;
   do {
     const int64_t sub = std::min(step, hops - done);
+    if (count && fold) {
+      for (kmc_ctx* c : m->ctxs) {
+        const uint64_t add = (uint64_t)c->n_exc * (uint64_t)sub;
+        if (c->count_pending + add > COUNT_COPY_MAX) {
+          ENS_HIP(launch_fold_counts(c->n_sites, c->count_stride, c->d_site_count.p,
+                                     c->d_counts64.p, m->stream));
+          c->count_pending = 0;
+        }
+        c->count_pending += add;
+      }
+    }
     ENS_HIP(launch_ens_keys(m->n_total, m->d_nets, m->d_exc_net, m->d_ebase, m->d_keys, m->d_ids,
-                            m->stream));
+                            m->d_live, m->stream));
     size_t have = m->sort_bytes;
     ENS_HIP(hipcub::DeviceRadixSort::SortPairs(m->d_sort_tmp, have, m->d_keys,
                                                m->d_keys + m->n_total, m->d_ids,
@@ -1352,6 +1420,14 @@ int ens_run_hops(kmc_ensemble* m, int64_t hops, int64_t chunk, bool count, bool 
     a.hop_budget = (uint64_t)sub;
     ENS_HIP(launch_hop_ens(a, count, sample, m->stream));
     done += sub;
+    ++launches;
+    if (done < hops && launches % LIVE_CHECK_EVERY == 0) {
+      uint32_t live = 0;
+      ENS_HIP(hipMemcpyAsync(&live, m->d_live, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                             m->stream));
+      ENS_HIP(hipStreamSynchronize(m->stream));
+      if (live == 0u) break;
+    }
   } while (done < hops);
   return KMC_OK;
 }
@@ -1387,7 +1463,7 @@ int kmc_autotune_schedule(kmc_ctx* c, int64_t trial_hops, int32_t n_cand, const 
   for (int i = 0; i < n_cand && st == KMC_OK; ++i) {
     st = save_state(c, bak, true);
     if (st == KMC_OK && hipEventRecord(c->ev0, c->stream) != hipSuccess) st = KMC_ERR_HIP;
-    if (st == KMC_OK) st = run_hops(c, a, trial_hops, cand[i], true, false, false);
+    if (st == KMC_OK) st = run_hops(c, a, trial_hops, cand[i], true, false, false, false);
     if (st == KMC_OK && hipEventRecord(c->ev1, c->stream) != hipSuccess) st = KMC_ERR_HIP;
     float ms = 0.f;
     if (st == KMC_OK && (hipEventSynchronize(c->ev1) != hipSuccess ||
@@ -1417,7 +1493,7 @@ int kmc_ensemble_destroy(kmc_ensemble* m) {
   (void)hipSetDevice(m->device);
   if (m->stream) (void)hipStreamSynchronize(m->stream);
   void* bufs[] = {m->d_nets, m->d_wave_net, m->d_exc_net, m->d_pbase, m->d_ebase,
-                  m->d_keys, m->d_ids,      m->d_sort_tmp};
+                  m->d_keys, m->d_ids,      m->d_sort_tmp, m->d_live};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (m->ev0) (void)hipEventDestroy(m->ev0);
@@ -1456,7 +1532,10 @@ int kmc_ensemble_create(int32_t n_networks, kmc_ctx* const* networks, kmc_ensemb
   kmc_ensemble* m = new kmc_ensemble();
   m->device = dev;
   m->ctxs.assign(networks, networks + n_networks);
-  for (kmc_ctx* c : m->ctxs) m->n_exc.push_back(c->n_exc);
+  for (kmc_ctx* c : m->ctxs) {
+    m->n_exc.push_back(c->n_exc);
+    m->max_exc = std::max<int64_t>(m->max_exc, c->n_exc);
+  }
   m->views.resize(n_networks);
   m->n_total = ne;
   m->n_positions = np;
@@ -1479,6 +1558,8 @@ int kmc_ensemble_create(int32_t n_networks, kmc_ctx* const* networks, kmc_ensemb
                                            m->d_ids, m->d_ids + ne, (int)ne, 0, m->end_bit,
                                            m->stream);
   if (e == hipSuccess) e = hipMalloc(&m->d_sort_tmp, std::max<size_t>(m->sort_bytes, 1));
+  if (e == hipSuccess) e = hipMalloc((void**)&m->d_live, sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMemsetAsync(m->d_live, 0, sizeof(uint32_t), m->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(m->d_wave_net, wave_net.data(), wave_net.size() * 2, hipMemcpyHostToDevice, m->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(m->d_exc_net, exc_net.data(), exc_net.size() * 2, hipMemcpyHostToDevice, m->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(m->d_pbase, pbase.data(), n_networks * 8, hipMemcpyHostToDevice, m->stream);
@@ -1508,7 +1589,7 @@ int kmc_ensemble_run(kmc_ensemble* m, int64_t hops_per_exciton, double t_max, in
   int s = ens_upload(m, [](size_t, net_view&) {});
   if (s != KMC_OK) return s;
   ENS_HIP(hipEventRecord(m->ev0, m->stream));
-  s = ens_run_hops(m, hops_per_exciton, m->resort_hops, count_sites != 0, sample, t_max);
+  s = ens_run_hops(m, hops_per_exciton, m->resort_hops, count_sites != 0, sample, t_max, true);
   if (s != KMC_OK) return s;
   ENS_HIP(hipEventRecord(m->ev1, m->stream));
   ENS_HIP(hipStreamSynchronize(m->stream));
@@ -1557,7 +1638,7 @@ int kmc_ensemble_autotune(kmc_ensemble* m, int64_t trial_hops, int32_t n_cand, c
       if (hipStreamSynchronize(m->ctxs[k]->stream) != hipSuccess) st = KMC_ERR_HIP;
     float ms = 0.f;
     if (st == KMC_OK && hipEventRecord(m->ev0, m->stream) != hipSuccess) st = KMC_ERR_HIP;
-    if (st == KMC_OK) st = ens_run_hops(m, trial_hops, cand[i], true, false, INFINITY);
+    if (st == KMC_OK) st = ens_run_hops(m, trial_hops, cand[i], true, false, INFINITY, false);
     if (st == KMC_OK && (hipEventRecord(m->ev1, m->stream) != hipSuccess ||
                          hipEventSynchronize(m->ev1) != hipSuccess ||
                          hipEventElapsedTime(&ms, m->ev0, m->ev1) != hipSuccess))

@@ -134,6 +134,43 @@ int kmc_forster_table(double gamma_0, int32_t n_theta, const double* theta, int3
   return KMC_OK;
 }
 
+int kmc_table_rate(int32_t n_chir, int32_t n_theta, const double* theta, int32_t n_z,
+                   const double* z_shift, int32_t n_a1, const double* a1, int32_t n_a2,
+                   const double* a2, const double* rates, int64_t n, const uint8_t* ci,
+                   const uint8_t* cj, const double* q_theta, const double* q_z, const double* q_a1,
+                   const double* q_a2, double* out, int32_t* nodes) {
+  // scattering_struct::get_rate (src/monte_carlo/scattering_struct.h:38-52) through the same
+  // table_rate the CSR kernels evaluate on the device
+  if (n_chir < 1 || n_chir > MAX_CHIR || n_theta < 1 || n_z < 1 || n_a1 < 1 || n_a2 < 1 ||
+      n < 0 || !theta || !z_shift || !a1 || !a2 || !rates || (n > 0 && (!q_theta || !q_z ||
+      !q_a1 || !q_a2 || !out)))
+    return KMC_ERR_ARG;
+  table_desc t;
+  t.n_chir = n_chir;
+  t.n_th = n_theta;
+  t.n_z = n_z;
+  t.n_a1 = n_a1;
+  t.n_a2 = n_a2;
+  t.rates = rates;
+  t.th = theta;
+  t.z = z_shift;
+  t.a1 = a1;
+  t.a2 = a2;
+  for (int64_t k = 0; k < n; ++k) {
+    const int c1 = ci ? ci[k] : 0, c2 = cj ? cj[k] : 0;
+    if (c1 >= n_chir || c2 >= n_chir) return KMC_ERR_ARG;
+    const pair_geom g{q_theta[k], q_z[k], q_a1[k], q_a2[k]};
+    out[k] = table_rate(t, c1, c2, g);
+    if (nodes) {
+      nodes[4 * k] = nearest_index(theta, n_theta, g.theta);
+      nodes[4 * k + 1] = nearest_index(z_shift, n_z, g.z);
+      nodes[4 * k + 2] = nearest_index(a1, n_a1, g.a1);
+      nodes[4 * k + 3] = nearest_index(a2, n_a2, g.a2);
+    }
+  }
+  return KMC_OK;
+}
+
 int kmc_generate_network(const kmc_netgen_params* p, int64_t capacity, double* x, double* y,
                          double* z, double* ox, double* oy, double* oz, int32_t* left,
                          int32_t* right, uint8_t* chirality, int32_t* tube_id, int64_t* n_out) {

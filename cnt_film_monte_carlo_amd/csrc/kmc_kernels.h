@@ -61,6 +61,9 @@ struct net_view {
   double velocity;
   const uint2* links;       // per site: {left, right} block line | drain bit, or LINK_NONE
   int32_t* fly_err;         // set when a flight exceeds FLY_MAX_PASSES site passes
+  // excitons still able to hop at the end of a launch (not trapped, t <= t_max), summed per
+  // wave; zeroed by the sort-key kernel ahead of every launch
+  uint32_t* live;
 };
 
 // Contact (steady-state) mode: particle slots [0, *n_live) of one buffer.
@@ -157,8 +160,9 @@ hipError_t launch_block_sizes(int64_t n, const site_hdr* hdr, const int32_t* ord
                               hipStream_t st);
 hipError_t launch_place_blocks(int64_t n, const int32_t* order, const int64_t* first, int64_t* roff,
                                hipStream_t st);
+// keys[e] = first line of exciton e's block, ids[e] = e; *live = 0 (the next launch's count)
 hipError_t launch_exciton_keys(int64_t n, const int32_t* site, const int64_t* roff, uint32_t* keys,
-                               uint32_t* ids, hipStream_t st);
+                               uint32_t* ids, uint32_t* live, hipStream_t st);
 hipError_t launch_fill_blocks(int64_t n, const site_hdr* hdr, const double* cum,
                               const edge_rec* edge, const int64_t* roff, const double* px,
                               const double* py, const double* pz, uint4* blocks, hipStream_t st);
@@ -173,8 +177,9 @@ hipError_t launch_morton_keys(int64_t n, const int32_t* cell_of, int32_t nx, int
 // out[i] = row word of site idx[i] (first line of its block | TGT_WIDE_BIT)
 hipError_t launch_gather_offsets(int64_t n, const int32_t* idx, const int64_t* roff,
                                  const site_hdr* hdr, uint32_t* out, hipStream_t st);
-hipError_t launch_sum_counts(int64_t n, int64_t stride, const uint32_t* copies, uint64_t* out,
-                             hipStream_t st);
+// totals[i] += the per-XCD departure counts of site i, which are then zeroed
+hipError_t launch_fold_counts(int64_t n, int64_t stride, uint32_t* copies, uint64_t* totals,
+                              hipStream_t st);
 hipError_t launch_pair_classes(int64_t n, int64_t nnz, const site_hdr* hdr, const edge_rec* edge,
                                const uint8_t* chir, int32_t n_chir, uint8_t* cls, uint32_t* counts,
                                hipStream_t st);
@@ -190,7 +195,8 @@ hipError_t launch_links(int64_t n, const int32_t* left, const int32_t* right, co
                         const int64_t* roff, const site_hdr* hdr, uint2* links, hipStream_t st);
 hipError_t launch_hop_ens(const ens_args& a, bool count, bool sample, hipStream_t st);
 hipError_t launch_ens_keys(int64_t n, const net_view* nets, const uint16_t* exc_net,
-                           const int64_t* ebase, uint64_t* keys, uint32_t* ids, hipStream_t st);
+                           const int64_t* ebase, uint64_t* keys, uint32_t* ids, uint32_t* live,
+                           hipStream_t st);
 hipError_t launch_reduce(const reduce_args& a, hipStream_t st);
 hipError_t launch_contact_step(const contact_args& a, bool fly, hipStream_t st);
 hipError_t launch_contact_keep(const contact_args& a, double y_lo, double y_hi, uint32_t* keep,

@@ -373,8 +373,10 @@ __global__ __launch_bounds__(256) void k_place_blocks(int64_t n, const int32_t* 
 __global__ __launch_bounds__(256) void k_exciton_keys(int64_t n, const int32_t* __restrict__ site,
                                                       const int64_t* __restrict__ roff,
                                                       uint32_t* __restrict__ keys,
-                                                      uint32_t* __restrict__ ids) {
+                                                      uint32_t* __restrict__ ids,
+                                                      uint32_t* __restrict__ live) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e == 0 && live) *live = 0u;
   if (e >= n) return;
   keys[e] = (uint32_t)roff[site[e]];
   ids[e] = (uint32_t)e;
@@ -757,8 +759,9 @@ __device__ __forceinline__ void fly_offset(const net_view& v, const flight_state
 // One exciton through its hop budget (event-driven form of particle::step,
 // src/monte_carlo/particle.cpp:57-80; FLY adds the drift of particle::fly for v > 0).  v holds everything of the exciton's network;
 // it is uniform across the wave, so its fields live in scalar registers.
+// Returns whether a further launch could move the exciton (t <= t_max, a dwell pending).
 template <bool COUNT, bool SAMPLE, bool TRACE, bool FLY = false>
-__device__ __forceinline__ void hop_exciton(const net_view& v, uint32_t e, uint64_t hop_budget,
+__device__ __forceinline__ bool hop_exciton(const net_view& v, uint32_t e, uint64_t hop_budget,
                                             double t_max, const trace_view& tv) {
 #pragma clang fp contract(off)
   double t = v.x.t_next[e];
@@ -891,7 +894,9 @@ __device__ __forceinline__ void hop_exciton(const net_view& v, uint32_t e, uint6
     v.x.s_off[e] = so;
     v.x.t_last[e] = tl;
     v.x.heading[e] = (uint8_t)hd;
+    return t <= t_max && t < INFINITY;
   }
+  return !trapped(row.n, row.gamma) && t <= t_max;
 }
 
 // The hop loop of one wave (v = 0), built around coop_load_row: every lane takes part in every
@@ -1013,6 +1018,11 @@ __device__ __forceinline__ void hop_wave(const net_view& v, uint32_t e, bool val
     }
   }
   if (COUNT) cc.flush(cbase);
+  if (v.live) {
+    // excitons that a further launch could move (the budget aside)
+    const uint64_t b = __ballot(valid && !trapped(row.n, row.gamma) && t <= t_max);
+    if (lane == 0u && b != 0ull) atomicAdd(v.live, (uint32_t)__popcll(b));
+  }
   if (!valid) return;
   v.x.site[e] = (int32_t)row.site;
   v.x.t_next[e] = t;
@@ -1083,9 +1093,15 @@ __global__ __launch_bounds__(256, 4) void k_hop(hop_args a) {
 template <bool COUNT, bool SAMPLE, bool TRACE>
 __global__ __launch_bounds__(256) void k_hop_fly(hop_args a) {
   const uint32_t pos = xcd_position();
-  if (pos >= a.v.x.n) return;
-  const uint32_t e = a.perm ? a.perm[pos] : pos;
-  hop_exciton<COUNT, SAMPLE, TRACE, true>(a.v, e, a.hop_budget, a.t_max, a.tv);
+  bool live = false;
+  if (pos < a.v.x.n) {
+    const uint32_t e = a.perm ? a.perm[pos] : pos;
+    live = hop_exciton<COUNT, SAMPLE, TRACE, true>(a.v, e, a.hop_budget, a.t_max, a.tv);
+  }
+  if (a.v.live) {
+    const uint64_t b = __ballot(live);
+    if ((threadIdx.x & 63u) == 0u && b != 0ull) atomicAdd(a.v.live, (uint32_t)__popcll(b));
+  }
 }
 
 // Several networks in one launch.  Network k's sorted excitons occupy positions
@@ -1156,8 +1172,10 @@ __global__ __launch_bounds__(256) void k_ens_keys(int64_t n, const net_view* __r
                                                   const uint16_t* __restrict__ exc_net,
                                                   const int64_t* __restrict__ ebase,
                                                   uint64_t* __restrict__ keys,
-                                                  uint32_t* __restrict__ ids) {
+                                                  uint32_t* __restrict__ ids,
+                                                  uint32_t* __restrict__ live) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g == 0 && live) *live = 0u;
   if (g >= n) return;
   const uint32_t k = exc_net[g];
   const net_view& v = nets[k];
@@ -1167,10 +1185,11 @@ __global__ __launch_bounds__(256) void k_ens_keys(int64_t n, const net_view* __r
 }
 
 hipError_t launch_ens_keys(int64_t n, const net_view* nets, const uint16_t* exc_net,
-                           const int64_t* ebase, uint64_t* keys, uint32_t* ids, hipStream_t st) {
+                           const int64_t* ebase, uint64_t* keys, uint32_t* ids, uint32_t* live,
+                           hipStream_t st) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_ens_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, nets,
-                     exc_net, ebase, keys, ids);
+                     exc_net, ebase, keys, ids, live);
   return hipGetLastError();
 }
 
@@ -1198,17 +1217,22 @@ hipError_t launch_hop(const hop_args& a, bool count, bool sample, bool trace, bo
   return hipErrorInvalidValue;
 }
 
-// Sum of the per-XCD copies of the departure counts.
-__global__ __launch_bounds__(256) void k_sum_counts(int64_t n, int64_t stride,
-                                                    const uint32_t* __restrict__ copies,
-                                                    uint64_t* __restrict__ out) {
+// The per-XCD copies of the departure counts folded into the 64-bit totals and zeroed (slots
+// and sites correspond one to one, so every thread owns its slot in all copies).
+__global__ __launch_bounds__(256) void k_fold_counts(int64_t n, int64_t stride,
+                                                     uint32_t* __restrict__ copies,
+                                                     uint64_t* __restrict__ totals) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   uint64_t s = 0;
   const uint32_t slot = count_slot((uint32_t)i, stride);
 #pragma unroll
-  for (int x = 0; x < KMC_COUNT_COPIES; ++x) s += copies[(size_t)x * stride + slot];
-  out[i] = s;
+  for (int x = 0; x < KMC_COUNT_COPIES; ++x) {
+    uint32_t* c = copies + (size_t)x * stride + slot;
+    s += *c;
+    *c = 0u;
+  }
+  totals[i] += s;
 }
 
 // =============================================================================================
@@ -1627,9 +1651,9 @@ hipError_t launch_place_blocks(int64_t n, const int32_t* order, const int64_t* f
 }
 
 hipError_t launch_exciton_keys(int64_t n, const int32_t* site, const int64_t* roff, uint32_t* keys,
-                               uint32_t* ids, hipStream_t st) {
+                               uint32_t* ids, uint32_t* live, hipStream_t st) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_exciton_keys, grid_for(n), dim3(256), 0, st, n, site, roff, keys, ids);
+  hipLaunchKernelGGL(k_exciton_keys, grid_for(n), dim3(256), 0, st, n, site, roff, keys, ids, live);
   return hipGetLastError();
 }
 
@@ -1758,10 +1782,10 @@ hipError_t launch_gather_offsets(int64_t n, const int32_t* idx, const int64_t* r
   return hipGetLastError();
 }
 
-hipError_t launch_sum_counts(int64_t n, int64_t stride, const uint32_t* copies, uint64_t* out,
-                             hipStream_t st) {
+hipError_t launch_fold_counts(int64_t n, int64_t stride, uint32_t* copies, uint64_t* totals,
+                              hipStream_t st) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_sum_counts, grid_for(n), dim3(256), 0, st, n, stride, copies, out);
+  hipLaunchKernelGGL(k_fold_counts, grid_for(n), dim3(256), 0, st, n, stride, copies, totals);
   return hipGetLastError();
 }
 

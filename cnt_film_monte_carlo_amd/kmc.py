@@ -90,6 +90,7 @@ _SIGS = {
     "kmc_run": (C.c_int, [_P, C.c_int64, C.c_double, C.c_int32, _D]),
     "kmc_get_excitons": (C.c_int, [_P, _I32, _D, _U64, _D, _U32, _U64]),
     "kmc_get_hop_counts": (C.c_int, [_P, _U64]),
+    "kmc_get_run_stats": (C.c_int, [_P, _I64, _I64]),
     "kmc_get_samples": (C.c_int, [_P, _D, _U32, _U32]),
     "kmc_get_trace": (C.c_int, [_P, _I32, _D, _I64]),
     "kmc_reduce_samples": (C.c_int, [_P, C.c_int32, _D, _D]),
@@ -100,6 +101,9 @@ _SIGS = {
     "kmc_forster_table": (C.c_int, [C.c_double, C.c_int32, _D, C.c_int32, _D, C.c_int32, _D,
                                     C.c_int32, _D, _D]),
     "kmc_linspace": (C.c_int, [C.c_double, C.c_double, C.c_int32, _D]),
+    "kmc_table_rate": (C.c_int, [C.c_int32, C.c_int32, _D, C.c_int32, _D, C.c_int32, _D,
+                                 C.c_int32, _D, _D, C.c_int64, _U8, _U8, _D, _D, _D, _D, _D,
+                                 _I32]),
     "kmc_save_rate_table": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int64, C.c_int64, C.c_int64,
                                       C.c_int64, _D, _D, _D, _D, _D]),
     "kmc_load_rate_table": (C.c_int, [C.c_char_p, C.c_char_p, _I64, _D, _D, _D, _D, _D]),
@@ -375,6 +379,12 @@ class Context:
         self._ck(self.lib.kmc_run(self.h, int(hops_per_exciton), float(t_max), int(count_sites),
                                   C.byref(ms) if timed else None))
         return ms.value
+
+    def run_stats(self):
+        """(hop launches of the last run(), excitons still able to hop after it)."""
+        n, live = C.c_int64(0), C.c_int64(0)
+        self._ck(self.lib.kmc_get_run_stats(self.h, C.byref(n), C.byref(live)))
+        return n.value, live.value
 
     def set_hop_schedule(self, resort_hops: int):
         """Hops per sub-launch of run() between exciton sorts (0: one launch)."""
@@ -654,6 +664,28 @@ def linspace(start: float, end: float, n: int) -> np.ndarray:
     out = np.zeros(n)
     _check(library().kmc_linspace(start, end, n, _ptr(out, _D)))
     return out
+
+
+def table_rate(theta, z_shift, a1, a2, rates, q_theta, q_z_shift, q_a1, q_a2, ci=None,
+               cj=None):
+    """scattering_struct::get_rate (src/monte_carlo/scattering_struct.h:38-52) on the host for
+    arrays of geometries: (rates at the nearest nodes, node indices [n][4]).  rates is one table
+    (n_theta, n_z, n_a1, n_a2) or n_chir^2 of them stacked in front."""
+    g = [_c(v, np.float64) for v in (theta, z_shift, a1, a2)]
+    r = _c(rates, np.float64)
+    per = g[0].size * g[1].size * g[2].size * g[3].size
+    n_chir = int(round(math.sqrt(r.size // per)))
+    q = [_c(np.atleast_1d(v), np.float64) for v in (q_theta, q_z_shift, q_a1, q_a2)]
+    n = q[0].size
+    c1 = None if ci is None else _c(np.broadcast_to(ci, (n,)), np.uint8)
+    c2 = None if cj is None else _c(np.broadcast_to(cj, (n,)), np.uint8)
+    out = np.zeros(n)
+    nodes = np.zeros((n, 4), np.int32)
+    _check(library().kmc_table_rate(n_chir, g[0].size, _ptr(g[0], _D), g[1].size, _ptr(g[1], _D),
+                                    g[2].size, _ptr(g[2], _D), g[3].size, _ptr(g[3], _D),
+                                    _ptr(r, _D), n, _ptr(c1, _U8), _ptr(c2, _U8),
+                                    *[_ptr(v, _D) for v in q], _ptr(out, _D), _ptr(nodes, _I32)))
+    return out, nodes
 
 
 def forster_table(gamma_0: float, theta, z_shift, a1, a2) -> np.ndarray:

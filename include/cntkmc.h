@@ -92,6 +92,18 @@ int kmc_set_rate_table(kmc_ctx* ctx, int32_t n_chir, int32_t n_theta, const doub
                        int32_t n_z, const double* z_shift, int32_t n_a1, const double* axis_shift_1,
                        int32_t n_a2, const double* axis_shift_2, const double* rates);
 
+/* The nearest-node lookup of scattering_struct::get_rate (src/monte_carlo/scattering_struct.h:
+ * 38-52) on the host for n geometries, over tables laid out as kmc_set_rate_table takes them:
+ * out[k] = table (ci[k] * n_chir + cj[k]) at the node nearest to (theta, z_shift, a1, a2)[k] on
+ * each axis, the first such node on ties (index_min of |grid - x|).  nodes (optional, [n][4])
+ * receives the four node indices.  The device build of the CSR uses the same lookup.  Host only. */
+int kmc_table_rate(int32_t n_chir, int32_t n_theta, const double* theta, int32_t n_z,
+                   const double* z_shift, int32_t n_a1, const double* axis_shift_1, int32_t n_a2,
+                   const double* axis_shift_2, const double* rates, int64_t n, const uint8_t* ci,
+                   const uint8_t* cj, const double* q_theta, const double* q_z_shift,
+                   const double* q_axis_shift_1, const double* q_axis_shift_2, double* out,
+                   int32_t* nodes);
+
 /* Job (b): CSR out-rate table with FP64 cumulative sums.  Replaces scatterer::find_neighbors and
  * set_max_rate (src/monte_carlo/scatterer.cpp:34-83, src/monte_carlo/scatterer.h:89-93,
  * src/monte_carlo/monte_carlo.h:426-440).  Rows list neighbours in the reference order (stencil
@@ -137,9 +149,15 @@ int kmc_set_trace(kmc_ctx* ctx, int64_t n_trace, int64_t max_hops);
  * drain site keeps the displacement and the pending dwell time (kubo_step, monte_carlo.cpp:330-336).
  * count_sites != 0 adds per-site departure counts.  kernel_ms (may be NULL) receives the device
  * time of the whole pass (all sub-launches and the exciton sorts between them), measured with
- * HIP events on the context stream. */
+ * HIP events on the context stream.  The run ends early once no exciton can hop any more (every
+ * one past t_max or trapped), so a budget far beyond the hops that t_max allows (the reference's
+ * while (time < max_time) loop, src/main.cpp:64-76) returns after the hops that happen. */
 int kmc_run(kmc_ctx* ctx, int64_t hops_per_exciton, double t_max, int32_t count_sites,
             double* kernel_ms);
+/* After kmc_run: the hop launches it made and the excitons still able to hop (t <= t_max and
+ * not trapped; -1 before the first run).  No reference counterpart (its loop tests the clock,
+ * src/main.cpp:64-76). */
+int kmc_get_run_stats(kmc_ctx* ctx, int64_t* launches, int64_t* live_excitons);
 /* Launch schedule of kmc_run (no reference counterpart; results do not depend on it): the hop
  * budget runs as sub-launches of at most resort_hops hops, each preceded by a sort of the
  * excitons by the memory block of their current site; 0 = one launch (default 625). */
@@ -229,7 +247,8 @@ int kmc_ensemble_autotune(kmc_ensemble* ens, int64_t trial_hops, int32_t n_candi
  * [E][3], drain (current) count, FNV-1a hash of the visited-site sequence.  NULL = skip. */
 int kmc_get_excitons(kmc_ctx* ctx, int32_t* site, double* t_next, uint64_t* hops, double* disp,
                      uint32_t* drains, uint64_t* seq_hash);
-/* Per-site departure counts (hops that left each site). */
+/* Per-site departure counts (hops that left each site) since kmc_inject, 64-bit: the device
+ * keeps 32-bit per-XCD copies that are folded into 64-bit totals before any copy could wrap. */
 int kmc_get_hop_counts(kmc_ctx* ctx, uint64_t* counts);
 /* Raw MSD records: disp [n_bins][E][3] doubles, drains [n_bins][E], n_recorded [E]. */
 int kmc_get_samples(kmc_ctx* ctx, double* disp, uint32_t* drains, uint32_t* n_recorded);

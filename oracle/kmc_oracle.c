@@ -344,6 +344,30 @@ static double get_rate(const orc_net* w, int ci, int cj, double theta, double zs
   return w->t_rates[idx];
 }
 
+/* get_rate over caller-given tables (scattering_struct.h:38-52) for n queries; nodes [n][4]
+ * receives the node indices (optional).  Tables as orc_net_set_table takes them. */
+int orc_table_rate(int n_chir, int n_th, const double* th, int n_z, const double* zs, int n_a1,
+                   const double* a1, int n_a2, const double* a2, const double* rates, int64_t n,
+                   const uint8_t* ci, const uint8_t* cj, const double* q_th, const double* q_z,
+                   const double* q_a1, const double* q_a2, double* out, int32_t* nodes) {
+  int64_t k;
+  for (k = 0; k < n; ++k) {
+    const unsigned i0 = arma_index_min_abs(th, n_th, q_th[k]);
+    const unsigned i1 = arma_index_min_abs(zs, n_z, q_z[k]);
+    const unsigned i2 = arma_index_min_abs(a1, n_a1, q_a1[k]);
+    const unsigned i3 = arma_index_min_abs(a2, n_a2, q_a2[k]);
+    const size_t tab = (size_t)(ci ? ci[k] : 0) * (size_t)n_chir + (size_t)(cj ? cj[k] : 0);
+    out[k] = rates[(((tab * n_th + i0) * n_z + i1) * n_a1 + i2) * n_a2 + i3];
+    if (nodes) {
+      nodes[4 * k] = (int32_t)i0;
+      nodes[4 * k + 1] = (int32_t)i1;
+      nodes[4 * k + 2] = (int32_t)i2;
+      nodes[4 * k + 3] = (int32_t)i3;
+    }
+  }
+  return 0;
+}
+
 /* check_neighbor geometry (src/monte_carlo/scatterer.cpp:44-61) */
 static void orc_geometry(const double* pi, const double* pj, const double* a1, const double* a2,
                          double* theta, double* zsh, double* ash1, double* ash2) {

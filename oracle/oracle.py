@@ -79,6 +79,8 @@ def library() -> C.CDLL:
             "orc_reduce_samples": (C.c_int, [C.c_int64, C.c_int32, C.c_int32, _U32, _D, _U32,
                                              _D]),
             "orc_max_threads": (C.c_int, []),
+            "orc_table_rate": (C.c_int, [C.c_int, C.c_int, _D, C.c_int, _D, C.c_int, _D, C.c_int,
+                                         _D, _D, C.c_int64, _U8, _U8, _D, _D, _D, _D, _D, _I32]),
             "orc_contact": (C.c_int, [C.c_int64, _D, _D, _D, _I64, _I32, _D, C.c_uint64, C.c_int,
                                       C.c_int64, C.c_int64, C.c_int64, C.c_double, C.c_int64,
                                       C.c_double, _I32, _I32, _F, _D,
@@ -126,6 +128,25 @@ def forster_table(gamma_0, th, z, a1, a2) -> np.ndarray:
 def select_index(cum, dice) -> int:
     c = _c(cum, np.float64)
     return library().orc_select_index(_p(c, _D), c.size, float(dice))
+
+
+def table_rate(theta, z_shift, a1, a2, rates, q_theta, q_z_shift, q_a1, q_a2, ci=None, cj=None):
+    """get_rate of scattering_struct (oracle restatement) for arrays of geometries:
+    (rates, node indices [n][4])."""
+    g = [_c(v, np.float64) for v in (theta, z_shift, a1, a2)]
+    r = _c(rates, np.float64)
+    per = g[0].size * g[1].size * g[2].size * g[3].size
+    n_chir = int(round(math.sqrt(r.size // per)))
+    q = [_c(np.atleast_1d(v), np.float64) for v in (q_theta, q_z_shift, q_a1, q_a2)]
+    n = q[0].size
+    c1 = None if ci is None else _c(np.broadcast_to(ci, (n,)), np.uint8)
+    c2 = None if cj is None else _c(np.broadcast_to(cj, (n,)), np.uint8)
+    out = np.zeros(n)
+    nodes = np.zeros((n, 4), np.int32)
+    library().orc_table_rate(n_chir, g[0].size, _p(g[0], _D), g[1].size, _p(g[1], _D), g[2].size,
+                             _p(g[2], _D), g[3].size, _p(g[3], _D), _p(r, _D), n, _p(c1, _U8),
+                             _p(c2, _U8), *[_p(v, _D) for v in q], _p(out, _D), _p(nodes, _I32))
+    return out, nodes
 
 
 def dwell_time(max_rate, r) -> float:

@@ -1,0 +1,155 @@
+"""GPU tests of how a run ends and what it counts: the early end of kmc_run once no exciton can
+hop (the reference's clock-bounded Kubo loop, src/main.cpp:64-76, called with a huge hop budget),
+64-bit departure counts past 2^32 per device copy, and the nearest-node lookup on exact midpoint
+ties in the device CSR build (scattering_struct.h:42-49)."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import site_arrays
+
+pytestmark = pytest.mark.gpu
+NM = 1e-9
+T_REL = 1e-12
+BIG = (1 << 63) // 2
+
+
+def say(*a):
+    print("[gpu-run-control]", *a, file=sys.stderr, flush=True)
+
+
+def _compare_state(g, o, label):
+    assert np.array_equal(g["site"], o.site), label
+    assert np.array_equal(g["hops"], o.hops), label
+    assert np.array_equal(g["hash"], o.hash), label
+    assert np.array_equal(g["drains"], o.drains), label
+    assert np.array_equal(g["disp"], o.dsp), label
+    fin = np.isfinite(o.t_next)
+    assert np.array_equal(np.isfinite(g["t_next"]), fin)
+    np.testing.assert_allclose(g["t_next"][fin], o.t_next[fin], rtol=T_REL, atol=0)
+
+
+def test_huge_budget_ends_at_t_max(kmc, orc, nw):
+    """kmc_run(ctx, INT64_MAX / 2, t_cap) with the default schedule returns once every exciton
+    is past t_cap, bit-equal to the oracle's run to the same clock, after a handful of launches."""
+    say("huge budget: film_5k_sites 1000 excitons, t_max cap, budget 2^62")
+    w = nw.film_small()
+    net = nw.build_network(w)
+    th, z, a1, a2, rates = nw.forster_tables(1)
+    ctx = nw.make_context(w, net, seed=13)
+    ctx.set_rate_table(th, z, a1, a2, rates)
+    ctx.build_cells()
+    ctx.build_rates(kmc.RATE_TABLE)
+    E = 1000
+    ctx.inject(E)
+    csr = ctx.get_csr()
+    ox = orc.Excitons(csr, net["flags"], seed=13, n_excitons=E)
+    t_cap = float(np.median(ox.t_next)) * 400
+    ctx.run(BIG, t_max=t_cap)
+    launches, live = ctx.run_stats()
+    say(f"  {launches} launches, {live} excitons live")
+    assert live == 0 and 1 <= launches <= 64
+    ox.run(BIG, t_max=t_cap)
+    g = ctx.get_excitons()
+    _compare_state(g, ox, "huge budget")
+    assert np.all(g["t_next"] > t_cap)
+    assert np.array_equal(ctx.get_hop_counts(), ox.site_count)
+    # a second call has nothing left to do: one launch, no hop
+    ctx.run(BIG, t_max=t_cap)
+    assert ctx.run_stats() == (1, 0)
+    assert np.array_equal(ctx.get_excitons()["hops"], g["hops"])
+    ctx.close()
+
+
+def test_ensemble_huge_budget_ends(kmc, nw):
+    say("ensemble huge budget: 3 networks")
+    ws = nw.ensemble(n_networks=3, seed=17, n_min=2_000, n_max=8_000)
+    th, z, a1, a2, rates = nw.forster_tables(1)
+    ctxs = []
+    for i, w in enumerate(ws):
+        c = nw.make_context(w, nw.build_network(w), seed=300 + i)
+        c.set_rate_table(th, z, a1, a2, rates)
+        c.build_cells()
+        c.build_rates(kmc.RATE_TABLE)
+        c.inject(w.n_excitons)
+        ctxs.append(c)
+    t_cap = 2e-11
+    ens = kmc.Ensemble(ctxs)
+    ens.run(BIG, t_max=t_cap)
+    for c in ctxs:
+        t = c.get_excitons()["t_next"]
+        assert np.all(t > t_cap)
+    ens.close()
+    for c in ctxs:
+        c.close()
+
+
+def test_counts_beyond_32_bits(kmc):
+    """Two sites that are each other's only neighbour: 10^6 excitons x 10^5 hops leave each site
+    5e10 times, about 6e9 per XCD copy, past 2^32.  The copies are folded into 64-bit totals
+    before they can wrap, so the counts are exact."""
+    E, H = 1_000_000, 100_000
+    say(f"ping-pong: 2 sites, {E} x {H} hops")
+    pos = np.array([[0.0, 0.0, 0.0], [10 * NM, 0.0, 0.0]])
+    ori = np.tile([1.0, 0.0, 0.0], (2, 1))
+    net = site_arrays(pos, ori, flags=np.array([1, 0], np.uint8))
+    ctx = kmc.Context(kmc.Config(cutoff=15 * NM, seed=9))
+    ctx.set_sites(net["x"], net["y"], net["z"], net["ox"], net["oy"], net["oz"],
+                  flags=net["flags"])
+    ctx.set_rate_table([0.0], [0.0], [0.0], [0.0], np.full(1, 1e12))
+    ctx.build_cells()
+    ctx.build_rates(kmc.RATE_TABLE)
+    ctx.inject(E)
+    ms = ctx.run(H)
+    say(f"  {ms:.1f} ms ({E * H / ms / 1e6:.3e} hops/s)")
+    g = ctx.get_excitons()
+    assert np.all(g["hops"] == H)
+    counts = ctx.get_hop_counts()
+    assert counts.tolist() == [E * ((H + 1) // 2), E * (H // 2)]
+    assert counts[0] // 8 > (1 << 32)
+    # the run continues and the totals keep adding up
+    ctx.run(1001)
+    assert ctx.get_hop_counts().tolist() == [E * ((H + 1002) // 2), E * ((H + 1001) // 2)]
+    ctx.close()
+
+
+def _tie_grid(x):
+    """Two nodes equally far from x in FP64 (x on the exact midpoint)."""
+    for k in range(20, 80):
+        d = math.ldexp(1.0, -k)
+        lo, hi = x - d, x + d
+        if lo != x and hi != x and abs(lo - x) == abs(hi - x):
+            return np.array([lo, hi])
+    raise AssertionError(f"no exact tie around {x}")
+
+
+def test_device_lookup_takes_first_node_on_ties(kmc, orc):
+    """The CSR build's nearest-node lookup on the device at exact midpoint ties on all four axes:
+    the first node, as index_min (scattering_struct.h:42-49) and the oracle give.  Any axis
+    taking its second node would pick a different rate."""
+    say("device lookup: midpoint ties")
+    pos = np.array([[0.0, 0.0, 0.0], [10 * NM, 1 * NM, 0.0]])
+    ori = np.tile([1.0, 0.0, 0.0], (2, 1))
+    net = site_arrays(pos, ori, flags=np.array([1, 0], np.uint8))
+    onet = orc.Net(net, 15 * NM)
+    geo = onet.csr(mode=1, pair_rates=np.ones(2), geometry=True)
+    q = [float(geo[k][0]) for k in ("theta", "z", "a1", "a2")]   # row of site 0
+    say(f"  geometry of 0 -> 1: theta {q[0]}, z {q[1]}, a1 {q[2]}, a2 {q[3]}")
+    grids = [_tie_grid(v) for v in q]
+    rates = (1.0 + np.arange(16, dtype=float)).reshape(2, 2, 2, 2) * 1e12
+    ctx = kmc.Context(kmc.Config(cutoff=15 * NM, seed=1))
+    ctx.set_sites(net["x"], net["y"], net["z"], net["ox"], net["oy"], net["oz"],
+                  flags=net["flags"])
+    ctx.set_rate_table(*grids, rates)
+    ctx.build_cells()
+    ctx.build_rates(kmc.RATE_TABLE)
+    g = ctx.get_csr()
+    onet.set_table(*grids, rates)
+    o = onet.csr(0)
+    assert np.array_equal(g["cum"], o["cum"])
+    assert g["cum"][0] == rates[0, 0, 0, 0]
+    r_host, nodes = kmc.table_rate(*grids, rates, *[[v] for v in q])
+    assert nodes.tolist() == [[0, 0, 0, 0]] and r_host[0] == g["cum"][0]
+    ctx.close()

@@ -345,3 +345,134 @@ def test_divide_free_rand_max_quotient(orc):
     bad = orc.library().orc_div_rand_max_mismatches(1, 0x80000000, 50_000_000, 0x5EED)
     assert bad == 0
     assert time.perf_counter() - t < 120
+
+
+# --------------------------------------------------------------------------------------------
+# Nearest-node lookup (scattering_struct::get_rate, src/monte_carlo/scattering_struct.h:38-52)
+# --------------------------------------------------------------------------------------------
+def _first_argmin(grid, x):
+    """Brute force: numpy's argmin returns the first index of the minimum."""
+    return int(np.argmin(np.abs(np.asarray(grid) - x)))
+
+
+def _last_argmin(grid, x):
+    d = np.abs(np.asarray(grid) - x)
+    return int(np.flatnonzero(d == d.min())[-1])
+
+
+def _tie_points(grid):
+    """Midpoints between neighbouring nodes at which both distances are equal in FP64."""
+    out = []
+    for i in range(len(grid) - 1):
+        m = 0.5 * (grid[i] + grid[i + 1])
+        if abs(grid[i] - m) == abs(grid[i + 1] - m):
+            out.append(m)
+    return out
+
+
+def test_nearest_node_midpoint_ties(orc, kmc):
+    """index_min of |grid - x| takes the FIRST minimum (scattering_struct.h:42-49).  x is placed
+    exactly on midpoints between nodes (equal distances in FP64), on an integer grid and on the
+    default table grids, plus random points; the oracle's lookup and the product's host lookup
+    (the table_rate the CSR kernels run) must both give numpy's first argmin on every axis.  A
+    last-minimum (or nearest-upper) rule fails: the tie set is checked to tell the two apart."""
+    th, z, a1, a2 = kmc.default_grids()
+    rng = np.random.default_rng(21)
+    # the first axis on an integer grid (every midpoint an exact tie), the others on the
+    # default table grids (the exact ties among their midpoints, the nodes, random points)
+    grids = [np.arange(11, dtype=float), z, a1, a2]
+    n_ties = 0
+    for ax, g in enumerate(grids):
+        ties = _tie_points(g)
+        n_ties += len(ties)
+        span = g[-1] - g[0]
+        xs = np.array(ties + list(g) + list(rng.uniform(g[0] - 0.1 * span, g[-1] + 0.1 * span, 200)))
+        axes = [np.array([0.0]) for _ in range(4)]
+        axes[ax] = g
+        rates = np.arange(g.size, dtype=float).reshape([g.size if k == ax else 1 for k in range(4)])
+        q = [np.zeros(xs.size) for _ in range(4)]
+        q[ax] = xs
+        expect = np.array([_first_argmin(g, x) for x in xs])
+        r_o, n_o = orc.table_rate(*axes, rates, *q)
+        r_p, n_p = kmc.table_rate(*axes, rates, *q)
+        assert np.array_equal(n_o[:, ax], expect), f"oracle, axis {ax}"
+        assert np.array_equal(n_p[:, ax], expect), f"product, axis {ax}"
+        assert np.array_equal(r_o, expect.astype(float)) and np.array_equal(r_p, r_o)
+        if ties:
+            last = np.array([_last_argmin(g, x) for x in xs])
+            assert np.any(last != expect), f"axis {ax}: no tie separates first and last minimum"
+    assert len(_tie_points(grids[0])) == 10
+    assert n_ties >= 10
+
+
+def test_nearest_node_four_axes_and_classes(orc, kmc):
+    """All four axes and the chirality-pair offset at once: random geometries (a fifth at exact
+    node midpoints) over three chiralities; oracle, product and a brute-force index agree."""
+    th, z, a1, a2 = kmc.default_grids()
+    rng = np.random.default_rng(5)
+    rates = rng.random((9, th.size, z.size, a1.size, a2.size)) * 1e12
+    n = 4000
+    q = [rng.uniform(g[0] - 0.1 * (g[-1] - g[0]), g[-1] + 0.1 * (g[-1] - g[0]), n)
+         for g in (th, z, a1, a2)]
+    for k, g in enumerate((th, z, a1, a2)):
+        t = _tie_points(g)
+        if t:
+            sel = rng.random(n) < 0.2
+            q[k][sel] = rng.choice(t, sel.sum())
+    ci = rng.integers(0, 3, n).astype(np.uint8)
+    cj = rng.integers(0, 3, n).astype(np.uint8)
+    idx = np.stack([[_first_argmin(g, x) for x in qq] for g, qq in zip((th, z, a1, a2), q)], 1)
+    expect = rates[ci.astype(int) * 3 + cj, idx[:, 0], idx[:, 1], idx[:, 2], idx[:, 3]]
+    r_o, n_o = orc.table_rate(th, z, a1, a2, rates, *q, ci=ci, cj=cj)
+    r_p, n_p = kmc.table_rate(th, z, a1, a2, rates, *q, ci=ci, cj=cj)
+    assert np.array_equal(n_o, idx) and np.array_equal(n_p, idx)
+    assert np.array_equal(r_o, expect) and np.array_equal(r_p, expect)
+
+
+# --------------------------------------------------------------------------------------------
+# Surrogate MLP gradient (oracle/mlp_oracle.py) against central finite differences
+# --------------------------------------------------------------------------------------------
+def test_mlp_gradient_central_differences():
+    """The FP64 loss gradient of mlp_oracle.loss_grad (bf=False) against central finite
+    differences of its loss, on coordinates of every parameter block (W0..W3, b0..b3, w4, b4).
+    Relative error bar 1e-7 of the block's gradient scale.  A dropped dsilu factor, a
+    transposed weight in the backward product or a flipped sign in one layer fails it (a
+    planted sign flip in W2 is checked to fail below)."""
+    from oracle import mlp_oracle as M
+
+    rng = np.random.default_rng(7)
+    p = M.init_params(3)
+    p += rng.normal(0, 0.05, p.size)          # biases and the padded inputs non-zero as well
+    x = rng.normal(0, 1, (24, M.IN))
+    t = rng.normal(0, 1, 24)
+    loss, g = M.loss_grad(p, x, t, bf=False)
+    assert np.isfinite(loss)
+
+    def fd(i, eps):
+        q = p.copy()
+        q[i] += eps
+        lp = M.loss_grad(q, x, t, bf=False)[0]
+        q[i] -= 2 * eps
+        lm = M.loss_grad(q, x, t, bf=False)[0]
+        return (lp - lm) / (2 * eps)
+
+    checked = 0
+    planted = g.copy()
+    for name, off, shape in M.LAYOUT:
+        size = int(np.prod(shape))
+        if name == "W2":
+            planted[off:off + size] *= -1.0
+        block = g[off:off + size]
+        scale = float(np.max(np.abs(block))) or 1.0
+        idx = rng.choice(size, min(size, 10), replace=False)
+        if name == "W0":
+            idx = np.concatenate([idx, [13 * M.WD + 5]])   # a padded input row too
+        for i in idx:
+            num = fd(off + int(i), 1e-5)
+            assert abs(num - g[off + i]) <= 1e-7 * scale, (name, int(i), num, g[off + i])
+            checked += 1
+    assert checked >= 90
+    # the planted mistake is caught by the same comparison
+    _, off, shape = next(L for L in M.LAYOUT if L[0] == "W2")
+    big = off + int(np.argmax(np.abs(g[off:off + int(np.prod(shape))])))
+    assert abs(fd(big, 1e-5) - planted[big]) > 1e-7 * abs(g[big])
