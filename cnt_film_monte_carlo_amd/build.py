@@ -1,7 +1,10 @@
 """Build libcntkmc.so (HIP, gfx950) in-tree with hipcc.
 
 Every HIP source under csrc/ is compiled to an object in _build/ and linked into
-cnt_film_monte_carlo_amd/libcntkmc.so, the C-ABI library declared in include/cntkmc.h.
+cnt_film_monte_carlo_amd/libcntkmc.so, the C-ABI library declared in include/cntkmc.h.  Then the
+C++ consumer of that ABI, host/kubo_main.cpp (the reference driver's Kubo loop over the
+monte_carlo host layer of host/monte_carlo.hpp), is compiled and linked against it into
+host/kubo_main.
 """
 from __future__ import annotations
 
@@ -17,6 +20,8 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "libcntkmc.so"
+HOST = PKG / "host"
+KUBO_MAIN = HOST / "kubo_main"
 
 ARCH = os.environ.get("CNTKMC_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
@@ -61,9 +66,24 @@ def build(verbose: bool = True) -> Path:
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stdout}\n{res.stderr}")
     os.replace(tmp, LIB)
+    build_host()
     if verbose:
-        print(f"built {LIB} from {len(srcs)} sources", file=sys.stderr)
+        print(f"built {LIB} from {len(srcs)} sources and {KUBO_MAIN.name}", file=sys.stderr)
     return LIB
+
+
+def build_host() -> Path:
+    """host/kubo_main: a C++ program over include/cntkmc.h, linked against libcntkmc.so (found
+    next to it through its rpath)."""
+    tmp = KUBO_MAIN.with_name(KUBO_MAIN.name + ".tmp")
+    cmd = [HIPCC, "-O2", "-std=c++17", "-Wall", f"-I{ROOT / 'include'}", f"-I{HOST}",
+           str(HOST / "kubo_main.cpp"), f"-L{PKG}", "-lcntkmc", "-Wl,-rpath,$ORIGIN/..",
+           "-o", str(tmp)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"kubo_main failed to build:\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, KUBO_MAIN)
+    return KUBO_MAIN
 
 
 if __name__ == "__main__":

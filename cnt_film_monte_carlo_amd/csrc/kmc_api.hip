@@ -1058,9 +1058,12 @@ int kmc_set_trace(kmc_ctx* c, int64_t n_trace, int64_t max_hops) {
 // the 64-bit totals before a launch that could take `pending` past UINT32_MAX.
 constexpr uint64_t COUNT_COPY_MAX = 0xFFFFFFFFull;
 
-// Sub-launches between two reads of the device's live-exciton count (a host synchronisation):
-// a run stops early once no exciton can hop (all past t_max or trapped).
+// Reads of the device's live-exciton count (each a host synchronisation) after launches 1, 2,
+// 4 and every 8th: a run stops early once no exciton can hop (all past t_max or trapped).
 constexpr int64_t LIVE_CHECK_EVERY = 8;
+static inline bool live_check_due(int64_t launches) {
+  return (launches & (launches - 1)) == 0 || launches % LIVE_CHECK_EVERY == 0;
+}
 
 static int fold_counts(kmc_ctx* c, hipStream_t st) {
   KMC_HIP(c, launch_fold_counts(c->n_sites, c->count_stride, c->d_site_count.p,
@@ -1110,7 +1113,7 @@ static int run_hops(kmc_ctx* c, hop_args a, int64_t hops, int64_t chunk, bool co
     KMC_HIP(c, launch_hop(a, count, sample, trace, fly, c->stream));
     done += sub;
     ++launches;
-    if (done < hops && launches % LIVE_CHECK_EVERY == 0) {
+    if (done < hops && live_check_due(launches)) {
       uint32_t live = 0;
       KMC_TRY(copy_out(c, &live, c->d_live.p, sizeof(uint32_t)));
       if (live == 0u) break;
@@ -1421,7 +1424,7 @@ int ens_run_hops(kmc_ensemble* m, int64_t hops, int64_t chunk, bool count, bool 
     ENS_HIP(launch_hop_ens(a, count, sample, m->stream));
     done += sub;
     ++launches;
-    if (done < hops && launches % LIVE_CHECK_EVERY == 0) {
+    if (done < hops && live_check_due(launches)) {
       uint32_t live = 0;
       ENS_HIP(hipMemcpyAsync(&live, m->d_live, sizeof(uint32_t), hipMemcpyDeviceToHost,
                              m->stream));

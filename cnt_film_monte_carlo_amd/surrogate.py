@@ -256,6 +256,36 @@ class RateSurrogate:
         return s
 
 
+def film_tables(chiralities, temperature: float, surrogate: RateSurrogate | None = None,
+                device: int = 0, length_cells: int = 10):
+    """The n^2 donor/acceptor rate tables of a film's chiralities on the default grid
+    (create_davoody_scatt_table, src/monte_carlo/monte_carlo.cpp:64-154), built by the
+    surrogate with the exact first-order rate recomputing the flagged nodes.  Returns
+    (theta, z, a1, a2, rates [n^2][n_theta][n_z][n_a1][n_a2], nodes recomputed exactly)."""
+    sur = surrogate or RateSurrogate.load(device=device)
+    th, z, a1, a2 = kmc.default_grids()
+    models = {c: exciton.CntModel(*c, length_cells, device=device) for c in chiralities}
+    tabs, n_exact = [], 0
+    try:
+        for ci in chiralities:
+            for cj in chiralities:
+                tr = exciton.Transfer(models[ci], models[cj], temperature, device=device)
+                try:
+                    tr.set_clamp(sur.domain.box_lo, sur.domain.box_hi)
+                    tab, nfb, _ = sur.table(CHIRALITIES.index(ci), CHIRALITIES.index(cj),
+                                            temperature, tr, th, z, a1, a2)
+                finally:
+                    tr.close()
+                tabs.append(tab)
+                n_exact += nfb
+    finally:
+        for m in models.values():
+            m.close()
+        if surrogate is None:
+            sur.close()
+    return th, z, a1, a2, np.stack(tabs), n_exact
+
+
 # ---------------------------------------------------------------------------------------------
 # training data
 # ---------------------------------------------------------------------------------------------

@@ -128,3 +128,18 @@ def test_estimate_diffusion_skips_incomplete_bins(kmc):
     red[7:, 1, 1] = 1e9
     est, _ = kmc.estimate_diffusion(red, tmax)
     assert est[0] == pytest.approx(0.25, rel=1e-12)
+
+
+def test_host_driver_is_built_and_reports_status():
+    """host/kubo_main (the C++ consumer of the ABI) is built next to the library; without a
+    GPU it reports kmc_create's status instead of crashing."""
+    import subprocess
+    exe = ROOT / "cnt_film_monte_carlo_amd" / "host" / "kubo_main"
+    assert exe.exists()
+    res = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert res.returncode == 2 and "usage" in res.stderr
+    res = subprocess.run([str(exe), "/nonexistent-dir", "10", "1", "1e-12"], capture_output=True,
+                         text=True, timeout=120)
+    if res.returncode == 0 or "no HIP device" not in res.stderr:
+        pytest.skip("a HIP device is present")
+    assert res.returncode == 1 and "status -5" in res.stderr
