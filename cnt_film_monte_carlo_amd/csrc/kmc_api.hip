@@ -19,12 +19,41 @@
 #include "kmc_mlp.h"
 
 #include <hipcub/device/device_radix_sort.hpp>
+#include <mutex>
 #include "philox.h"
 
 using namespace cntkmc;
 
+// Scratch of the allocation-free MSD reduction (kmc_reduce_samples_noalloc): the partial sums
+// of one call, n_bins * n_batches * 5 doubles at most.  Every device holds its own copy until
+// the process ends; host code uses only the address hipGetSymbolAddress returns for the
+// device (the array's name is a host-side stand-in, not a device address).
+constexpr size_t REDUCE_SCRATCH_DOUBLES = 32768;  // 256 KB per device
+static __device__ __align__(256) double scratch[REDUCE_SCRATCH_DOUBLES];
+
 namespace {
 thread_local std::string g_last_error;
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+
+// Once per device, with that device current (kmc_create): the device address of `scratch`.
+hipError_t scratch_setup(int dev) {
+  if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) return hipSuccess;  // the no-alloc entry refuses
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return hipSuccess;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e == hipSuccess) g_scratch_ptr[dev] = static_cast<double*>(p);
+  return e;
+}
+
+double* scratch_of(int dev) {
+  if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_ptr[dev];
+}
 }  // namespace
 
 namespace cntkmc {
@@ -424,6 +453,7 @@ int kmc_create(const kmc_config* cfg, kmc_ctx** out) {
   kmc_ctx* c = new kmc_ctx();
   c->cfg = *cfg;
   hipError_t e = hipSetDevice(cfg->device);
+  if (e == hipSuccess) e = scratch_setup(cfg->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&c->ev0);
   if (e == hipSuccess) e = hipEventCreate(&c->ev1);
@@ -1289,6 +1319,45 @@ int kmc_reduce_samples(kmc_ctx* c, int32_t n_batches, double* out, double* kerne
   }
   return KMC_OK;
 }
+
+int kmc_reduce_samples_noalloc(kmc_ctx* c, int32_t n_batches, double* out, double* kernel_ms) {
+  // kmc_reduce_samples with the partial sums in the device's static scratch: no allocation
+  if (!c || !out) return fail(c, KMC_ERR_ARG, "kmc_reduce_samples_noalloc: null argument");
+  if (c->n_bins == 0) return fail(c, KMC_ERR_STATE, "kmc_reduce_samples_noalloc: no sampling grid");
+  if (n_batches < 1 || n_batches > c->n_exc || n_batches > 65535)
+    return fail(c, KMC_ERR_ARG, "kmc_reduce_samples_noalloc: need 1 <= n_batches <= min(E, 65535)");
+  const size_t nout = (size_t)c->n_bins * n_batches * 5;
+  if (nout > REDUCE_SCRATCH_DOUBLES)
+    return fail(c, KMC_ERR_CAPACITY,
+                "kmc_reduce_samples_noalloc: n_bins * n_batches * 5 = " + std::to_string(nout) +
+                    " exceeds the static scratch of " + std::to_string(REDUCE_SCRATCH_DOUBLES) +
+                    " doubles (use kmc_reduce_samples)");
+  double* ptr = scratch_of(c->cfg.device);
+  if (!ptr)
+    return fail(c, KMC_ERR_STATE, "kmc_reduce_samples_noalloc: no static scratch on this device");
+  KMC_HIP(c, hipSetDevice(c->cfg.device));
+  reduce_args a{};
+  a.n_exc = c->n_exc;
+  a.n_bins = c->n_bins;
+  a.n_batches = n_batches;
+  a.k_rec = c->x_krec.p;
+  a.rec_dx = c->r_dx.p;
+  a.rec_dy = c->r_dy.p;
+  a.rec_dz = c->r_dz.p;
+  a.rec_drain = c->r_drain.p;
+  a.out = ptr;  // every one of the nout partials is written by the kernel: nothing to clear
+  KMC_HIP(c, hipEventRecord(c->ev0, c->stream));
+  KMC_HIP(c, launch_reduce(a, c->stream));
+  KMC_HIP(c, hipEventRecord(c->ev1, c->stream));
+  KMC_TRY(copy_out(c, out, ptr, nout * sizeof(double)));
+  if (kernel_ms) {
+    float ms = 0.f;
+    KMC_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    *kernel_ms = ms;
+  }
+  return KMC_OK;
+}
+
 
 }  // extern "C"
 

@@ -94,6 +94,7 @@ _SIGS = {
     "kmc_get_samples": (C.c_int, [_P, _D, _U32, _U32]),
     "kmc_get_trace": (C.c_int, [_P, _I32, _D, _I64]),
     "kmc_reduce_samples": (C.c_int, [_P, C.c_int32, _D, _D]),
+    "kmc_reduce_samples_noalloc": (C.c_int, [_P, C.c_int32, _D, _D]),
     "kmc_estimate_diffusion": (C.c_int, [_D, C.c_int32, C.c_int32, C.c_double, C.c_int32,
                                          C.c_int32, _D, _D]),
     "kmc_generate_network": (C.c_int, [C.POINTER(_NetParams), C.c_int64, _D, _D, _D, _D, _D, _D,
@@ -504,6 +505,15 @@ class Context:
         out = np.zeros((self.n_bins, n_batches, 5))
         ms = C.c_double(0.0)
         self._ck(self.lib.kmc_reduce_samples(self.h, int(n_batches), _ptr(out, _D), C.byref(ms)))
+        return out, ms.value
+
+    def reduce_samples_noalloc(self, n_batches: int):
+        """reduce_samples through the allocation-free entry point (a static device scratch of
+        32768 doubles: n_bins * n_batches * 5 must fit)."""
+        out = np.zeros((self.n_bins, n_batches, 5))
+        ms = C.c_double(0.0)
+        self._ck(self.lib.kmc_reduce_samples_noalloc(self.h, int(n_batches), _ptr(out, _D),
+                                                     C.byref(ms)))
         return out, ms.value
 
 

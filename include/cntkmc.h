@@ -261,6 +261,13 @@ int kmc_get_trace(kmc_ctx* ctx, int32_t* sites, double* times, int64_t* n_hops);
  * kubo_save_avg_dispalcement_squared (monte_carlo.cpp:382-409) and save_currents
  * (src/monte_carlo/monte_carlo.h:584-643). */
 int kmc_reduce_samples(kmc_ctx* ctx, int32_t n_batches, double* out, double* kernel_ms);
+/* kmc_reduce_samples without any device allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The partial sums go to a static 256 KB device array per
+ * GPU (held until the process ends), so n_bins * n_batches * 5 must not exceed 32768
+ * (KMC_ERR_CAPACITY otherwise).  Calls on one GPU share that array: calls that overlap on one
+ * GPU (from several threads or contexts) race.  Same output as kmc_reduce_samples, bit for bit. */
+int kmc_reduce_samples_noalloc(kmc_ctx* ctx, int32_t n_batches, double* out, double* kernel_ms);
 
 /* Diffusion coefficient per axis from reduced samples (host arithmetic, no device work).
  * Least-squares slope through the origin of MSD(t) over bins k in [k_lo, k_hi) with a full
