@@ -368,8 +368,9 @@ int pack_rows(kmc_ctx* c) {
   }
   int64_t lines = 0;
   KMC_TRY(copy_out(c, &lines, c->d_roff.p + n, sizeof(int64_t)));
-  if (lines > (int64_t)TGT_LINE_MASK)
-    return fail(c, KMC_ERR_CAPACITY, "pack_rows: CSR too large for 30-bit block offsets");
+  // the wave loop addresses the blocks with 32-bit byte offsets: at most 2^25 lines (4 GiB)
+  if (lines > ((int64_t)1 << 25))
+    return fail(c, KMC_ERR_CAPACITY, "pack_rows: row blocks beyond 4 GiB (2^25 lines of 128 B)");
   c->n_units = lines * ROW_UNITS;
   KMC_TRY(dalloc(c, c->d_blocks, c->n_units));
   KMC_HIP(c, hipMemsetAsync(c->d_blocks.p, 0, c->n_units * sizeof(uint4), c->stream));

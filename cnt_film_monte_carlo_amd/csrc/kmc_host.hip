@@ -171,6 +171,17 @@ int kmc_table_rate(int32_t n_chir, int32_t n_theta, const double* theta, int32_t
   return KMC_OK;
 }
 
+int kmc_ff_time(int64_t n, const double* gamma, const uint32_t* r, double* out) {
+#pragma clang fp contract(off)
+  // ff_time (src/monte_carlo/scatterer.h:74-80) with the hop kernels' arithmetic
+  if (n < 0 || (n > 0 && (!gamma || !r || !out))) return KMC_ERR_ARG;
+  for (int64_t k = 0; k < n; ++k) {
+    if (!(gamma[k] > 0.0) || r[k] == 0u || r[k] > 0x7FFFFFFFu) return KMC_ERR_ARG;
+    out[k] = -recip(gamma[k]) * log_unit(div_rand_max((double)r[k]));
+  }
+  return KMC_OK;
+}
+
 int kmc_generate_network(const kmc_netgen_params* p, int64_t capacity, double* x, double* y,
                          double* z, double* ox, double* oy, double* oz, int32_t* left,
                          int32_t* right, uint8_t* chirality, int32_t* tube_id, int64_t* n_out) {

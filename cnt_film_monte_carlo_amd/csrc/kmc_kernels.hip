@@ -295,8 +295,7 @@ __device__ __forceinline__ bool trapped(uint32_t n, double gamma) { return n == 
 __device__ __forceinline__ double dwell_time(uint32_t n, double gamma, uint32_t r) {
 #pragma clang fp contract(off)
   if (trapped(n, gamma)) return INFINITY;
-  const double inv = 1.0 / gamma;
-  return -inv * log(div_rand_max((double)r));
+  return -recip(gamma) * log_unit(div_rand_max((double)r));
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -304,7 +303,9 @@ __device__ __forceinline__ double dwell_time(uint32_t n, double gamma, uint32_t 
 // line), so a hop with at most ROW_INLINE neighbours touches exactly one line.  First line, in
 // 16-byte units:
 //   u0      n | site | Gamma (FP64)
-//   u1..u4  cum[0..8)   FP64 prefix sums (unused slots past n are zero)
+//   u1..u4  cum[0..8)   FP64 prefix sums; slots n-1..7 hold +inf (the last entry is Gamma,
+//                       which the header carries: a count of the slots <= dice is then the
+//                       selected index without a comparison against n)
 //   u5, u6  tgt[0..8)   row word of the target: first line of its block | TGT_WIDE_BIT when
 //                       that block has continuation lines | EDGE_DRAIN_BIT for a drain target
 //   u7      x, y, z     FP32 position of the site | 0
@@ -405,12 +406,13 @@ __global__ __launch_bounds__(256) void k_pack_rows(int64_t n, const site_hdr* __
   const uint32_t m = h.n > ROW_INLINE ? h.n - ROW_INLINE : 0u;
   double* ce = reinterpret_cast<double*>(b + ROW_UNITS);
   uint32_t* te = reinterpret_cast<uint32_t*>(ce + m);
+  for (uint32_t k = (h.n > 0u ? h.n - 1u : 0u); k < ROW_INLINE; ++k) c[k] = INFINITY;
   for (uint32_t k = 0; k < h.n; ++k) {
     const edge_rec e = edge[h.rs + k];
     const uint32_t j = e.col & EDGE_COL_MASK;
     const uint32_t tg = row_word((uint32_t)roff[j], hdr[j].n) | (e.col & EDGE_DRAIN_BIT);
     if (k < ROW_INLINE) {
-      c[k] = cum[h.rs + k];
+      if (k + 1u < h.n) c[k] = cum[h.rs + k];
       t[k] = tg;
     } else {
       ce[k - ROW_INLINE] = cum[h.rs + k];
@@ -479,7 +481,19 @@ __device__ __forceinline__ row_regs load_row(const uint4* __restrict__ blocks, u
 typedef __attribute__((address_space(3))) void lds_void_t;
 typedef __attribute__((address_space(1))) void global_void_t;
 
-__device__ __forceinline__ row_regs coop_load_row(const uint4* __restrict__ blocks, uint32_t W,
+// The row as the wave loop holds it: header, the eight inline cumulative rates and the site's
+// position.  Its targets stay in the wave's LDS copy of the line and are read by index
+// (target_lds), so they take no registers.
+struct coop_row {
+  uint32_t n, site;
+  double gamma;
+  double c[8];
+  float x, y, z;
+};
+
+// Byte offsets into the row blocks are 32-bit (pack_rows refuses blocks of 4 GiB or more), so
+// each load takes the blocks' base from scalar registers and one 32-bit offset per lane.
+__device__ __forceinline__ coop_row coop_load_row(const uint4* __restrict__ blocks, uint32_t W,
                                                   uint4* lds_wave, uint32_t lane) {
   const uint32_t line = W & TGT_LINE_MASK;
   const uint32_t unit = ((lane & 7u) - (lane >> 3)) & 7u;
@@ -489,33 +503,58 @@ __device__ __forceinline__ row_regs coop_load_row(const uint4* __restrict__ bloc
   for (uint32_t k = 0; k < 8; ++k) L[k] = (uint32_t)__shfl((int)line, (int)(8u * k + (lane >> 3)), 64);
   // earlier reads of this LDS region are complete before it is overwritten
   asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+  const char* base = reinterpret_cast<const char*>(blocks);
 #pragma unroll
   for (uint32_t k = 0; k < 8; ++k) {
-    const uint4* src = blocks + (size_t)ROW_UNITS * L[k] + unit;
-    __builtin_amdgcn_global_load_lds((global_void_t*)src, (lds_void_t*)(lds_wave + 64u * k), 16, 0,
-                                     0);
+    const uint32_t off = L[k] * (uint32_t)(16u * ROW_UNITS) + unit * 16u;
+    __builtin_amdgcn_global_load_lds((global_void_t*)(base + off), (lds_void_t*)(lds_wave + 64u * k),
+                                     16, 0, 0);
   }
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   const uint4* my = lds_wave + 8u * lane;
-  return row_from_units(my[(0u + lane) & 7u], my[(1u + lane) & 7u], my[(2u + lane) & 7u],
-                        my[(3u + lane) & 7u], my[(4u + lane) & 7u], my[(5u + lane) & 7u],
-                        my[(6u + lane) & 7u], my[(7u + lane) & 7u]);
+  const uint4 h = my[lane & 7u], q0 = my[(1u + lane) & 7u], q1 = my[(2u + lane) & 7u],
+              q2 = my[(3u + lane) & 7u], q3 = my[(4u + lane) & 7u], p = my[(7u + lane) & 7u];
+  coop_row r;
+  r.n = h.x;
+  r.site = h.y;
+  r.gamma = __hiloint2double((int)h.w, (int)h.z);
+  r.c[0] = __hiloint2double((int)q0.y, (int)q0.x);
+  r.c[1] = __hiloint2double((int)q0.w, (int)q0.z);
+  r.c[2] = __hiloint2double((int)q1.y, (int)q1.x);
+  r.c[3] = __hiloint2double((int)q1.w, (int)q1.z);
+  r.c[4] = __hiloint2double((int)q2.y, (int)q2.x);
+  r.c[5] = __hiloint2double((int)q2.w, (int)q2.z);
+  r.c[6] = __hiloint2double((int)q3.y, (int)q3.x);
+  r.c[7] = __hiloint2double((int)q3.w, (int)q3.z);
+  r.x = __uint_as_float(p.x);
+  r.y = __uint_as_float(p.y);
+  r.z = __uint_as_float(p.z);
+  return r;
+}
+
+// Target word idx < ROW_INLINE of the lane's row from the wave's LDS copy (units 5 and 6,
+// rotated by the lane as coop_load_row stores them).  Valid until the next coop_load_row.
+__device__ __forceinline__ uint32_t target_lds(const uint4* lds_wave, uint32_t lane, uint32_t idx) {
+  const uint32_t slot = (5u + (idx >> 2) + lane) & 7u;
+  return reinterpret_cast<const uint32_t*>(lds_wave + 8u * lane + slot)[idx & 3u];
 }
 
 // Index picked by update_state's binary search (scatterer.cpp:17-30): the first entry whose
 // cumulative rate exceeds dice, or n-1.  Rows are non-decreasing (rates >= 0 are enforced when
 // the CSR is built), so that index equals the number of leading entries k < n-1 with
-// cum[k] <= dice, counted here without a dependent load chain.
+// cum[k] <= dice.  The inline slots n-1..7 hold +inf, so the count over the eight inline slots
+// is that number for n <= 9 without any comparison against n; a longer row continues on its
+// next line.
 __device__ __forceinline__ uint32_t select_entry(const uint4* __restrict__ blocks, uint32_t W,
-                                                 const row_regs& r, double dice) {
-  const uint32_t lim = r.n - 1u;
+                                                 const double* c8, uint32_t n, double dice) {
   uint32_t idx = 0;
 #pragma unroll
-  for (uint32_t q = 0; q < 8; ++q) idx += (q < lim && r.c[q] <= dice) ? 1u : 0u;
-  if (idx == 8u && lim > 8u) {
-    // cum[8..16): the first 64 bytes of the continuation line, which a row with lim > 8 always
+  for (uint32_t q = 0; q < 8; ++q) idx += (c8[q] <= dice) ? 1u : 0u;
+  if (idx == 8u && n > 9u) {
+    // cum[8..16): the first 64 bytes of the continuation line, which a row with n > 9 always
     // owns, read in one batch of four 16-byte loads before any comparison; then further entries
     // one at a time
+    const uint32_t lim = n - 1u;
     const uint4* cu = blocks + (size_t)ROW_UNITS * (W & TGT_LINE_MASK) + ROW_UNITS;
     const uint4 e0 = cu[0], e1 = cu[1], e2 = cu[2], e3 = cu[3];
     const double c[8] = {__hiloint2double((int)e0.y, (int)e0.x), __hiloint2double((int)e0.w, (int)e0.z),
@@ -534,15 +573,19 @@ __device__ __forceinline__ uint32_t select_entry(const uint4* __restrict__ block
   return idx;
 }
 
+// Target word of a continuation entry idx >= ROW_INLINE of a row with n entries.
+__device__ __forceinline__ uint32_t target_wide(const uint4* __restrict__ blocks, uint32_t W,
+                                                uint32_t n, uint32_t idx) {
+  const uint32_t m = n - ROW_INLINE;
+  const double* ce =
+      reinterpret_cast<const double*>(blocks + (size_t)ROW_UNITS * (W & TGT_LINE_MASK) + ROW_UNITS);
+  return reinterpret_cast<const uint32_t*>(ce + m)[idx - ROW_INLINE];
+}
+
 // Target word of entry idx: from the registers for idx < 8, else from the continuation lines.
 __device__ __forceinline__ uint32_t target_of(const uint4* __restrict__ blocks, uint32_t W,
                                               const row_regs& r, uint32_t idx) {
-  if (idx >= ROW_INLINE) {
-    const uint32_t m = r.n - ROW_INLINE;
-    const double* ce = reinterpret_cast<const double*>(blocks + (size_t)ROW_UNITS * (W & TGT_LINE_MASK) +
-                                                       ROW_UNITS);
-    return reinterpret_cast<const uint32_t*>(ce + m)[idx - ROW_INLINE];
-  }
+  if (idx >= ROW_INLINE) return target_wide(blocks, W, r.n, idx);
   const uint4 q = idx < 4u ? r.t0 : r.t1;
   const uint32_t k = idx & 3u;
   return k == 0u ? q.x : k == 1u ? q.y : k == 2u ? q.z : q.w;
@@ -781,6 +824,7 @@ __device__ __forceinline__ bool hop_exciton(const net_view& v, uint32_t e, uint6
   const uint64_t h_end = h + hop_budget;
   const size_t E = v.x.n;
   count_cache cc;
+  hop_words hw;
   uint32_t* cbase = COUNT ? v.site_count + (size_t)xcd_id() * v.count_stride : nullptr;
 
   while (true) {
@@ -835,6 +879,7 @@ __device__ __forceinline__ bool hop_exciton(const net_view& v, uint32_t e, uint6
         tl = t;
         t = INFINITY;
         ++h;
+        hw.have = false;
         continue;
       }
     } else if (h >= h_end || trapped(row.n, row.gamma) || !(t <= t_max)) {
@@ -842,9 +887,10 @@ __device__ __forceinline__ bool hop_exciton(const net_view& v, uint32_t e, uint6
     }
 
     // event selection (scatterer::update_state, scatterer.cpp:17-30)
-    const u32x4 w = draw4(v.seed, e, h, STREAM_HOP);
-    const double dice = div_rand_max(row.gamma * (double)r31(w.v[0]));
-    const uint32_t idx = select_entry(v.blocks, R, row, dice);
+    uint32_t w_sel, w_dw;
+    hw.get(v.seed, e, h, w_sel, w_dw);
+    const double dice = div_rand_max(row.gamma * (double)r31(w_sel));
+    const uint32_t idx = select_entry(v.blocks, R, row.c, row.n, dice);
     const uint32_t tg = target_of(v.blocks, R, row, idx);
     if (COUNT) cc.add(count_slot(row.site, v.count_stride), cbase);
     const float sx = row.x, sy = row.y, sz = row.z;
@@ -857,13 +903,13 @@ __device__ __forceinline__ bool hop_exciton(const net_view& v, uint32_t e, uint6
     dy = dy + min_image((double)row.y - (double)sy, v.per_len[1], v.half_per[1]);
     dz = dz + min_image((double)row.z - (double)sz, v.per_len[2], v.half_per[2]);
     hs = (hs ^ (uint64_t)row.site) * FNV_PRIME;
-    const uint32_t r = nonzero_draw(w.v[1], w.v[3], v.seed, e, h, STREAM_HOP_REDRAW);
+    const uint32_t r = nonzero_draw1(w_dw, v.seed, e, h, STREAM_HOP_REDRAW);
     t = t + dwell_time(row.n, row.gamma, r);
     // drain: count current and re-inject, keeping displacement and dwell (kubo_step,
     // src/monte_carlo/monte_carlo.cpp:330-336)
     if (tg & EDGE_DRAIN_BIT) {
       ++nd;
-      R = v.src_roff[r31(w.v[2]) % v.n_src];
+      R = v.src_roff[r31(draw4(v.seed, e, h, STREAM_REINJECT).v[0]) % v.n_src];
       row = load_row(v.blocks, R);
       hs = (hs ^ ((uint64_t)row.site | (1ull << 32))) * FNV_PRIME;
     }
@@ -935,14 +981,18 @@ __device__ __forceinline__ void hop_wave(const net_view& v, uint32_t e, bool val
   }
   const uint64_t h_end = h + hop_budget;
   const size_t E = v.x.n;
+  // the next MSD record falls due once the clock passes t_rec = (k + 1) dt_sample
+  double t_rec = INFINITY;
+  if (SAMPLE && k < (uint32_t)v.n_bins) t_rec = (double)(k + 1u) * v.dt_sample;
   bool active = valid;
   bool pend = false;           // a hop waits for its target row
   uint32_t r_dw = 0;           // its dwell draw
   float px = 0.f, py = 0.f, pz = 0.f;  // position of the site it left
   double t_hop = 0.0;
+  hop_words hw;
   count_cache cc;
   uint32_t* cbase = COUNT ? v.site_count + (size_t)xcd_id() * v.count_stride : nullptr;
-  row_regs row;
+  coop_row row;
 
   while (true) {
     row = coop_load_row(v.blocks, W, lds_wave, lane);
@@ -961,31 +1011,32 @@ __device__ __forceinline__ void hop_wave(const net_view& v, uint32_t e, bool val
       pend = false;
     }
     if (SAMPLE && active) {
-      while (k < (uint32_t)v.n_bins) {
-        const double tk = (double)(k + 1u) * v.dt_sample;
-        if (!(tk < t)) break;
+      while (t_rec < t) {
         const size_t r = (size_t)k * E + e;
         v.rec_dx[r] = dx;
         v.rec_dy[r] = dy;
         v.rec_dz[r] = dz;
         v.rec_drain[r] = nd;
         ++k;
+        t_rec = k < (uint32_t)v.n_bins ? (double)(k + 1u) * v.dt_sample : INFINITY;
       }
     }
     active = active && !(h >= h_end || trapped(row.n, row.gamma) || !(t <= t_max));
     if (__ballot(active) == 0) break;
     if (active) {
       // event selection (scatterer::update_state, scatterer.cpp:17-30)
-      const u32x4 w = draw4(v.seed, e, h, STREAM_HOP);
-      const double dice = div_rand_max(row.gamma * (double)r31(w.v[0]));
-      const uint32_t idx = select_entry(v.blocks, W, row, dice);
-      const uint32_t tg = target_of(v.blocks, W, row, idx);
+      uint32_t w_sel, w_dw;
+      hw.get(v.seed, e, h, w_sel, w_dw);
+      const double dice = div_rand_max(row.gamma * (double)r31(w_sel));
+      const uint32_t idx = select_entry(v.blocks, W, row.c, row.n, dice);
+      const uint32_t tg = idx < ROW_INLINE ? target_lds(lds_wave, lane, idx)
+                                           : target_wide(v.blocks, W, row.n, idx);
       if (COUNT) {
         const uint32_t off = (W & TGT_LINE_MASK) - win_base;
         if (off < COUNT_WINDOW) atomicAdd(win + off, 1u);
         else cc.add(count_slot(row.site, v.count_stride), cbase);
       }
-      const uint32_t r = nonzero_draw(w.v[1], w.v[3], v.seed, e, h, STREAM_HOP_REDRAW);
+      const uint32_t r = nonzero_draw1(w_dw, v.seed, e, h, STREAM_HOP_REDRAW);
       t_hop = t;
       if (tg & EDGE_DRAIN_BIT) {
         // drain: the dwell drawn at the drain site, then re-injection keeping displacement and
@@ -998,7 +1049,7 @@ __device__ __forceinline__ void hop_wave(const net_view& v, uint32_t e, bool val
         hs = (hs ^ (uint64_t)hd.y) * FNV_PRIME;
         t = t + dwell_time(hd.x, __hiloint2double((int)hd.w, (int)hd.z), r);
         ++nd;
-        W = v.src_roff[r31(w.v[2]) % v.n_src];
+        W = v.src_roff[r31(draw4(v.seed, e, h, STREAM_REINJECT).v[0]) % v.n_src];
         const uint32_t src_site = v.blocks[(size_t)ROW_UNITS * (W & TGT_LINE_MASK)].y;
         hs = (hs ^ ((uint64_t)src_site | (1ull << 32))) * FNV_PRIME;
         if (TRACE && e < tv.n_trace && tr < tv.tr_max) {
@@ -1345,6 +1396,7 @@ __global__ __launch_bounds__(256) void k_contact_fly(contact_args a) {
     flight_state g{R, row.site, row.x, row.y, row.z, a.s.s_off[e], (uint32_t)a.s.heading[e],
                    0.0, 0.0, 0.0, 0u, 0ull};
     uint64_t made = 0;
+    hop_words hw;
     while (t <= a.t_end) {
       if (!fly(a.fv, g, t - tl, id, h)) {
         atomicOr(a.err, CONTACT_ERR_FLY);
@@ -1365,12 +1417,13 @@ __global__ __launch_bounds__(256) void k_contact_fly(contact_args a) {
         atomicOr(a.err, CONTACT_ERR_HOPS);
         break;
       }
-      const u32x4 w = draw4(a.seed, id, h, STREAM_HOP);
-      const double dice = div_rand_max(row.gamma * (double)r31(w.v[0]));
-      const uint32_t idx = select_entry(a.blocks, R, row, dice);
+      uint32_t w_sel, w_dw;
+      hw.get(a.seed, id, h, w_sel, w_dw);
+      const double dice = div_rand_max(row.gamma * (double)r31(w_sel));
+      const uint32_t idx = select_entry(a.blocks, R, row.c, row.n, dice);
       R = target_of(a.blocks, R, row, idx) & ~EDGE_DRAIN_BIT;
       row = load_row(a.blocks, R);
-      const uint32_t r = nonzero_draw(w.v[1], w.v[3], a.seed, id, h, STREAM_HOP_REDRAW);
+      const uint32_t r = nonzero_draw1(w_dw, a.seed, id, h, STREAM_HOP_REDRAW);
       g = flight_state{R, row.site, row.x, row.y, row.z, 0.0, g.heading, 0.0, 0.0, 0.0, 0u, 0ull};
       tl = t;
       t = t + dwell_time(row.n, row.gamma, r);
@@ -1455,7 +1508,8 @@ __global__ __launch_bounds__(256, 4) void k_contact_wave(contact_args a) {
       W = row_word(L0, a.blocks[(size_t)ROW_UNITS * L0].x);
     }
     bool active = valid, pend = false;
-    row_regs row;
+    hop_words hw;
+    coop_row row;
     while (true) {
       row = coop_load_row(a.blocks, W, lds_wave, lane);
       if (pend) {
@@ -1472,11 +1526,14 @@ __global__ __launch_bounds__(256, 4) void k_contact_wave(contact_args a) {
       }
       if (__ballot(active) == 0) break;
       if (active) {
-        const u32x4 w = draw4(a.seed, id, h, STREAM_HOP);
-        const double dice = div_rand_max(row.gamma * (double)r31(w.v[0]));
-        const uint32_t idx = select_entry(a.blocks, W, row, dice);
-        W = target_of(a.blocks, W, row, idx) & ~EDGE_DRAIN_BIT;
-        r_dw = nonzero_draw(w.v[1], w.v[3], a.seed, id, h, STREAM_HOP_REDRAW);
+        uint32_t w_sel, w_dw;
+        hw.get(a.seed, id, h, w_sel, w_dw);
+        const double dice = div_rand_max(row.gamma * (double)r31(w_sel));
+        const uint32_t idx = select_entry(a.blocks, W, row.c, row.n, dice);
+        const uint32_t tg = idx < ROW_INLINE ? target_lds(lds_wave, lane, idx)
+                                             : target_wide(a.blocks, W, row.n, idx);
+        W = tg & ~EDGE_DRAIN_BIT;
+        r_dw = nonzero_draw1(w_dw, a.seed, id, h, STREAM_HOP_REDRAW);
         pend = true;
         ++h;
         ++made;

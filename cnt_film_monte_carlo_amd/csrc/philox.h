@@ -51,11 +51,13 @@ __host__ __device__ __forceinline__ u32x4 philox4x32_10(uint32_t c0, uint32_t c1
   return out;
 }
 
-// Draw streams.  Counter = {hop lo, hop hi, exciton, stream}; key = seed.
+// Draw streams.  Counter = {c lo, c hi, exciton, stream}; key = seed.
 enum : uint32_t {
-  STREAM_HOP = 0u,            // per hop: w0 selection, w1 dwell, w2 re-injection, w3 dwell spare
+  STREAM_HOP = 0u,            // counter h >> 1: hops 2P and 2P + 1 share one call, the even hop
+                              // taking words 0 (selection) and 1 (dwell), the odd hop 2 and 3
   STREAM_INIT = 1u,           // injection: w0 site, w1 dwell, w2 heading, w3 dwell spare
-  STREAM_HOP_REDRAW = 2u,     // + k: further dwell candidates after two zero words
+  STREAM_HOP_REDRAW = 2u,     // + k, counter h: further dwell candidates after a zero word
+  STREAM_REINJECT = 0x10000000u,  // counter h: w0 the source site after a drain at hop h
   STREAM_INIT_REDRAW = 0x80000000u,
   STREAM_FLY_DRAIN = 0x40000000u,  // + j: re-injection site after the j-th drain of one flight
   STREAM_CONTACT = 0x20000000u,    // contact mode, a new particle: w0 site, w1 dwell, w3 spare
@@ -90,5 +92,46 @@ __host__ __device__ __forceinline__ uint32_t nonzero_draw(uint32_t wa, uint32_t 
   }
   return 1u;
 }
+
+// First non-zero 31-bit draw from one dwell word: the word, then the redraw streams.
+__host__ __device__ __forceinline__ uint32_t nonzero_draw1(uint32_t w, uint64_t seed,
+                                                           uint32_t exciton, uint64_t hop,
+                                                           uint32_t redraw_stream) {
+  const uint32_t r = r31(w);
+  if (r != 0u) return r;
+  for (uint32_t k = 0; k < REDRAW_LIMIT; ++k) {
+    const u32x4 v = draw4(seed, exciton, hop, redraw_stream + k);
+    if (r31(v.v[0]) != 0u) return r31(v.v[0]);
+    if (r31(v.v[1]) != 0u) return r31(v.v[1]);
+    if (r31(v.v[2]) != 0u) return r31(v.v[2]);
+    if (r31(v.v[3]) != 0u) return r31(v.v[3]);
+  }
+  return 1u;
+}
+
+// The selection and dwell words of consecutive hops, one Philox call per pair of hops: the
+// words of the odd hop of a pair are kept from the call the even hop made.  `have` is cleared
+// when a hop index is passed over without drawing.
+struct hop_words {
+  uint32_t w2 = 0u, w3 = 0u;
+  bool have = false;  // w2, w3 are the words of hop h + 1 after the even hop h drew
+
+  __host__ __device__ __forceinline__ void get(uint64_t seed, uint32_t e, uint64_t h,
+                                               uint32_t& sel, uint32_t& dw) {
+    if ((h & 1u) != 0u && have) {
+      sel = w2;
+      dw = w3;
+      have = false;
+      return;
+    }
+    const u32x4 w = draw4(seed, e, h >> 1, STREAM_HOP);
+    const bool odd = (h & 1u) != 0u;
+    sel = odd ? w.v[2] : w.v[0];
+    dw = odd ? w.v[3] : w.v[1];
+    w2 = w.v[2];
+    w3 = w.v[3];
+    have = !odd;
+  }
+};
 
 }  // namespace cntkmc

@@ -102,6 +102,7 @@ _SIGS = {
     "kmc_forster_table": (C.c_int, [C.c_double, C.c_int32, _D, C.c_int32, _D, C.c_int32, _D,
                                     C.c_int32, _D, _D]),
     "kmc_linspace": (C.c_int, [C.c_double, C.c_double, C.c_int32, _D]),
+    "kmc_ff_time": (C.c_int, [C.c_int64, _D, _U32, _D]),
     "kmc_table_rate": (C.c_int, [C.c_int32, C.c_int32, _D, C.c_int32, _D, C.c_int32, _D,
                                  C.c_int32, _D, _D, C.c_int64, _U8, _U8, _D, _D, _D, _D, _D,
                                  _I32]),
@@ -696,6 +697,16 @@ def table_rate(theta, z_shift, a1, a2, rates, q_theta, q_z_shift, q_a1, q_a2, ci
                                     _ptr(r, _D), n, _ptr(c1, _U8), _ptr(c2, _U8),
                                     *[_ptr(v, _D) for v in q], _ptr(out, _D), _ptr(nodes, _I32)))
     return out, nodes
+
+
+def ff_time(gamma, r) -> np.ndarray:
+    """ff_time (src/monte_carlo/scatterer.h:74-80), -(1/gamma) log(r / RAND_MAX), with the hop
+    kernels' logarithm, evaluated on the host (arrays broadcast)."""
+    g, rr = np.broadcast_arrays(np.asarray(gamma, np.float64), np.asarray(r, np.uint32))
+    g, rr = _c(g, np.float64), _c(rr, np.uint32)
+    out = np.zeros(g.size)
+    _check(library().kmc_ff_time(g.size, _ptr(g, _D), _ptr(rr, _U32), _ptr(out, _D)))
+    return out.reshape(g.shape)
 
 
 def forster_table(gamma_0: float, theta, z_shift, a1, a2) -> np.ndarray:

@@ -104,6 +104,12 @@ int kmc_table_rate(int32_t n_chir, int32_t n_theta, const double* theta, int32_t
                    const double* q_axis_shift_1, const double* q_axis_shift_2, double* out,
                    int32_t* nodes);
 
+/* The dwell of ff_time (src/monte_carlo/scatterer.h:74-80), -(1/gamma) log(r / RAND_MAX), for n
+ * draws r in [1, 2^31 - 1] and positive total rates gamma, evaluated on the host with the hop
+ * kernels' own logarithm (an IEEE reciprocal where the device refines its hardware estimate).
+ * For inspection and tests of that arithmetic.  Host only. */
+int kmc_ff_time(int64_t n, const double* gamma, const uint32_t* r, double* out);
+
 /* Job (b): CSR out-rate table with FP64 cumulative sums.  Replaces scatterer::find_neighbors and
  * set_max_rate (src/monte_carlo/scatterer.cpp:34-83, src/monte_carlo/scatterer.h:89-93,
  * src/monte_carlo/monte_carlo.h:426-440).  Rows list neighbours in the reference order (stencil

@@ -10,11 +10,13 @@
  * reference lines it follows.  Build: oracle/Makefile (gcc -O2 -fopenmp -ffp-contract=off).
  *
  * Draw mapping shared with the product (the counter-based generator replaces std::rand()):
- *   Philox4x32-10, key = {seed lo, seed hi}, counter = {hop lo, hop hi, exciton, stream}.
- *   hop h    : stream 0 -> w0 selection, w1 dwell, w2 re-injection, w3 dwell spare
- *   injection: stream 1 -> w0 site, w1 dwell, w2 heading, w3 dwell spare
- *   a 31-bit draw is w >> 1; dwell draws of 0 move to the spare word, then to streams
- *   2+k (hops) or 0x80000000+k (injection), k < 16, words in order; after that r = 1.
+ *   Philox4x32-10, key = {seed lo, seed hi}, counter = {c lo, c hi, exciton, stream}.
+ *   hop h    : stream 0 at counter h >> 1 (one call for hops 2P and 2P + 1) -> words 2(h & 1)
+ *              selection and 2(h & 1) + 1 dwell; a dwell word of 0 moves to streams 2+k at
+ *              counter h; a drain at hop h re-injects on word 0 of stream 0x10000000, counter h
+ *   injection: stream 1 -> w0 site, w1 dwell, w2 heading, w3 dwell spare; a zero dwell moves
+ *              to the spare word, then to streams 0x80000000+k
+ *   a 31-bit draw is w >> 1; redraws take k < 16 streams, words in order; after that r = 1.
  *
  * Parity pins (see tests/): Philox against the published Random123 known-answer vectors;
  * geometry against closed forms; cell lists and neighbour sets against brute force; the
@@ -86,6 +88,35 @@ static uint32_t dwell_draw(uint64_t seed, uint32_t e, uint64_t hop, uint32_t w1,
       if ((w[i] >> 1) != 0u) return w[i] >> 1;
   }
   return 1u;
+}
+
+/* The selection and dwell words of hop h of exciton (or contact particle) e: hops 2P and 2P + 1
+ * share the Philox call of counter P = h >> 1 on stream 0, the even hop taking words 0 and 1,
+ * the odd hop words 2 and 3. */
+static void hop_words(uint64_t seed, uint32_t e, uint64_t h, uint32_t* sel, uint32_t* dw) {
+  uint32_t w[4];
+  draws(seed, e, h >> 1, 0u, w);
+  *sel = w[2 * (unsigned)(h & 1u)];
+  *dw = w[2 * (unsigned)(h & 1u) + 1];
+}
+
+/* ff_time's draw for the dwell word dw of hop h: zero moves to streams 2 + k at counter h */
+static uint32_t hop_dwell_draw(uint64_t seed, uint32_t e, uint64_t h, uint32_t dw) {
+  uint32_t k, i, w[4];
+  if ((dw >> 1) != 0u) return dw >> 1;
+  for (k = 0; k < 16u; ++k) {
+    draws(seed, e, h, 2u + k, w);
+    for (i = 0; i < 4; ++i)
+      if ((w[i] >> 1) != 0u) return w[i] >> 1;
+  }
+  return 1u;
+}
+
+/* the source-site word of a re-injection after a drain at hop h (stream 0x10000000) */
+static uint32_t reinject_word(uint64_t seed, uint32_t e, uint64_t h) {
+  uint32_t w[4];
+  draws(seed, e, h, 0x10000000u, w);
+  return w[0];
 }
 
 /* ------------------------------------------------------------------------------------------
@@ -776,7 +807,7 @@ static void orc_one(const orc_run_args* a, int64_t e, orc_cand* nb, double* rowb
     const double* row;
     const int32_t* rcol;
     int64_t n;
-    uint32_t w[4], r;
+    uint32_t w_sel, w_dw, r;
     unsigned idx;
     double dice;
     int32_t j;
@@ -860,8 +891,8 @@ static void orc_one(const orc_run_args* a, int64_t e, orc_cand* nb, double* rowb
       break;
     }
 
-    draws(a->seed, (uint32_t)e, h, 0u, w);
-    dice = row[n - 1] * (double)(w[0] >> 1) / ORC_RAND_MAX;
+    hop_words(a->seed, (uint32_t)e, h, &w_sel, &w_dw);
+    dice = row[n - 1] * (double)(w_sel >> 1) / ORC_RAND_MAX;
     idx = orc_select(row, (unsigned)n, dice);
     j = rcol[idx];
     if (a->site_count) {
@@ -872,7 +903,7 @@ static void orc_one(const orc_run_args* a, int64_t e, orc_cand* nb, double* rowb
     dy = dy + orc_min_image((double)a->pos[3 * j + 1] - (double)a->pos[3 * s + 1], a->per_len[1]);
     dz = dz + orc_min_image((double)a->pos[3 * j + 2] - (double)a->pos[3 * s + 2], a->per_len[2]);
     hs = (hs ^ (uint64_t)(uint32_t)j) * FNV_PR;
-    r = dwell_draw(a->seed, (uint32_t)e, h, w[1], w[3], 2u);
+    r = hop_dwell_draw(a->seed, (uint32_t)e, h, w_dw);
     t_hop = t;
     {
       double tj;
@@ -905,7 +936,7 @@ static void orc_one(const orc_run_args* a, int64_t e, orc_cand* nb, double* rowb
     drain = (a->flags[j] & 2u) != 0;
     if (drain) {
       ++nd;
-      s = a->src[(w[2] >> 1) % (uint32_t)a->n_src];
+      s = a->src[(reinject_word(a->seed, (uint32_t)e, h) >> 1) % (uint32_t)a->n_src];
       hs = (hs ^ ((uint64_t)(uint32_t)s | (1ull << 32))) * FNV_PR;
     }
     if (fly) {
@@ -1132,7 +1163,7 @@ int orc_contact(int64_t n_sites, const double* x, const double* y, const double*
       f.hs = 0;
       while (p->t_next <= t_end) {
         int64_t b, n;
-        uint32_t w[4], r;
+        uint32_t w_sel, w_dw, r;
         double dice;
         if (velocity > 0.0) {
           /* the flight up to the hop; the hop is from the site reached */
@@ -1149,10 +1180,10 @@ int orc_contact(int64_t n_sites, const double* x, const double* y, const double*
           }
           break;
         }
-        draws(seed, p->id, p->hops, 0u, w);
-        dice = (cum[b + n - 1] * (double)(w[0] >> 1)) / ORC_RAND_MAX;
+        hop_words(seed, p->id, p->hops, &w_sel, &w_dw);
+        dice = (cum[b + n - 1] * (double)(w_sel >> 1)) / ORC_RAND_MAX;
         p->site = col[b + orc_select(cum + b, (unsigned)n, dice)];
-        r = dwell_draw(seed, p->id, p->hops, w[1], w[3], 2u);
+        r = hop_dwell_draw(seed, p->id, p->hops, w_dw);
         if (velocity > 0.0) {
           f.s = p->site;
           f.off = 0.0;

@@ -143,3 +143,36 @@ def test_host_driver_is_built_and_reports_status():
     if res.returncode == 0 or "no HIP device" not in res.stderr:
         pytest.skip("a HIP device is present")
     assert res.returncode == 1 and "status -5" in res.stderr
+
+
+def test_ff_time_logarithm_accuracy(kmc, orc):
+    """The hop kernels' dwell -(1/Gamma) log(r / RAND_MAX) (ff_time, scatterer.h:74-80) uses its
+    own logarithm (atanh series after the fdlibm range reduction).  On the host (exact
+    reciprocal) it must lie within two ulps of the exact value for edge and random draws (the
+    logarithm's own error, 0.74 ulp measured, plus the roundings of 1/Gamma and the product),
+    checked in 40-digit decimal arithmetic, and within 4e-16 relative of glibc's."""
+    from decimal import Decimal, getcontext
+    getcontext().prec = 40
+    D = 2147483647
+    rng = np.random.default_rng(12)
+    r = np.concatenate([np.arange(1, 4000), D - np.arange(0, 4000), 2 ** np.arange(31) - 1,
+                        2 ** np.arange(1, 31), rng.integers(1, D + 1, 200_000)])
+    r = r[(r >= 1) & (r <= D)].astype(np.uint32)
+    gamma = 3.7e12
+    got = kmc.ff_time(gamma, r)
+    ref = np.array([orc.dwell_time(gamma, int(v)) for v in r[:: max(1, r.size // 20000)]])
+    sub = got[:: max(1, r.size // 20000)]
+    nz = ref != 0
+    assert np.all(sub[~nz] == 0)
+    assert np.max(np.abs(sub[nz] - ref[nz]) / ref[nz]) < 4e-16
+    idx = np.concatenate([np.arange(0, 8000, 37), rng.integers(0, r.size, 1500)])
+    worst = 0.0
+    for i in idx:
+        q = Decimal(float(r[i]) / D)          # the correctly rounded quotient the kernels use
+        exact = -(q.ln()) / Decimal(gamma)
+        if exact == 0:
+            assert got[i] == 0
+            continue
+        ulp = Decimal(float(np.spacing(float(exact))))
+        worst = max(worst, float(abs(Decimal(float(got[i])) - exact) / ulp))
+    assert worst <= 2.0, worst
