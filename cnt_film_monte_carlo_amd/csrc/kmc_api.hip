@@ -288,8 +288,7 @@ net_view view_of(kmc_ctx* c) {
   v.n_src = (uint32_t)c->h_src.size();
   v.seed = c->cfg.seed;
   for (int k = 0; k < 3; ++k) {
-    v.per_len[k] = (c->cfg.periodic_mask & (1 << k)) ? c->cfg.period_len[k] : 0.0;
-    v.half_per[k] = 0.5 * v.per_len[k];
+    v.half_per[k] = (c->cfg.periodic_mask & (1 << k)) ? 0.5 * c->cfg.period_len[k] : INFINITY;
   }
   v.n_bins = c->n_bins;
   v.dt_sample = c->n_bins > 0 ? c->t_sample_max / c->n_bins : 0.0;

@@ -47,8 +47,8 @@ struct net_view {
   const uint32_t* src_roff; // first lines of the source sites' blocks
   uint32_t n_src;
   uint64_t seed;
-  double per_len[3];        // period per axis (0 = open) for the minimum-image displacement
-  double half_per[3];       // 0.5 * per_len (kernel arguments, so the thresholds stay scalar)
+  double half_per[3];       // half the period on a periodic axis, +inf on an open one (the
+                            // minimum-image displacement; kernel arguments, so they stay scalar)
   // MSD records [n_bins][x.n]
   int32_t n_bins;
   double dt_sample;

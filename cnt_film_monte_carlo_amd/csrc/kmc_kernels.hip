@@ -493,8 +493,10 @@ struct coop_row {
 
 // Byte offsets into the row blocks are 32-bit (pack_rows refuses blocks of 4 GiB or more), so
 // each load takes the blocks' base from scalar registers and one 32-bit offset per lane.
-__device__ __forceinline__ coop_row coop_load_row(const uint4* __restrict__ blocks, uint32_t W,
-                                                  uint4* lds_wave, uint32_t lane) {
+// coop_issue starts the eight direct-to-LDS loads of the wave's 64 first lines; coop_finish waits
+// for them (and for any other vector memory operation of the wave) and reads the lane's row.
+__device__ __forceinline__ void coop_issue(const uint4* __restrict__ blocks, uint32_t W,
+                                           uint4* lds_wave, uint32_t lane) {
   const uint32_t line = W & TGT_LINE_MASK;
   const uint32_t unit = ((lane & 7u) - (lane >> 3)) & 7u;
   // the eight source lines first (eight lane exchanges in flight together), then the loads
@@ -510,6 +512,9 @@ __device__ __forceinline__ coop_row coop_load_row(const uint4* __restrict__ bloc
     __builtin_amdgcn_global_load_lds((global_void_t*)(base + off), (lds_void_t*)(lds_wave + 64u * k),
                                      16, 0, 0);
   }
+}
+
+__device__ __forceinline__ coop_row coop_finish(const uint4* lds_wave, uint32_t lane) {
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   const uint4* my = lds_wave + 8u * lane;
   const uint4 h = my[lane & 7u], q0 = my[(1u + lane) & 7u], q1 = my[(2u + lane) & 7u],
@@ -530,6 +535,13 @@ __device__ __forceinline__ coop_row coop_load_row(const uint4* __restrict__ bloc
   r.y = __uint_as_float(p.y);
   r.z = __uint_as_float(p.z);
   return r;
+}
+
+// Both in one: the row of W, in registers.
+__device__ __forceinline__ coop_row coop_load_row(const uint4* __restrict__ blocks, uint32_t W,
+                                                  uint4* lds_wave, uint32_t lane) {
+  coop_issue(blocks, W, lds_wave, lane);
+  return coop_finish(lds_wave, lane);
 }
 
 // Target word idx < ROW_INLINE of the lane's row from the wave's LDS copy (units 5 and 6,
@@ -593,23 +605,12 @@ __device__ __forceinline__ uint32_t target_of(const uint4* __restrict__ blocks, 
 
 // One coordinate of a hop's displacement: FP64 difference of the two FP32 positions (exact for
 // sites this close), shifted by one period on a periodic axis when it exceeds half of it.
-__device__ __forceinline__ double min_image(double d, double period) {
+// `half` is half the period on a periodic axis and +inf on an open one (a kernel argument, so
+// it stays in scalar registers); the period is half + half, exact.  One comparison of |d| on
+// the common path; the shift itself is rare.
+__device__ __forceinline__ double min_image(double d, double half) {
 #pragma clang fp contract(off)
-  if (period > 0.0) {
-    if (d > 0.5 * period) d -= period;
-    else if (d < -0.5 * period) d += period;
-  }
-  return d;
-}
-
-// The same with 0.5 * period passed in (a kernel argument: the thresholds stay in scalar
-// registers instead of occupying vector registers across the hop loop); halving is exact.
-__device__ __forceinline__ double min_image(double d, double period, double half) {
-#pragma clang fp contract(off)
-  if (period > 0.0) {
-    if (d > half) d -= period;
-    else if (d < -half) d += period;
-  }
+  if (__builtin_expect(fabs(d) > half, 0)) d = d > half ? d - (half + half) : d + (half + half);
   return d;
 }
 
@@ -725,9 +726,9 @@ __device__ __forceinline__ void fly_segment(const net_view& v, const flight_stat
   px = __uint_as_float(p.x);
   py = __uint_as_float(p.y);
   pz = __uint_as_float(p.z);
-  sx = min_image((double)px - (double)f.x, v.per_len[0], v.half_per[0]);
-  sy = min_image((double)py - (double)f.y, v.per_len[1], v.half_per[1]);
-  sz = min_image((double)pz - (double)f.z, v.per_len[2], v.half_per[2]);
+  sx = min_image((double)px - (double)f.x, v.half_per[0]);
+  sy = min_image((double)py - (double)f.y, v.half_per[1]);
+  sz = min_image((double)pz - (double)f.z, v.half_per[2]);
 }
 
 // Flight of duration tau.  A site reached on the way becomes the current site; a drain site
@@ -899,9 +900,9 @@ __device__ __forceinline__ bool hop_exciton(const net_view& v, uint32_t e, uint6
     // dwell at the new site (particle.cpp:183 -> ff_time)
     R = tg & ~EDGE_DRAIN_BIT;
     row = load_row(v.blocks, R);
-    dx = dx + min_image((double)row.x - (double)sx, v.per_len[0], v.half_per[0]);
-    dy = dy + min_image((double)row.y - (double)sy, v.per_len[1], v.half_per[1]);
-    dz = dz + min_image((double)row.z - (double)sz, v.per_len[2], v.half_per[2]);
+    dx = dx + min_image((double)row.x - (double)sx, v.half_per[0]);
+    dy = dy + min_image((double)row.y - (double)sy, v.half_per[1]);
+    dz = dz + min_image((double)row.z - (double)sz, v.half_per[2]);
     hs = (hs ^ (uint64_t)row.site) * FNV_PRIME;
     const uint32_t r = nonzero_draw1(w_dw, v.seed, e, h, STREAM_HOP_REDRAW);
     t = t + dwell_time(row.n, row.gamma, r);
@@ -945,12 +946,20 @@ __device__ __forceinline__ bool hop_exciton(const net_view& v, uint32_t e, uint6
   return !trapped(row.n, row.gamma) && t <= t_max;
 }
 
-// The hop loop of one wave (v = 0), built around coop_load_row: every lane takes part in every
-// row load, so the loop runs until no lane of the wave can hop; a lane without an exciton (valid
-// false) or whose exciton has stopped keeps loading its current row.  A hop is finished once its
-// target row is in registers (displacement, visit hash, dwell at the new site), which keeps every
-// exciton's arithmetic and its order identical to hop_exciton.  A drain target's header and
-// position are gathered by the lanes concerned, and their next row load is the source site's.
+// The hop loop of one wave (v = 0), built around the cooperative row load: every lane takes part
+// in every row load, so the loop runs until no lane of the wave can hop; a lane without an
+// exciton (valid false) or whose exciton has stopped keeps loading some row.  Each iteration
+// takes the row of the exciton's current site and
+//   (A) selects the next hop from it (draws, dice, target) and issues the load of the next row
+//       -- the target's, or the source site's after a drain -- before anything else, so the load
+//       is in flight while
+//   (B) the hop into this row finishes (displacement, visit hash, dwell at this site) and the
+//       MSD records fall due, and
+//   (C) the hop selected in (A) happens if the exciton's clock allows it (t <= t_max), else the
+//       exciton stops at this site and the issued load is simply not used.
+// Every exciton's arithmetic and its order are those of hop_exciton (and of the oracle); only
+// the selection is computed before the clock test that decides whether it is used.  A drain
+// target's header and position are gathered by the lanes concerned.
 // COUNT: departures from the block lines [win_base, win_base + COUNT_WINDOW) are counted in the
 // workgroup's LDS window `win` (zeroed and flushed by the kernel), the rest through a
 // four-entry register LRU into the XCD's copy of the counts.
@@ -985,6 +994,8 @@ __device__ __forceinline__ void hop_wave(const net_view& v, uint32_t e, bool val
   double t_rec = INFINITY;
   if (SAMPLE && k < (uint32_t)v.n_bins) t_rec = (double)(k + 1u) * v.dt_sample;
   bool active = valid;
+  bool live = false;           // when stopped: able to hop in a later launch
+  uint32_t site_now = 0;       // when stopped: the site it stopped at
   bool pend = false;           // a hop waits for its target row
   uint32_t r_dw = 0;           // its dwell draw
   float px = 0.f, py = 0.f, pz = 0.f;  // position of the site it left
@@ -992,15 +1003,30 @@ __device__ __forceinline__ void hop_wave(const net_view& v, uint32_t e, bool val
   hop_words hw;
   count_cache cc;
   uint32_t* cbase = COUNT ? v.site_count + (size_t)xcd_id() * v.count_stride : nullptr;
-  coop_row row;
 
+  coop_issue(v.blocks, W, lds_wave, lane);
   while (true) {
-    row = coop_load_row(v.blocks, W, lds_wave, lane);
+    const coop_row row = coop_finish(lds_wave, lane);
+    // (A) the hop from this site, if the exciton may still hop (scatterer::update_state,
+    // scatterer.cpp:17-30), and the load of the row it leads to
+    const bool can = active && h < h_end && !trapped(row.n, row.gamma);
+    uint32_t tg = 0, r_next = 0, W_next = W;
+    if (can) {
+      uint32_t w_sel, w_dw;
+      hw.get(v.seed, e, h, w_sel, w_dw);
+      const double dice = div_rand_max(row.gamma * (double)r31(w_sel));
+      const uint32_t idx = select_entry(v.blocks, W, row.c, row.n, dice);
+      tg = idx < ROW_INLINE ? target_lds(lds_wave, lane, idx) : target_wide(v.blocks, W, row.n, idx);
+      r_next = nonzero_draw1(w_dw, v.seed, e, h, STREAM_HOP_REDRAW);
+      W_next = (tg & EDGE_DRAIN_BIT) ? v.src_roff[r31(draw4(v.seed, e, h, STREAM_REINJECT).v[0]) % v.n_src]
+                                     : (tg & ~EDGE_DRAIN_BIT);
+    }
+    coop_issue(v.blocks, W_next, lds_wave, lane);
+    // (B) the hop into this site finishes: dwell at the new site (particle.cpp:183 -> ff_time)
     if (pend) {
-      // dwell at the new site (particle.cpp:183 -> ff_time)
-      dx = dx + min_image((double)row.x - (double)px, v.per_len[0], v.half_per[0]);
-      dy = dy + min_image((double)row.y - (double)py, v.per_len[1], v.half_per[1]);
-      dz = dz + min_image((double)row.z - (double)pz, v.per_len[2], v.half_per[2]);
+      dx = dx + min_image((double)row.x - (double)px, v.half_per[0]);
+      dy = dy + min_image((double)row.y - (double)py, v.half_per[1]);
+      dz = dz + min_image((double)row.z - (double)pz, v.half_per[2]);
       hs = (hs ^ (uint64_t)row.site) * FNV_PRIME;
       t = t + dwell_time(row.n, row.gamma, r_dw);
       if (TRACE && e < tv.n_trace && tr < tv.tr_max) {
@@ -1021,36 +1047,32 @@ __device__ __forceinline__ void hop_wave(const net_view& v, uint32_t e, bool val
         t_rec = k < (uint32_t)v.n_bins ? (double)(k + 1u) * v.dt_sample : INFINITY;
       }
     }
-    active = active && !(h >= h_end || trapped(row.n, row.gamma) || !(t <= t_max));
-    if (__ballot(active) == 0) break;
-    if (active) {
-      // event selection (scatterer::update_state, scatterer.cpp:17-30)
-      uint32_t w_sel, w_dw;
-      hw.get(v.seed, e, h, w_sel, w_dw);
-      const double dice = div_rand_max(row.gamma * (double)r31(w_sel));
-      const uint32_t idx = select_entry(v.blocks, W, row.c, row.n, dice);
-      const uint32_t tg = idx < ROW_INLINE ? target_lds(lds_wave, lane, idx)
-                                           : target_wide(v.blocks, W, row.n, idx);
+    // (C) the hop of (A) happens at t if t <= t_max; otherwise the exciton stops here
+    const bool hop = can && t <= t_max;
+    if (active && !hop) {
+      active = false;
+      site_now = row.site;
+      live = !trapped(row.n, row.gamma) && t <= t_max;
+    }
+    if (hop) {
       if (COUNT) {
         const uint32_t off = (W & TGT_LINE_MASK) - win_base;
         if (off < COUNT_WINDOW) atomicAdd(win + off, 1u);
         else cc.add(count_slot(row.site, v.count_stride), cbase);
       }
-      const uint32_t r = nonzero_draw1(w_dw, v.seed, e, h, STREAM_HOP_REDRAW);
       t_hop = t;
       if (tg & EDGE_DRAIN_BIT) {
         // drain: the dwell drawn at the drain site, then re-injection keeping displacement and
         // dwell (kubo_step, src/monte_carlo/monte_carlo.cpp:330-336)
         const uint4* bd = v.blocks + (size_t)ROW_UNITS * (tg & TGT_LINE_MASK);
         const uint4 hd = bd[0], pd = bd[7];
-        dx = dx + min_image((double)__uint_as_float(pd.x) - (double)row.x, v.per_len[0], v.half_per[0]);
-        dy = dy + min_image((double)__uint_as_float(pd.y) - (double)row.y, v.per_len[1], v.half_per[1]);
-        dz = dz + min_image((double)__uint_as_float(pd.z) - (double)row.z, v.per_len[2], v.half_per[2]);
+        dx = dx + min_image((double)__uint_as_float(pd.x) - (double)row.x, v.half_per[0]);
+        dy = dy + min_image((double)__uint_as_float(pd.y) - (double)row.y, v.half_per[1]);
+        dz = dz + min_image((double)__uint_as_float(pd.z) - (double)row.z, v.half_per[2]);
         hs = (hs ^ (uint64_t)hd.y) * FNV_PRIME;
-        t = t + dwell_time(hd.x, __hiloint2double((int)hd.w, (int)hd.z), r);
+        t = t + dwell_time(hd.x, __hiloint2double((int)hd.w, (int)hd.z), r_next);
         ++nd;
-        W = v.src_roff[r31(draw4(v.seed, e, h, STREAM_REINJECT).v[0]) % v.n_src];
-        const uint32_t src_site = v.blocks[(size_t)ROW_UNITS * (W & TGT_LINE_MASK)].y;
+        const uint32_t src_site = v.blocks[(size_t)ROW_UNITS * (W_next & TGT_LINE_MASK)].y;
         hs = (hs ^ ((uint64_t)src_site | (1ull << 32))) * FNV_PRIME;
         if (TRACE && e < tv.n_trace && tr < tv.tr_max) {
           tv.tr_site[(size_t)e * tv.tr_max + tr] = (int32_t)src_site;
@@ -1058,24 +1080,27 @@ __device__ __forceinline__ void hop_wave(const net_view& v, uint32_t e, bool val
           ++tr;
         }
       } else {
-        W = tg & ~EDGE_DRAIN_BIT;
         pend = true;
-        r_dw = r;
+        r_dw = r_next;
         px = row.x;
         py = row.y;
         pz = row.z;
       }
+      W = W_next;
       ++h;
     }
+    if (__ballot(active) == 0) break;
   }
+  // the last issued load lands in this wave's LDS rows: wait for it before the wave moves on
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   if (COUNT) cc.flush(cbase);
   if (v.live) {
     // excitons that a further launch could move (the budget aside)
-    const uint64_t b = __ballot(valid && !trapped(row.n, row.gamma) && t <= t_max);
+    const uint64_t b = __ballot(valid && live);
     if (lane == 0u && b != 0ull) atomicAdd(v.live, (uint32_t)__popcll(b));
   }
   if (!valid) return;
-  v.x.site[e] = (int32_t)row.site;
+  v.x.site[e] = (int32_t)site_now;
   v.x.t_next[e] = t;
   v.x.hops[e] = h;
   v.x.dx[e] = dx;
