@@ -59,6 +59,19 @@ __host__ __device__ __forceinline__ double recip(double d) {
 // |s| <= 3 - 2 sqrt(2).  k ln 2 is added in two parts (k ln2_hi exact for |k| < 2^11).  About
 // 40 VALU operations instead of the ~75 of the library log, within two ulps of the exact value
 // (checked against glibc on the host); event times need 1e-12 relative.
+// fma(a, b, c) as one VOP3 v_fma_f64 on the device.  The compiler otherwise evaluates a Horner
+// step with loop-invariant coefficients as a copy of the coefficient plus the two-address
+// v_fmac_f64: two instructions where one does.  Same rounding (one FMA).
+__host__ __device__ __forceinline__ double fma3(double a, double b, double c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double d;
+  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+  return d;
+#else
+  return __builtin_fma(a, b, c);
+#endif
+}
+
 constexpr double LN2_HI = 0x1.62e42fefa3800p-1;   // ln 2 to 42 bits
 constexpr double LN2_LO = 0x1.ef35793c76730p-45;  // ln 2 - LN2_HI
 __host__ __device__ __forceinline__ double log_unit(double q) {
@@ -71,15 +84,15 @@ __host__ __device__ __forceinline__ double log_unit(double q) {
   const double s = f * recip(2.0 + f);
   const double z = s * s;
   double R = 2.0 / 21.0;
-  R = __builtin_fma(R, z, 2.0 / 19.0);
-  R = __builtin_fma(R, z, 2.0 / 17.0);
-  R = __builtin_fma(R, z, 2.0 / 15.0);
-  R = __builtin_fma(R, z, 2.0 / 13.0);
-  R = __builtin_fma(R, z, 2.0 / 11.0);
-  R = __builtin_fma(R, z, 2.0 / 9.0);
-  R = __builtin_fma(R, z, 2.0 / 7.0);
-  R = __builtin_fma(R, z, 2.0 / 5.0);
-  R = __builtin_fma(R, z, 2.0 / 3.0);
+  R = fma3(R, z, 2.0 / 19.0);
+  R = fma3(R, z, 2.0 / 17.0);
+  R = fma3(R, z, 2.0 / 15.0);
+  R = fma3(R, z, 2.0 / 13.0);
+  R = fma3(R, z, 2.0 / 11.0);
+  R = fma3(R, z, 2.0 / 9.0);
+  R = fma3(R, z, 2.0 / 7.0);
+  R = fma3(R, z, 2.0 / 5.0);
+  R = fma3(R, z, 2.0 / 3.0);
   R = R * z;
   const double hfsq = 0.5 * f * f;
   const double dk = (double)k;

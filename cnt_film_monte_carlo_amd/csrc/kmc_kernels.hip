@@ -551,6 +551,17 @@ __device__ __forceinline__ uint32_t target_lds(const uint4* lds_wave, uint32_t l
   return reinterpret_cast<const uint32_t*>(lds_wave + 8u * lane + slot)[idx & 3u];
 }
 
+// The number of entries <= d of a non-decreasing c[0..8), in 0..8: a three-step binary search
+// over registers (selects, no indexing) and one more comparison for 8.  Step one tells count >= 4,
+// step two count >= base + 2 (base = 0 or 4), step three count >= base + 2 b1 + 1.
+__device__ __forceinline__ uint32_t count_le8(const double* c, double d) {
+  const bool b2 = c[3] <= d;
+  const bool b1 = (b2 ? c[5] : c[1]) <= d;
+  const bool b0 = (b2 ? (b1 ? c[6] : c[4]) : (b1 ? c[2] : c[0])) <= d;
+  const uint32_t idx = (b2 ? 4u : 0u) | (b1 ? 2u : 0u) | (b0 ? 1u : 0u);
+  return (idx == 7u && c[7] <= d) ? 8u : idx;
+}
+
 // Index picked by update_state's binary search (scatterer.cpp:17-30): the first entry whose
 // cumulative rate exceeds dice, or n-1.  Rows are non-decreasing (rates >= 0 are enforced when
 // the CSR is built), so that index equals the number of leading entries k < n-1 with
@@ -559,9 +570,7 @@ __device__ __forceinline__ uint32_t target_lds(const uint4* lds_wave, uint32_t l
 // next line.
 __device__ __forceinline__ uint32_t select_entry(const uint4* __restrict__ blocks, uint32_t W,
                                                  const double* c8, uint32_t n, double dice) {
-  uint32_t idx = 0;
-#pragma unroll
-  for (uint32_t q = 0; q < 8; ++q) idx += (c8[q] <= dice) ? 1u : 0u;
+  uint32_t idx = count_le8(c8, dice);
   if (idx == 8u && n > 9u) {
     // cum[8..16): the first 64 bytes of the continuation line, which a row with n > 9 always
     // owns, read in one batch of four 16-byte loads before any comparison; then further entries

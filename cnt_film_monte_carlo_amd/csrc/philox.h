@@ -24,6 +24,9 @@ __host__ __device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32
 #endif
 }
 
+// On the device the key (the seed) must be the same for every lane of the wave: its ten round
+// values are stepped in scalar registers by inline adds, so that the compiler does not keep all
+// twenty of them live across a caller's loop.
 __host__ __device__ __forceinline__ u32x4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2,
                                                         uint32_t c3, uint32_t k0, uint32_t k1) {
   constexpr uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
@@ -40,8 +43,15 @@ __host__ __device__ __forceinline__ u32x4 philox4x32_10(uint32_t c0, uint32_t c1
     c1 = lo1;
     c2 = n2;
     c3 = lo0;
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (r < 9) {
+      asm volatile("s_add_u32 %0, %0, 0x9E3779B9" : "+s"(k0) : : "scc");
+      asm volatile("s_add_u32 %0, %0, 0xBB67AE85" : "+s"(k1) : : "scc");
+    }
+#else
     k0 += W0;
     k1 += W1;
+#endif
   }
   u32x4 out;
   out.v[0] = c0;
