@@ -140,7 +140,7 @@ struct kmc_ctx {
   dbuf<uint32_t> x_drains, x_krec;
   dbuf<uint8_t> x_heading;
   dbuf<double> x_soff, x_tlast;
-  dbuf<uint32_t> d_site_count;  // KMC_COUNT_COPIES copies, count_stride apart
+  dbuf<uint32_t> d_site_count;  // KMC_COUNT_COPIES copies, count_stride apart, per block line
   dbuf<uint64_t> d_counts64;    // running 64-bit totals the copies are folded into
   int64_t count_stride = 0;
   uint64_t count_pending = 0;   // bound on any copy's count since the last fold
@@ -1017,7 +1017,8 @@ int kmc_inject(kmc_ctx* c, int64_t n_excitons) {
   KMC_TRY(dalloc(c, c->x_heading, E));
   KMC_TRY(dalloc(c, c->x_soff, E));
   KMC_TRY(dalloc(c, c->x_tlast, E));
-  c->count_stride = (c->n_sites + COUNT_ALIGN - 1) / COUNT_ALIGN * COUNT_ALIGN;
+  const int64_t n_lines = c->n_units / ROW_UNITS;  // counters per block line
+  c->count_stride = (n_lines + COUNT_ALIGN - 1) / COUNT_ALIGN * COUNT_ALIGN;
   KMC_TRY(dalloc(c, c->d_site_count, (size_t)KMC_COUNT_COPIES * c->count_stride));
   KMC_HIP(c, hipMemsetAsync(c->d_site_count.p, 0, c->d_site_count.n * sizeof(uint32_t),
                             c->stream));
@@ -1096,8 +1097,8 @@ static inline bool live_check_due(int64_t launches) {
 }
 
 static int fold_counts(kmc_ctx* c, hipStream_t st) {
-  KMC_HIP(c, launch_fold_counts(c->n_sites, c->count_stride, c->d_site_count.p,
-                                c->d_counts64.p, st));
+  KMC_HIP(c, launch_fold_counts(c->n_units / ROW_UNITS, c->count_stride, c->d_blocks.p,
+                                c->d_site_count.p, c->d_counts64.p, st));
   c->count_pending = 0;
   return KMC_OK;
 }
@@ -1475,8 +1476,8 @@ int ens_run_hops(kmc_ensemble* m, int64_t hops, int64_t chunk, bool count, bool 
       for (kmc_ctx* c : m->ctxs) {
         const uint64_t add = (uint64_t)c->n_exc * (uint64_t)sub;
         if (c->count_pending + add > COUNT_COPY_MAX) {
-          ENS_HIP(launch_fold_counts(c->n_sites, c->count_stride, c->d_site_count.p,
-                                     c->d_counts64.p, m->stream));
+          ENS_HIP(launch_fold_counts(c->n_units / ROW_UNITS, c->count_stride, c->d_blocks.p,
+                                     c->d_site_count.p, c->d_counts64.p, m->stream));
           c->count_pending = 0;
         }
         c->count_pending += add;

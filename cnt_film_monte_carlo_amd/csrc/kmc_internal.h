@@ -114,12 +114,14 @@ constexpr uint64_t FNV_PRIME = 0x100000001b3ull;
 constexpr int MAX_CHIR = 16;
 #define KMC_COUNT_COPIES 8u  // one departure-count array per XCD
 constexpr int64_t COUNT_ALIGN = 1024;  // each copy starts on a 4 KB boundary
-// Slot of site s in a departure-count copy of count_stride (a multiple of 32) counters: the 32
-// counters of a 128-byte line hold sites 32 apart in slot order, s -> (s % 32) stride / 32 +
-// s / 32.  Excitons sorted by region hop among sites with neighbouring ids, so without this their
-// atomics crowd onto a few lines and L2 channels (2.3x slower at ten excitons per site).
-__host__ __device__ __forceinline__ uint32_t count_slot(uint32_t s, int64_t count_stride) {
-  return (s & 31u) * (uint32_t)(count_stride >> 5) + (s >> 5);
+// Departure counts are kept per block line (the first line of the departure site's block, in
+// the Morton placement of the blocks), so the counters an XCD's excitons touch -- one compact
+// region of the film -- form a compact, L2-resident range.  Slot of line L in a copy of
+// count_stride (a multiple of 32) counters: the 32 counters of a 128-byte line hold lines 32
+// apart, L -> (L % 32) stride / 32 + L / 32, so that excitons hopping among neighbouring blocks
+// spread their atomics over many cache lines and L2 channels.
+__host__ __device__ __forceinline__ uint32_t count_slot(uint32_t L, int64_t count_stride) {
+  return (L & 31u) * (uint32_t)(count_stride >> 5) + (L >> 5);
 }
 
 struct __align__(16) site_hdr {

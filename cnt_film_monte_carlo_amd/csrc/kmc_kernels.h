@@ -54,7 +54,7 @@ struct net_view {
   double dt_sample;
   double *rec_dx, *rec_dy, *rec_dz;
   uint32_t* rec_drain;
-  // per-site departure counts: KMC_COUNT_COPIES arrays, count_stride apart
+  // departure counts per block line: KMC_COUNT_COPIES arrays, count_stride apart
   uint32_t* site_count;
   int64_t count_stride;
   // drift along the tubes (particle::fly), used when velocity > 0
@@ -177,9 +177,10 @@ hipError_t launch_morton_keys(int64_t n, const int32_t* cell_of, int32_t nx, int
 // out[i] = row word of site idx[i] (first line of its block | TGT_WIDE_BIT)
 hipError_t launch_gather_offsets(int64_t n, const int32_t* idx, const int64_t* roff,
                                  const site_hdr* hdr, uint32_t* out, hipStream_t st);
-// totals[i] += the per-XCD departure counts of site i, which are then zeroed
-hipError_t launch_fold_counts(int64_t n, int64_t stride, uint32_t* copies, uint64_t* totals,
-                              hipStream_t st);
+// totals[site] += the per-XCD departure counts of each block line (the site's first line), which
+// are then zeroed
+hipError_t launch_fold_counts(int64_t n_lines, int64_t stride, const uint4* blocks,
+                              uint32_t* copies, uint64_t* totals, hipStream_t st);
 hipError_t launch_pair_classes(int64_t n, int64_t nnz, const site_hdr* hdr, const edge_rec* edge,
                                const uint8_t* chir, int32_t n_chir, uint8_t* cls, uint32_t* counts,
                                hipStream_t st);

@@ -902,7 +902,7 @@ __device__ __forceinline__ bool hop_exciton(const net_view& v, uint32_t e, uint6
     const double dice = div_rand_max(row.gamma * (double)r31(w_sel));
     const uint32_t idx = select_entry(v.blocks, R, row.c, row.n, dice);
     const uint32_t tg = target_of(v.blocks, R, row, idx);
-    if (COUNT) cc.add(count_slot(row.site, v.count_stride), cbase);
+    if (COUNT) cc.add(count_slot(R & TGT_LINE_MASK, v.count_stride), cbase);
     const float sx = row.x, sy = row.y, sz = row.z;
     const double t_hop = t;
 
@@ -973,6 +973,7 @@ __device__ __forceinline__ bool hop_exciton(const net_view& v, uint32_t e, uint6
 // workgroup's LDS window `win` (zeroed and flushed by the kernel), the rest through a
 // four-entry register LRU into the XCD's copy of the counts.
 constexpr uint32_t COUNT_WINDOW = 2048;
+constexpr uint32_t COUNT_NONE = 0xFFFFFFFFu;
 
 template <bool COUNT, bool SAMPLE, bool TRACE>
 __device__ __forceinline__ void hop_wave(const net_view& v, uint32_t e, bool valid,
@@ -1012,10 +1013,21 @@ __device__ __forceinline__ void hop_wave(const net_view& v, uint32_t e, bool val
   hop_words hw;
   count_cache cc;
   uint32_t* cbase = COUNT ? v.site_count + (size_t)xcd_id() * v.count_stride : nullptr;
+  // the departure of the last hop is counted at the top of the next iteration, so that an
+  // eviction's device atomic has a whole iteration to complete before the wave waits on its
+  // next row load (one counter tracks both)
+  uint32_t count_line = COUNT_NONE;
+  auto count_departure = [&]() {
+    const uint32_t off = count_line - win_base;
+    if (off < COUNT_WINDOW) atomicAdd(win + off, 1u);
+    else cc.add(count_slot(count_line, v.count_stride), cbase);
+    count_line = COUNT_NONE;
+  };
 
   coop_issue(v.blocks, W, lds_wave, lane);
   while (true) {
     const coop_row row = coop_finish(lds_wave, lane);
+    if (COUNT && count_line != COUNT_NONE) count_departure();
     // (A) the hop from this site, if the exciton may still hop (scatterer::update_state,
     // scatterer.cpp:17-30), and the load of the row it leads to
     const bool can = active && h < h_end && !trapped(row.n, row.gamma);
@@ -1064,11 +1076,7 @@ __device__ __forceinline__ void hop_wave(const net_view& v, uint32_t e, bool val
       live = !trapped(row.n, row.gamma) && t <= t_max;
     }
     if (hop) {
-      if (COUNT) {
-        const uint32_t off = (W & TGT_LINE_MASK) - win_base;
-        if (off < COUNT_WINDOW) atomicAdd(win + off, 1u);
-        else cc.add(count_slot(row.site, v.count_stride), cbase);
-      }
+      if (COUNT) count_line = W & TGT_LINE_MASK;
       t_hop = t;
       if (tg & EDGE_DRAIN_BIT) {
         // drain: the dwell drawn at the drain site, then re-injection keeping displacement and
@@ -1102,6 +1110,7 @@ __device__ __forceinline__ void hop_wave(const net_view& v, uint32_t e, bool val
   }
   // the last issued load lands in this wave's LDS rows: wait for it before the wave moves on
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  if (COUNT && count_line != COUNT_NONE) count_departure();
   if (COUNT) cc.flush(cbase);
   if (v.live) {
     // excitons that a further launch could move (the budget aside)
@@ -1166,10 +1175,7 @@ __global__ __launch_bounds__(256, 4) void k_hop(hop_args a) {
     uint32_t* cbase = a.v.site_count + (size_t)xcd_id() * a.v.count_stride;
     for (uint32_t i = threadIdx.x; i < COUNT_WINDOW; i += blockDim.x) {
       const uint32_t c = win[i];
-      if (c) {
-        const uint32_t site = a.v.blocks[(size_t)ROW_UNITS * (base + i)].y;
-        atomicAdd(cbase + count_slot(site, a.v.count_stride), c);
-      }
+      if (c) atomicAdd(cbase + count_slot(base + i, a.v.count_stride), c);
     }
   }
 }
@@ -1233,10 +1239,7 @@ __global__ __launch_bounds__(256, 4) void k_hop_ens(ens_args a) {
     uint32_t* cbase = v.site_count + (size_t)xcd_id() * v.count_stride;
     for (uint32_t i = threadIdx.x; i < COUNT_WINDOW; i += blockDim.x) {
       const uint32_t c = win[i];
-      if (c) {
-        const uint32_t site = v.blocks[(size_t)ROW_UNITS * (base + i)].y;
-        atomicAdd(cbase + count_slot(site, v.count_stride), c);
-      }
+      if (c) atomicAdd(cbase + count_slot(base + i, v.count_stride), c);
     }
   }
 }
@@ -1302,22 +1305,27 @@ hipError_t launch_hop(const hop_args& a, bool count, bool sample, bool trace, bo
   return hipErrorInvalidValue;
 }
 
-// The per-XCD copies of the departure counts folded into the 64-bit totals and zeroed (slots
-// and sites correspond one to one, so every thread owns its slot in all copies).
-__global__ __launch_bounds__(256) void k_fold_counts(int64_t n, int64_t stride,
+// The per-XCD copies of the departure counts folded into the 64-bit per-site totals and zeroed.
+// Thread q owns block line q (slots and lines correspond one to one) in all copies; only a
+// block's first line is ever counted, and its header gives the site.
+__global__ __launch_bounds__(256) void k_fold_counts(int64_t n_lines, int64_t stride,
+                                                     const uint4* __restrict__ blocks,
                                                      uint32_t* __restrict__ copies,
                                                      uint64_t* __restrict__ totals) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n_lines) return;
   uint64_t s = 0;
-  const uint32_t slot = count_slot((uint32_t)i, stride);
+  const uint32_t slot = count_slot((uint32_t)q, stride);
 #pragma unroll
   for (int x = 0; x < KMC_COUNT_COPIES; ++x) {
     uint32_t* c = copies + (size_t)x * stride + slot;
-    s += *c;
-    *c = 0u;
+    const uint32_t v = *c;
+    if (v) {
+      s += v;
+      *c = 0u;
+    }
   }
-  totals[i] += s;
+  if (s) totals[blocks[(size_t)ROW_UNITS * q].y] += s;
 }
 
 // =============================================================================================
@@ -1873,10 +1881,11 @@ hipError_t launch_gather_offsets(int64_t n, const int32_t* idx, const int64_t* r
   return hipGetLastError();
 }
 
-hipError_t launch_fold_counts(int64_t n, int64_t stride, uint32_t* copies, uint64_t* totals,
-                              hipStream_t st) {
-  if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_fold_counts, grid_for(n), dim3(256), 0, st, n, stride, copies, totals);
+hipError_t launch_fold_counts(int64_t n_lines, int64_t stride, const uint4* blocks,
+                              uint32_t* copies, uint64_t* totals, hipStream_t st) {
+  if (n_lines == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fold_counts, grid_for(n_lines), dim3(256), 0, st, n_lines, stride, blocks,
+                     copies, totals);
   return hipGetLastError();
 }
 
