@@ -45,12 +45,81 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(net, w, tables, csr, seconds=12.0):
-    """The oracle (CPU restatement of the reference's OpenMP path) on a bounded sample of the
-    same workload: 'faithful' recomputes each site's neighbour list and rates on every hop as
-    scatterer.cpp:12 does; 'cached' reads the same CSR as the GPU."""
-    import numpy as np
+def counter_record(key, hops_per_launch):
+    """The record `key` of profiles/hop_kernel_counters.json if its kernel-source hash is that
+    of the sources in this tree (and, for a hop workload, its hops per launch are this run's);
+    else None."""
+    from cnt_film_monte_carlo_amd import build as B
 
+    f = ROOT / "profiles" / "hop_kernel_counters.json"
+    if not f.exists():
+        return None
+    try:
+        rec = json.loads(f.read_text()).get(key)
+    except ValueError:
+        return None
+    if not rec or rec.get("source_hash") != B.kernel_source_hash():
+        if rec:
+            log(f"counter record '{key}' was taken on other kernel sources: not used")
+        return None
+    if hops_per_launch is not None and round(rec["hops_per_launch"]) != round(hops_per_launch):
+        log(f"counter record '{key}' has {rec['hops_per_launch']:.4g} hops per launch, this run "
+            f"{hops_per_launch:.4g}: not used")
+        return None
+    return rec
+
+
+def cpu_model() -> str:
+    """The host CPU's model string (lscpu's "Model name", else /proc/cpuinfo's)."""
+    import subprocess
+
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _sized_sample(make, run, E, seconds, h_max):
+    """Hops per exciton H of a fresh sample of E excitons sized so that run(sample, H) takes
+    about `seconds` (a short pilot first, one re-size if the run came out short); returns
+    (H, wall seconds of the last run)."""
+    pilot = make()
+    run(pilot, 5)
+    h = 20
+    while True:
+        t0 = time.perf_counter()
+        run(pilot, h)
+        dt = time.perf_counter() - t0
+        if dt > 0.5 or h >= h_max:
+            break
+        h = min(h_max, h * 4)
+    H = int(max(50, min(h_max, h * seconds / max(dt, 1e-6))))
+    for _ in range(2):
+        ex = make()
+        t0 = time.perf_counter()
+        run(ex, H)
+        dt = time.perf_counter() - t0
+        if dt >= 0.8 * seconds or H >= h_max:
+            break
+        H = int(max(50, min(h_max, H * 1.1 * seconds / dt)))
+    return H, dt
+
+
+def cpu_baseline(net, w, tables, csr, seconds=12.0):
+    """The oracle (CPU restatement of the reference's OpenMP path) on bounded samples of the
+    same workload, each sized to about `seconds` of CPU work: 'faithful' recomputes each site's
+    neighbour list and rates on every hop as scatterer.cpp:12 does; 'cached' reads the same CSR
+    as the GPU."""
     from cnt_film_monte_carlo_amd import kmc
     from oracle import oracle as orc
 
@@ -58,42 +127,30 @@ def cpu_baseline(net, w, tables, csr, seconds=12.0):
     onet = orc.Net(net, w.cutoff, 0.4e-9, kmc.PERIODIC_Y, (0, 0, 0), (0, w.period_y[1], 0))
     onet.set_table(th, z, a1, a2, rates)
     threads = orc.library().orc_max_threads()
+    model = cpu_model()
     E = 4096
-    pilot = orc.Excitons(csr, net["flags"], seed=99, n_excitons=E)
-    pilot.run(5, net=onet)
-    t0 = time.perf_counter()
-    pilot.run(200, net=onet)
-    rate = E * 200 / (time.perf_counter() - t0)
-    H = int(max(50, min(200_000, rate * seconds / E)))
-    ex = orc.Excitons(csr, net["flags"], seed=99, n_excitons=E)
-    t0 = time.perf_counter()
-    ex.run(H, net=onet)
-    dt = time.perf_counter() - t0
-    if dt < seconds / 3 and H < 200_000:
-        # the short pilot underestimates the steady rate: size the sample again from this run
-        H = int(max(50, min(200_000, H * seconds / dt)))
-        ex = orc.Excitons(csr, net["flags"], seed=99, n_excitons=E)
-        t0 = time.perf_counter()
-        ex.run(H, net=onet)
-        dt = time.perf_counter() - t0
+    H, dt = _sized_sample(lambda: orc.Excitons(csr, net["flags"], seed=99, n_excitons=E),
+                          lambda ex, h: ex.run(h, net=onet), E, seconds, 200_000)
     faithful = E * H / dt
-    log(f"cpu faithful: {E}x{H} hops in {dt:.2f} s = {faithful:.3e} hops/s on {threads} threads")
-    Hc = 2000
-    exc = orc.Excitons(csr, net["flags"], seed=99, n_excitons=E * 4)
-    t0 = time.perf_counter()
-    exc.run(Hc)
-    dtc = time.perf_counter() - t0
-    cached = 4 * E * Hc / dtc
-    log(f"cpu cached-CSR: {4 * E}x{Hc} hops in {dtc:.2f} s = {cached:.3e} hops/s")
+    log(f"cpu faithful: {E}x{H} hops in {dt:.2f} s = {faithful:.3e} hops/s on {threads} threads "
+        f"({model})")
+    Ec = 4 * E
+    Hc, dtc = _sized_sample(lambda: orc.Excitons(csr, net["flags"], seed=99, n_excitons=Ec),
+                            lambda ex, h: ex.run(h), Ec, seconds, 2_000_000)
+    cached = Ec * Hc / dtc
+    log(f"cpu cached-CSR: {Ec}x{Hc} hops in {dtc:.2f} s = {cached:.3e} hops/s")
     return {
         "value": faithful,
         "unit": "hops/s",
         "cores": threads,
+        "cpu_model": model,
         "kind": "port",
-        "sample": (f"{E} excitons x {H} hops on the same {net['n']:,}-site network, neighbours "
-                   f"and rates recomputed per hop (scatterer.cpp:12), OpenMP over excitons"),
+        "sample": (f"{E} excitons x {H} hops ({dt:.1f} s) on the same {net['n']:,}-site "
+                   f"network, neighbours and rates recomputed per hop (scatterer.cpp:12), "
+                   f"OpenMP over excitons"),
         "cached_csr_value": cached,
-        "cached_csr_sample": f"{4 * E} excitons x {Hc} hops over the GPU-built CSR",
+        "cached_csr_sample": (f"{Ec} excitons x {Hc} hops ({dtc:.1f} s) over the GPU-built CSR, "
+                              f"OpenMP over excitons"),
     }
 
 
@@ -475,12 +532,14 @@ def main():
     ctx.set_samples(args.bins, t_sample_max)
     ctx.sync()
 
-    kernel_ms = []
+    kernel_ms, hop_ms, n_launch = [], [], 0
     t_start = time.perf_counter()
     for i in range(args.steps):
         ms = ctx.run(STEP_HOPS, count_sites=True, timed=True)
         kernel_ms.append(ms)
-        log(f"step {i}: hop kernel {ms:.1f} ms")
+        hop_ms.append(ctx.hop_time())
+        n_launch += ctx.run_stats(live=False)[0]
+        log(f"step {i}: {ms:.1f} ms, of which hop kernel {hop_ms[-1]:.1f} ms")
     red, red_ms = ctx.reduce_samples(32)
     ctx.sync()
     t_end = time.perf_counter()
@@ -491,9 +550,9 @@ def main():
     counts = ctx.get_hop_counts().astype(np.float64)
     d_v = float((counts * deg).sum() / counts.sum())
     bytes_per_hop = 24.0 + 8.0 * d_v
-    launches = -(-STEP_HOPS // RESORT_HOPS)          # hop-kernel launches per step
-    avg_ms = float(np.mean(kernel_ms)) / launches       # device time per launch (sorts included)
-    hops_per_launch = hops_total / (args.steps * launches)
+    launches = n_launch / args.steps                    # hop-kernel launches per step
+    avg_ms = float(np.sum(hop_ms)) / n_launch           # hop-kernel device time per launch
+    hops_per_launch = hops_total / n_launch
     achieved = hops_per_launch * bytes_per_hop / (avg_ms * 1e-3)
     try:
         D, ci = kmc.estimate_diffusion(red, t_sample_max, args.bins // 4, args.bins)
@@ -512,17 +571,26 @@ def main():
     contact = None if args.no_contact else contact_leg(w, net, tabs)
 
 
-    # HBM bytes per hop-kernel launch measured with rocprofv3 counters on this workload
-    # (FETCH_SIZE and WRITE_SIZE passes scaled by a plain-copy calibration); None if unmeasured
-    traffic = None
-    tf = ROOT / "profiles" / "hop_kernel_traffic.json"
-    if tf.exists():
-        try:
-            rec = json.loads(tf.read_text()).get(w.name)
-            if rec and int(rec.get("hops_per_launch", 0)) == int(round(hops_per_launch)):
-                traffic = rec.get("bytes_per_launch")
-        except Exception:
-            traffic = None
+    # HBM bytes, L2 requests and VALU instructions per hop-kernel launch from the rocprofv3
+    # counter passes of bench_counters.py, used only if they were taken on these sources
+    ctr = counter_record(w.name, hops_per_launch)
+    traffic = ctr["bytes_per_launch"] if ctr else None
+    valu = None
+    if ctr:
+        valu = {
+            "insts_per_wave_hop": ctr["valu_insts_per_wave_hop"],
+            "issue_frac": ctr["valu_insts"] * 4 / (1024 * 2.4e9 * avg_ms * 1e-3),
+            "l2_requests_per_hop": ctr["l2_requests_per_hop"],
+            "note": "SQ_INSTS_VALU / (hops / 64); x 4 cycles / (1,024 SIMDs x 2.4 GHz x this "
+                    "run's launch time); TCC_REQ_sum / hops",
+        }
+    sur_ctr = counter_record("surrogate", None)
+    if sur_ctr and sur_roof:
+        sur_roof["mfma_flop_counted"] = sur_ctr.get("mfma_flop")
+        sur_roof["mfma_busy_cycles"] = sur_ctr.get("mfma_busy_cycles")
+        if sur_ctr.get("mfma_flop"):
+            sur_roof["mfma_tflops_counted"] = sur_ctr["mfma_flop"] / (sur_roof["avg_launch_ms"]
+                                                                      * 1e-3) / 1e12
 
     cpu = None
     if not args.no_cpu:
@@ -588,6 +656,10 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
             "traffic": traffic,
+            "traffic_counters": ("profiles/hop_kernel_counters.json (sources hash "
+                                 f"{ctr['source_hash'][:12]})" if ctr else
+                                 "none taken on these kernel sources"),
+            "valu": valu,
             "bytes_per_hop": bytes_per_hop,
             "visit_weighted_degree": d_v,
             "avg_launch_ms": avg_ms,

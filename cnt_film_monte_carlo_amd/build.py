@@ -45,6 +45,23 @@ def sources() -> list[Path]:
     return sorted(CSRC.glob("*.hip"))
 
 
+def kernel_source_hash() -> str:
+    """SHA-256 over the library's sources (csrc/*.hip, csrc/*.h, include/*.h) and the device
+    flags they are compiled with (include paths left out, so the hash does not depend on where
+    the tree lies).  Hardware-counter records carry it, so that a record taken on other kernels
+    is recognised as such."""
+    import hashlib
+
+    h = hashlib.sha256()
+    flags = [f for f in COMMON if not f.startswith("-I")]
+    h.update(" ".join(flags).encode())
+    files = sorted([*CSRC.glob("*.hip"), *CSRC.glob("*.h"), *(ROOT / "include").glob("*.h")])
+    for f in files:
+        h.update(f.name.encode() + b"\0")
+        h.update(f.read_bytes())
+    return h.hexdigest()
+
+
 def _compile(src: Path) -> Path:
     obj = BUILD / (src.stem + ".o")
     cmd = [HIPCC, *COMMON, "-c", str(src), "-o", str(obj)]

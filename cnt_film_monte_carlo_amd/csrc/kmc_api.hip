@@ -146,6 +146,10 @@ struct kmc_ctx {
   uint64_t count_pending = 0;   // bound on any copy's count since the last fold
   dbuf<uint32_t> d_live;        // excitons able to hop after the last launch
   int64_t run_launches = 0;     // hop launches of the last kmc_run
+  // hop-kernel timing of a timed kmc_run: one (start, stop) event pair per launch
+  std::vector<hipEvent_t> hop_ev;
+  bool time_hops = false;
+  double hop_ms = -1.0;         // summed hop-kernel time of the last timed kmc_run
 
   // MSD records
   int32_t n_bins = 0;
@@ -496,6 +500,8 @@ int kmc_destroy(kmc_ctx* c) {
   dfree(c->c_live_out); dfree(c->c_next); dfree(c->c_err); dfree(c->c_pop_out); dfree(c->c_cur_out);
   dfree(c->c_made); dfree(c->c_skeys); dfree(c->c_sids); dfree(c->c_sort_tmp);
   for (hipEvent_t e : c->c_ev)
+    if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : c->hop_ev)
     if (e) (void)hipEventDestroy(e);
   for (int b = 0; b < 2; ++b) {
     dfree(c->c_site[b]); dfree(c->c_t[b]); dfree(c->c_hops[b]); dfree(c->c_id[b]);
@@ -1141,7 +1147,16 @@ static int run_hops(kmc_ctx* c, hop_args a, int64_t hops, int64_t chunk, bool co
                                                   c->d_keys.p + E, c->d_ids.p, c->d_ids.p + E,
                                                   (int)E, 0, 32, c->stream));
     a.hop_budget = (uint64_t)sub;
+    if (c->time_hops) {
+      while ((int64_t)c->hop_ev.size() < 2 * (launches + 1)) {
+        hipEvent_t ev = nullptr;
+        KMC_HIP(c, hipEventCreate(&ev));
+        c->hop_ev.push_back(ev);
+      }
+      KMC_HIP(c, hipEventRecord(c->hop_ev[2 * launches], c->stream));
+    }
     KMC_HIP(c, launch_hop(a, count, sample, trace, fly, c->stream));
+    if (c->time_hops) KMC_HIP(c, hipEventRecord(c->hop_ev[2 * launches + 1], c->stream));
     done += sub;
     ++launches;
     if (done < hops && live_check_due(launches)) {
@@ -1151,6 +1166,16 @@ static int run_hops(kmc_ctx* c, hop_args a, int64_t hops, int64_t chunk, bool co
     }
   } while (done < hops);
   c->run_launches = launches;
+  if (c->time_hops) {
+    double sum = 0.0;
+    for (int64_t i = 0; i < launches; ++i) {
+      float ms = 0.f;
+      KMC_HIP(c, hipEventSynchronize(c->hop_ev[2 * i + 1]));
+      KMC_HIP(c, hipEventElapsedTime(&ms, c->hop_ev[2 * i], c->hop_ev[2 * i + 1]));
+      sum += ms;
+    }
+    c->hop_ms = sum;
+  }
   if (fly) {
     int32_t flag = 0;
     KMC_TRY(copy_out(c, &flag, c->d_fly_err.p, sizeof(int32_t)));
@@ -1208,8 +1233,12 @@ int kmc_run(kmc_ctx* c, int64_t hops_per_exciton, double t_max, int32_t count_si
   a.tv.tr_time = c->tr_time.p;
   a.tv.tr_n = c->tr_n.p;
   KMC_HIP(c, hipEventRecord(c->ev0, c->stream));
-  KMC_TRY(run_hops(c, a, hops_per_exciton, c->resort_hops, count_sites != 0, c->n_bins > 0,
-                   c->n_trace > 0 && c->tr_max > 0, true));
+  c->time_hops = kernel_ms != nullptr;
+  c->hop_ms = -1.0;
+  const int st = run_hops(c, a, hops_per_exciton, c->resort_hops, count_sites != 0,
+                          c->n_bins > 0, c->n_trace > 0 && c->tr_max > 0, true);
+  c->time_hops = false;
+  if (st != KMC_OK) return st;
   KMC_HIP(c, hipEventRecord(c->ev1, c->stream));
   if (kernel_ms) {
     KMC_HIP(c, hipEventSynchronize(c->ev1));
@@ -1248,6 +1277,14 @@ int kmc_get_hop_counts(kmc_ctx* c, uint64_t* counts) {
   KMC_HIP(c, hipSetDevice(c->cfg.device));
   if (c->count_pending) KMC_TRY(fold_counts(c, c->stream));
   KMC_TRY(copy_out(c, counts, c->d_counts64.p, c->n_sites * sizeof(uint64_t)));
+  return KMC_OK;
+}
+
+int kmc_get_hop_time(kmc_ctx* c, double* hop_kernel_ms) {
+  if (!c || !hop_kernel_ms) return fail(c, KMC_ERR_ARG, "kmc_get_hop_time: bad argument");
+  if (c->hop_ms < 0.0)
+    return fail(c, KMC_ERR_STATE, "kmc_get_hop_time: the last kmc_run was not timed");
+  *hop_kernel_ms = c->hop_ms;
   return KMC_OK;
 }
 

@@ -91,6 +91,7 @@ _SIGS = {
     "kmc_get_excitons": (C.c_int, [_P, _I32, _D, _U64, _D, _U32, _U64]),
     "kmc_get_hop_counts": (C.c_int, [_P, _U64]),
     "kmc_get_run_stats": (C.c_int, [_P, _I64, _I64]),
+    "kmc_get_hop_time": (C.c_int, [_P, _D]),
     "kmc_get_samples": (C.c_int, [_P, _D, _U32, _U32]),
     "kmc_get_trace": (C.c_int, [_P, _I32, _D, _I64]),
     "kmc_reduce_samples": (C.c_int, [_P, C.c_int32, _D, _D]),
@@ -382,11 +383,19 @@ class Context:
                                   C.byref(ms) if timed else None))
         return ms.value
 
-    def run_stats(self):
-        """(hop launches of the last run(), excitons still able to hop after it)."""
-        n, live = C.c_int64(0), C.c_int64(0)
-        self._ck(self.lib.kmc_get_run_stats(self.h, C.byref(n), C.byref(live)))
-        return n.value, live.value
+    def run_stats(self, live: bool = True):
+        """(hop launches of the last run(), excitons still able to hop after it; None for the
+        latter with live=False, which reads nothing from the device)."""
+        n, lv = C.c_int64(0), C.c_int64(0)
+        self._ck(self.lib.kmc_get_run_stats(self.h, C.byref(n), C.byref(lv) if live else None))
+        return n.value, (lv.value if live else None)
+
+    def hop_time(self) -> float:
+        """Device ms of the hop-kernel launches alone in the last timed run() (sorts between
+        sub-launches left out)."""
+        ms = C.c_double(0.0)
+        self._ck(self.lib.kmc_get_hop_time(self.h, C.byref(ms)))
+        return ms.value
 
     def set_hop_schedule(self, resort_hops: int):
         """Hops per sub-launch of run() between exciton sorts (0: one launch)."""

@@ -164,6 +164,11 @@ int kmc_run(kmc_ctx* ctx, int64_t hops_per_exciton, double t_max, int32_t count_
  * not trapped; -1 before the first run).  No reference counterpart (its loop tests the clock,
  * src/main.cpp:64-76). */
 int kmc_get_run_stats(kmc_ctx* ctx, int64_t* launches, int64_t* live_excitons);
+/* After a kmc_run given kernel_ms: the device time of its hop-kernel launches alone (summed
+ * over the sub-launches, sorts and copies between them left out), from HIP events recorded on
+ * the context stream around each launch.  KMC_ERR_STATE if the last run was not timed.  No
+ * reference counterpart (measurement only). */
+int kmc_get_hop_time(kmc_ctx* ctx, double* hop_kernel_ms);
 /* Launch schedule of kmc_run (no reference counterpart; results do not depend on it): the hop
  * budget runs as sub-launches of at most resort_hops hops, each preceded by a sort of the
  * excitons by the memory block of their current site; 0 = one launch (default 625). */
