@@ -1,7 +1,7 @@
 """GPU tests of how a run ends and what it counts: the early end of kmc_run once no exciton can
 hop (the reference's clock-bounded Kubo loop, src/main.cpp:64-76, called with a huge hop budget),
-64-bit departure counts past 2^32 per device copy, and the nearest-node lookup on exact midpoint
-ties in the device CSR build (scattering_struct.h:42-49)."""
+64-bit departure counts past 2^32 per device copy, the nearest-node lookup on exact midpoint
+ties in the device CSR build (scattering_struct.h:42-49), and the hop-kernel timing of a run."""
 import math
 import sys
 
@@ -65,6 +65,30 @@ def test_huge_budget_ends_at_t_max(kmc, orc, nw):
     assert ctx.run_stats() == (1, 0)
     assert np.array_equal(ctx.get_excitons()["hops"], g["hops"])
     ctx.close()
+
+
+def test_hop_kernel_time(kmc, nw):
+    """kmc_get_hop_time: refused before a timed run; after one, positive and at most the
+    whole pass's time (the sorts between the sub-launches are left out of it)."""
+    say("hop time: film_5k_sites 4096 excitons, 2500 hops in 4 sub-launches")
+    w = nw.film_small()
+    net = nw.build_network(w)
+    th, z, a1, a2, rates = nw.forster_tables(1)
+    ctx = nw.make_context(w, net, seed=5)
+    ctx.set_rate_table(th, z, a1, a2, rates)
+    ctx.build_cells()
+    ctx.build_rates(kmc.RATE_TABLE)
+    ctx.inject(4096)
+    with pytest.raises(kmc.KmcError):
+        ctx.hop_time()
+    ms = ctx.run(2500, timed=True)
+    hop = ctx.hop_time()
+    say(f"  pass {ms:.3f} ms, hop kernels {hop:.3f} ms, {ctx.run_stats(live=False)[0]} launches")
+    assert ctx.run_stats(live=False)[0] == 4
+    assert 0.0 < hop <= ms
+    ctx.run(100, timed=False)
+    with pytest.raises(kmc.KmcError):
+        ctx.hop_time()
 
 
 def test_ensemble_huge_budget_ends(kmc, nw):
