@@ -969,11 +969,39 @@ __device__ __forceinline__ bool hop_exciton(const net_view& v, uint32_t e, uint6
 // Every exciton's arithmetic and its order are those of hop_exciton (and of the oracle); only
 // the selection is computed before the clock test that decides whether it is used.  A drain
 // target's header and position are gathered by the lanes concerned.
-// COUNT: departures from the block lines [win_base, win_base + COUNT_WINDOW) are counted in the
-// workgroup's LDS window `win` (zeroed and flushed by the kernel), the rest through a
-// four-entry register LRU into the XCD's copy of the counts.
-constexpr uint32_t COUNT_WINDOW = 2048;
+// COUNT: departures from the block lines [win_base, win_base + COUNT_WINDOW_LINES) are counted
+// in the workgroup's LDS window `win` (zeroed and flushed by the kernel), the rest through a
+// four-entry register LRU into the XCD's copy of the counts.  The window holds 16-bit counters,
+// two lines per word, so that 8 KB of LDS cover 4,096 lines.  A counter never wraps: the
+// departure whose add returns 0x7FFF (taking the counter to 0x8000) subtracts 0x8000 from it
+// again and adds 0x8000 to the line's device count.  Exactly one lane sees each such return,
+// and its subtraction follows its add within a few iterations of the other waves, far fewer
+// than the 0x7FFF departures that would carry into the neighbouring counter.
+constexpr uint32_t COUNT_WINDOW_WORDS = 2048;
+constexpr uint32_t COUNT_WINDOW_LINES = 2 * COUNT_WINDOW_WORDS;
 constexpr uint32_t COUNT_NONE = 0xFFFFFFFFu;
+
+// One departure from window line `off` (block line `line`).
+__device__ __forceinline__ void window_count(uint32_t* win, uint32_t off, uint32_t line,
+                                             uint32_t* __restrict__ cbase, int64_t stride) {
+  const uint32_t sh = (off & 1u) << 4;
+  const uint32_t old = atomicAdd(win + (off >> 1), 1u << sh);
+  if (__builtin_expect(((old >> sh) & 0xFFFFu) == 0x7FFFu, 0)) {
+    atomicSub(win + (off >> 1), 0x8000u << sh);
+    atomicAdd(cbase + count_slot(line, stride), 0x8000u);
+  }
+}
+
+// The workgroup's window into the XCD's copy of the counts, one atomic per nonzero line (after
+// a barrier that follows every window_count of the workgroup).
+__device__ __forceinline__ void window_flush(const uint32_t* win, uint32_t base,
+                                             uint32_t* __restrict__ cbase, int64_t stride) {
+  for (uint32_t i = threadIdx.x; i < COUNT_WINDOW_WORDS; i += blockDim.x) {
+    const uint32_t c = win[i];
+    if (c & 0xFFFFu) atomicAdd(cbase + count_slot(base + 2u * i, stride), c & 0xFFFFu);
+    if (c >> 16) atomicAdd(cbase + count_slot(base + 2u * i + 1u, stride), c >> 16);
+  }
+}
 
 template <bool COUNT, bool SAMPLE, bool TRACE>
 __device__ __forceinline__ void hop_wave(const net_view& v, uint32_t e, bool valid,
@@ -1019,7 +1047,7 @@ __device__ __forceinline__ void hop_wave(const net_view& v, uint32_t e, bool val
   uint32_t count_line = COUNT_NONE;
   auto count_departure = [&]() {
     const uint32_t off = count_line - win_base;
-    if (off < COUNT_WINDOW) atomicAdd(win + off, 1u);
+    if (off < COUNT_WINDOW_LINES) window_count(win, off, count_line, cbase, v.count_stride);
     else cc.add(count_slot(count_line, v.count_stride), cbase);
     count_line = COUNT_NONE;
   };
@@ -1144,7 +1172,7 @@ __device__ __forceinline__ uint32_t xcd_position() {
 template <bool COUNT, bool SAMPLE, bool TRACE>
 __global__ __launch_bounds__(256, 4) void k_hop(hop_args a) {
   __shared__ uint4 rows[4 * 64 * ROW_UNITS];  // 8 KB of staged rows per wave
-  __shared__ uint32_t win[COUNT_WINDOW];       // 8 KB: the workgroup's departure-count window
+  __shared__ uint32_t win[COUNT_WINDOW_WORDS]; // 8 KB: the workgroup's departure-count window
   const uint32_t pos = xcd_position();
   const uint32_t lane = threadIdx.x & 63u;
   const bool wave_live = __builtin_amdgcn_readfirstlane(pos - lane) < a.v.x.n;
@@ -1157,26 +1185,23 @@ __global__ __launch_bounds__(256, 4) void k_hop(hop_args a) {
                                      nullptr, 0u);
   } else {
     // The workgroup's excitons are consecutive in the order sorted by block line, and their
-    // hops in one sub-launch stay mostly among nearby blocks: departures from the COUNT_WINDOW
+    // hops in one sub-launch stay mostly among nearby blocks: departures from the window's
     // lines around the first one's line are summed in LDS and flushed once, per line, at the end.
-    for (uint32_t i = threadIdx.x; i < COUNT_WINDOW; i += blockDim.x) win[i] = 0u;
+    for (uint32_t i = threadIdx.x; i < COUNT_WINDOW_WORDS; i += blockDim.x) win[i] = 0u;
     const uint32_t p0 = pos - threadIdx.x;
     uint32_t base = 0u;
     if (p0 < a.v.x.n) {
       const uint32_t e0 = a.perm ? a.perm[p0] : p0;
       const uint32_t L0 = (uint32_t)a.v.roff[a.v.x.site[e0]];
-      base = L0 > COUNT_WINDOW / 2 ? L0 - COUNT_WINDOW / 2 : 0u;
+      base = L0 > COUNT_WINDOW_LINES / 2 ? L0 - COUNT_WINDOW_LINES / 2 : 0u;
     }
     __syncthreads();
     if (wave_live)
       hop_wave<COUNT, SAMPLE, TRACE>(a.v, e, valid, a.hop_budget, a.t_max, a.tv, lds_wave, lane,
                                      win, base);
     __syncthreads();
-    uint32_t* cbase = a.v.site_count + (size_t)xcd_id() * a.v.count_stride;
-    for (uint32_t i = threadIdx.x; i < COUNT_WINDOW; i += blockDim.x) {
-      const uint32_t c = win[i];
-      if (c) atomicAdd(cbase + count_slot(base + i, a.v.count_stride), c);
-    }
+    window_flush(win, base, a.v.site_count + (size_t)xcd_id() * a.v.count_stride,
+                 a.v.count_stride);
   }
 }
 
@@ -1205,7 +1230,7 @@ __global__ __launch_bounds__(256) void k_hop_fly(hop_args a) {
 template <bool COUNT, bool SAMPLE>
 __global__ __launch_bounds__(256, 4) void k_hop_ens(ens_args a) {
   __shared__ uint4 rows[4 * 64 * ROW_UNITS];  // 8 KB of staged rows per wave
-  __shared__ uint32_t win[COUNT_WINDOW];       // 8 KB: the workgroup's departure-count window
+  __shared__ uint32_t win[COUNT_WINDOW_WORDS]; // 8 KB: the workgroup's departure-count window
   const uint32_t pos = xcd_position();
   const uint32_t p0 = pos - threadIdx.x;       // the workgroup's first position
   if (p0 >= a.n_positions) return;             // the whole workgroup is past the end
@@ -1223,24 +1248,20 @@ __global__ __launch_bounds__(256, 4) void k_hop_ens(ens_args a) {
       hop_wave<COUNT, SAMPLE, false>(v, e, valid, a.hop_budget, a.t_max, trace_view{}, lds_wave,
                                      lane, nullptr, 0u);
   } else {
-    for (uint32_t i = threadIdx.x; i < COUNT_WINDOW; i += blockDim.x) win[i] = 0u;
+    for (uint32_t i = threadIdx.x; i < COUNT_WINDOW_WORDS; i += blockDim.x) win[i] = 0u;
     const int64_t q0 = (int64_t)p0 - a.pbase[net];
     uint32_t base = 0u;
     if (q0 < v.x.n) {
       const uint32_t e0 = (uint32_t)((int64_t)a.perm[eb + q0] - eb);
       const uint32_t L0 = (uint32_t)v.roff[v.x.site[e0]];
-      base = L0 > COUNT_WINDOW / 2 ? L0 - COUNT_WINDOW / 2 : 0u;
+      base = L0 > COUNT_WINDOW_LINES / 2 ? L0 - COUNT_WINDOW_LINES / 2 : 0u;
     }
     __syncthreads();
     if (wave_live)
       hop_wave<COUNT, SAMPLE, false>(v, e, valid, a.hop_budget, a.t_max, trace_view{}, lds_wave,
                                      lane, win, base);
     __syncthreads();
-    uint32_t* cbase = v.site_count + (size_t)xcd_id() * v.count_stride;
-    for (uint32_t i = threadIdx.x; i < COUNT_WINDOW; i += blockDim.x) {
-      const uint32_t c = win[i];
-      if (c) atomicAdd(cbase + count_slot(base + i, v.count_stride), c);
-    }
+    window_flush(win, base, v.site_count + (size_t)xcd_id() * v.count_stride, v.count_stride);
   }
 }
 

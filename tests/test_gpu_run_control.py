@@ -117,7 +117,9 @@ def test_ensemble_huge_budget_ends(kmc, nw):
 def test_counts_beyond_32_bits(kmc):
     """Two sites that are each other's only neighbour: 10^6 excitons x 10^5 hops leave each site
     5e10 times, about 6e9 per XCD copy, past 2^32.  The copies are folded into 64-bit totals
-    before they can wrap, so the counts are exact."""
+    before they can wrap, so the counts are exact.  Each workgroup's 16-bit LDS window counter
+    of either site takes 80,000 departures per 625-hop launch, so the handover of 0x8000 counts
+    to the device copy runs many times as well."""
     E, H = 1_000_000, 100_000
     say(f"ping-pong: 2 sites, {E} x {H} hops")
     pos = np.array([[0.0, 0.0, 0.0], [10 * NM, 0.0, 0.0]])
