@@ -19,6 +19,7 @@
 #include "kmc_mlp.h"
 
 #include <hipcub/device/device_radix_sort.hpp>
+#include <hipcub/device/device_reduce.hpp>
 #include <mutex>
 #include "philox.h"
 
@@ -151,9 +152,11 @@ struct kmc_ctx {
   bool time_hops = false;
   double hop_ms = -1.0;         // summed hop-kernel time of the last timed kmc_run
 
-  // MSD records
+  // MSD records: the grid of n_bins sample times, recorded win_bins at a time from bin0
   int32_t n_bins = 0;
+  int32_t bin0 = 0, win_bins = 0;
   double t_sample_max = 0.0;
+  dbuf<uint32_t> d_min;         // smallest next-sample index, when a window is moved on
   dbuf<double> r_dx, r_dy, r_dz, d_reduced;
   dbuf<uint32_t> r_drain;
 
@@ -283,6 +286,21 @@ exciton_arrays excitons_of(kmc_ctx* c) {
 }
 
 // The hop loop's view of one context (its excitons, blocks, samples and counts).
+// Bins of the open sample window (the last window of the grid may be shorter).
+static inline int32_t window_bins(const kmc_ctx* c) {
+  return std::min(c->win_bins, c->n_bins - c->bin0);
+}
+
+// Zeroes the first nrec records (a window's worth).
+static int clear_records(kmc_ctx* c, size_t nrec) {
+  if (nrec == 0) return KMC_OK;
+  KMC_HIP(c, hipMemsetAsync(c->r_dx.p, 0, nrec * sizeof(double), c->stream));
+  KMC_HIP(c, hipMemsetAsync(c->r_dy.p, 0, nrec * sizeof(double), c->stream));
+  KMC_HIP(c, hipMemsetAsync(c->r_dz.p, 0, nrec * sizeof(double), c->stream));
+  KMC_HIP(c, hipMemsetAsync(c->r_drain.p, 0, nrec * sizeof(uint32_t), c->stream));
+  return KMC_OK;
+}
+
 net_view view_of(kmc_ctx* c) {
   net_view v{};
   v.x = excitons_of(c);
@@ -294,7 +312,8 @@ net_view view_of(kmc_ctx* c) {
   for (int k = 0; k < 3; ++k) {
     v.half_per[k] = (c->cfg.periodic_mask & (1 << k)) ? 0.5 * c->cfg.period_len[k] : INFINITY;
   }
-  v.n_bins = c->n_bins;
+  v.bin0 = c->bin0;
+  v.n_bins = c->n_bins > 0 ? c->bin0 + window_bins(c) : 0;  // the window's end on the grid
   v.dt_sample = c->n_bins > 0 ? c->t_sample_max / c->n_bins : 0.0;
   v.rec_dx = c->r_dx.p;
   v.rec_dy = c->r_dy.p;
@@ -314,7 +333,7 @@ void reset_downstream(kmc_ctx* c, int level) {
   if (level <= 0) c->cells_ok = false;
   if (level <= 1) c->csr_state = 0;
   c->n_exc = 0;
-  c->n_bins = 0;
+  c->n_bins = c->bin0 = c->win_bins = 0;
   c->n_trace = 0;
 }
 
@@ -496,6 +515,7 @@ int kmc_destroy(kmc_ctx* c) {
   dfree(c->x_heading); dfree(c->x_soff); dfree(c->x_tlast); dfree(c->d_links); dfree(c->d_fly_err);
   dfree(c->d_site_count); dfree(c->r_dx); dfree(c->r_dy); dfree(c->r_dz);
   dfree(c->d_reduced); dfree(c->r_drain); dfree(c->tr_site); dfree(c->tr_time); dfree(c->tr_n);
+  dfree(c->d_min);
   dfree(c->c_sites); dfree(c->c_keep); dfree(c->c_off); dfree(c->c_scan_tmp); dfree(c->c_live);
   dfree(c->c_live_out); dfree(c->c_next); dfree(c->c_err); dfree(c->c_pop_out); dfree(c->c_cur_out);
   dfree(c->c_made); dfree(c->c_skeys); dfree(c->c_sids); dfree(c->c_sort_tmp);
@@ -1035,7 +1055,7 @@ int kmc_inject(kmc_ctx* c, int64_t n_excitons) {
   KMC_HIP(c, hipMemsetAsync(c->d_live.p, 0, sizeof(uint32_t), c->stream));
   c->run_launches = 0;
   c->n_exc = E;
-  c->n_bins = 0;
+  c->n_bins = c->bin0 = c->win_bins = 0;
   c->n_trace = 0;
   if (c->velocity > 0.0) KMC_TRY(check_links(c, "kmc_inject"));
   c->exc_velocity = c->velocity;
@@ -1045,27 +1065,80 @@ int kmc_inject(kmc_ctx* c, int64_t n_excitons) {
   return KMC_OK;
 }
 
-int kmc_set_samples(kmc_ctx* c, int32_t n_bins, double t_sample_max) {
-  if (!c) return fail(nullptr, KMC_ERR_ARG, "kmc_set_samples: null context");
-  if (c->n_exc == 0) return fail(c, KMC_ERR_STATE, "kmc_set_samples: call kmc_inject first");
+// Allocates and clears the records of one window of `window` bins (the whole grid when window
+// equals n_bins) and starts it at bin 0; every exciton's next sample is bin 0.
+static int set_samples(kmc_ctx* c, int32_t n_bins, double t_sample_max, int32_t window,
+                       const char* who) {
+  if (!c) return fail(nullptr, KMC_ERR_ARG, std::string(who) + ": null context");
+  if (c->n_exc == 0) return fail(c, KMC_ERR_STATE, std::string(who) + ": call kmc_inject first");
   if (n_bins < 0 || (n_bins > 0 && !(t_sample_max > 0.0 && std::isfinite(t_sample_max))))
-    return fail(c, KMC_ERR_ARG, "kmc_set_samples: need n_bins >= 0 and 0 < t_sample_max < inf");
+    return fail(c, KMC_ERR_ARG,
+                std::string(who) + ": need n_bins >= 0 and 0 < t_sample_max < inf");
+  if (n_bins > 0 && (window < 1 || window > n_bins))
+    return fail(c, KMC_ERR_ARG, std::string(who) + ": need 1 <= window_bins <= n_bins");
   KMC_HIP(c, hipSetDevice(c->cfg.device));
-  const size_t nrec = (size_t)n_bins * c->n_exc;
+  const size_t nrec = (size_t)(n_bins > 0 ? window : 0) * c->n_exc;
   KMC_TRY(dalloc(c, c->r_dx, nrec));
   KMC_TRY(dalloc(c, c->r_dy, nrec));
   KMC_TRY(dalloc(c, c->r_dz, nrec));
   KMC_TRY(dalloc(c, c->r_drain, nrec));
   KMC_HIP(c, hipMemsetAsync(c->x_krec.p, 0, c->n_exc * sizeof(uint32_t), c->stream));
-  if (nrec) {
-    KMC_HIP(c, hipMemsetAsync(c->r_dx.p, 0, nrec * sizeof(double), c->stream));
-    KMC_HIP(c, hipMemsetAsync(c->r_dy.p, 0, nrec * sizeof(double), c->stream));
-    KMC_HIP(c, hipMemsetAsync(c->r_dz.p, 0, nrec * sizeof(double), c->stream));
-    KMC_HIP(c, hipMemsetAsync(c->r_drain.p, 0, nrec * sizeof(uint32_t), c->stream));
-  }
+  KMC_TRY(clear_records(c, nrec));
   KMC_HIP(c, hipStreamSynchronize(c->stream));
   c->n_bins = n_bins;
+  c->bin0 = 0;
+  c->win_bins = n_bins > 0 ? window : 0;
   c->t_sample_max = t_sample_max;
+  return KMC_OK;
+}
+
+int kmc_set_samples(kmc_ctx* c, int32_t n_bins, double t_sample_max) {
+  return set_samples(c, n_bins, t_sample_max, n_bins, "kmc_set_samples");
+}
+
+int kmc_set_samples_window(kmc_ctx* c, int32_t n_bins, double t_sample_max,
+                           int32_t window_bins) {
+  return set_samples(c, n_bins, t_sample_max, window_bins, "kmc_set_samples_window");
+}
+
+int kmc_get_sample_window(kmc_ctx* c, int32_t* first_bin, int32_t* n_window) {
+  if (!c) return fail(nullptr, KMC_ERR_ARG, "kmc_get_sample_window: null context");
+  if (c->n_bins == 0) return fail(c, KMC_ERR_STATE, "kmc_get_sample_window: no sampling grid");
+  if (first_bin) *first_bin = c->bin0;
+  if (n_window) *n_window = window_bins(c);
+  return KMC_OK;
+}
+
+int kmc_next_sample_window(kmc_ctx* c) {
+  if (!c) return fail(nullptr, KMC_ERR_ARG, "kmc_next_sample_window: null context");
+  if (c->n_bins == 0) return fail(c, KMC_ERR_STATE, "kmc_next_sample_window: no sampling grid");
+  const int32_t end = c->bin0 + window_bins(c);
+  if (end >= c->n_bins)
+    return fail(c, KMC_ERR_STATE, "kmc_next_sample_window: the last window is already open");
+  KMC_HIP(c, hipSetDevice(c->cfg.device));
+  // every exciton must have recorded the whole window (its clock past the window's last sample
+  // time), or its records would be lost: the smallest next-sample index decides
+  KMC_TRY(dalloc(c, c->d_min, 1));
+  size_t need = 0;
+  KMC_HIP(c, hipcub::DeviceReduce::Min(nullptr, need, c->x_krec.p, c->d_min.p, (int)c->n_exc,
+                                       c->stream));
+  if (need > c->sort_tmp_bytes) {
+    KMC_TRY(dalloc(c, c->d_sort_tmp, need));
+    c->sort_tmp_bytes = need;
+  }
+  KMC_HIP(c, hipcub::DeviceReduce::Min(c->d_sort_tmp.p, need, c->x_krec.p, c->d_min.p,
+                                       (int)c->n_exc, c->stream));
+  uint32_t lo = 0;
+  KMC_TRY(copy_out(c, &lo, c->d_min.p, sizeof(uint32_t)));
+  if (lo < (uint32_t)end)
+    return fail(c, KMC_ERR_STATE,
+                "kmc_next_sample_window: some exciton has recorded only " + std::to_string(lo) +
+                    " of the " + std::to_string(end) +
+                    " samples up to the window's end (run to t >= " +
+                    std::to_string(c->t_sample_max * end / c->n_bins) + " s first)");
+  c->bin0 = end;
+  KMC_TRY(clear_records(c, (size_t)c->win_bins * c->n_exc));
+  KMC_HIP(c, hipStreamSynchronize(c->stream));
   return KMC_OK;
 }
 
@@ -1304,7 +1377,7 @@ int kmc_get_run_stats(kmc_ctx* c, int64_t* launches, int64_t* live_excitons) {
 int kmc_get_samples(kmc_ctx* c, double* disp, uint32_t* drains, uint32_t* n_recorded) {
   if (!c) return fail(nullptr, KMC_ERR_ARG, "kmc_get_samples: null context");
   if (c->n_bins == 0) return fail(c, KMC_ERR_STATE, "kmc_get_samples: no sampling grid");
-  const size_t nrec = (size_t)c->n_bins * c->n_exc;
+  const size_t nrec = (size_t)window_bins(c) * c->n_exc;
   if (disp) {
     std::vector<double> v(nrec);
     const dbuf<double>* src[3] = {&c->r_dx, &c->r_dy, &c->r_dz};
@@ -1334,11 +1407,12 @@ int kmc_reduce_samples(kmc_ctx* c, int32_t n_batches, double* out, double* kerne
   if (n_batches < 1 || n_batches > c->n_exc || n_batches > 65535)
     return fail(c, KMC_ERR_ARG, "kmc_reduce_samples: need 1 <= n_batches <= min(E, 65535)");
   KMC_HIP(c, hipSetDevice(c->cfg.device));
-  const size_t nout = (size_t)c->n_bins * n_batches * 5;
+  const size_t nout = (size_t)window_bins(c) * n_batches * 5;
   KMC_TRY(dalloc(c, c->d_reduced, nout));
   reduce_args a{};
   a.n_exc = c->n_exc;
-  a.n_bins = c->n_bins;
+  a.bin0 = c->bin0;
+  a.n_bins = window_bins(c);
   a.n_batches = n_batches;
   a.k_rec = c->x_krec.p;
   a.rec_dx = c->r_dx.p;
@@ -1364,10 +1438,10 @@ int kmc_reduce_samples_noalloc(kmc_ctx* c, int32_t n_batches, double* out, doubl
   if (c->n_bins == 0) return fail(c, KMC_ERR_STATE, "kmc_reduce_samples_noalloc: no sampling grid");
   if (n_batches < 1 || n_batches > c->n_exc || n_batches > 65535)
     return fail(c, KMC_ERR_ARG, "kmc_reduce_samples_noalloc: need 1 <= n_batches <= min(E, 65535)");
-  const size_t nout = (size_t)c->n_bins * n_batches * 5;
+  const size_t nout = (size_t)window_bins(c) * n_batches * 5;
   if (nout > REDUCE_SCRATCH_DOUBLES)
     return fail(c, KMC_ERR_CAPACITY,
-                "kmc_reduce_samples_noalloc: n_bins * n_batches * 5 = " + std::to_string(nout) +
+                "kmc_reduce_samples_noalloc: window bins * n_batches * 5 = " + std::to_string(nout) +
                     " exceeds the static scratch of " + std::to_string(REDUCE_SCRATCH_DOUBLES) +
                     " doubles (use kmc_reduce_samples)");
   double* ptr = scratch_of(c->cfg.device);
@@ -1376,7 +1450,8 @@ int kmc_reduce_samples_noalloc(kmc_ctx* c, int32_t n_batches, double* out, doubl
   KMC_HIP(c, hipSetDevice(c->cfg.device));
   reduce_args a{};
   a.n_exc = c->n_exc;
-  a.n_bins = c->n_bins;
+  a.bin0 = c->bin0;
+  a.n_bins = window_bins(c);
   a.n_batches = n_batches;
   a.k_rec = c->x_krec.p;
   a.rec_dx = c->r_dx.p;

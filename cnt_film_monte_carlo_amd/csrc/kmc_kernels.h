@@ -49,8 +49,9 @@ struct net_view {
   uint64_t seed;
   double half_per[3];       // half the period on a periodic axis, +inf on an open one (the
                             // minimum-image displacement; kernel arguments, so they stay scalar)
-  // MSD records [n_bins][x.n]
-  int32_t n_bins;
+  // MSD records of the sample window [bin0, n_bins): sample k (time (k + 1) dt_sample, k the
+  // index on the whole grid) is row k - bin0 of [n_bins - bin0][x.n]
+  int32_t bin0, n_bins;
   double dt_sample;
   double *rec_dx, *rec_dy, *rec_dz;
   uint32_t* rec_drain;
@@ -129,6 +130,7 @@ struct ens_args {
 
 struct reduce_args {
   int64_t n_exc;
+  int32_t bin0;             // grid index of row 0 of the records
   int32_t n_bins, n_batches;
   const uint32_t* k_rec;
   const double *rec_dx, *rec_dy, *rec_dz;

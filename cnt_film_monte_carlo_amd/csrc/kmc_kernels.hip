@@ -842,7 +842,7 @@ __device__ __forceinline__ bool hop_exciton(const net_view& v, uint32_t e, uint6
       while (k < (uint32_t)v.n_bins) {
         const double tk = (double)(k + 1u) * v.dt_sample;
         if (!(tk < t)) break;
-        const size_t r = (size_t)k * E + e;
+        const size_t r = (size_t)(k - (uint32_t)v.bin0) * E + e;
         if (FLY) {  // the flight from the last event up to tk
           flight_state g{R, row.site, row.x, row.y, row.z, so, hd, dx, dy, dz, nd, hs};
           if (!fly(v, g, tk - tl, e, h)) {
@@ -1087,7 +1087,7 @@ __device__ __forceinline__ void hop_wave(const net_view& v, uint32_t e, bool val
     }
     if (SAMPLE && active) {
       while (t_rec < t) {
-        const size_t r = (size_t)k * E + e;
+        const size_t r = (size_t)(k - (uint32_t)v.bin0) * E + e;
         v.rec_dx[r] = dx;
         v.rec_dy[r] = dy;
         v.rec_dz[r] = dz;
@@ -1366,7 +1366,7 @@ __global__ __launch_bounds__(256) void k_reduce_samples(reduce_args a) {
   const int64_t e0 = (int64_t)b * E / a.n_batches, e1 = (int64_t)(b + 1) * E / a.n_batches;
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   for (int64_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
-    if ((uint32_t)k < a.k_rec[e]) {
+    if ((uint32_t)(a.bin0 + k) < a.k_rec[e]) {
       const size_t r = (size_t)k * E + e;
       const double x = a.rec_dx[r], y = a.rec_dy[r], z = a.rec_dz[r];
       acc[0] += 1.0;

@@ -86,6 +86,9 @@ _SIGS = {
     "kmc_set_csr": (C.c_int, [_P, C.c_int64, _I64, _I32, _D, _F]),
     "kmc_inject": (C.c_int, [_P, C.c_int64]),
     "kmc_set_samples": (C.c_int, [_P, C.c_int32, C.c_double]),
+    "kmc_set_samples_window": (C.c_int, [_P, C.c_int32, C.c_double, C.c_int32]),
+    "kmc_get_sample_window": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "kmc_next_sample_window": (C.c_int, [_P]),
     "kmc_set_trace": (C.c_int, [_P, C.c_int64, C.c_int64]),
     "kmc_run": (C.c_int, [_P, C.c_int64, C.c_double, C.c_int32, _D]),
     "kmc_get_excitons": (C.c_int, [_P, _I32, _D, _U64, _D, _U32, _U64]),
@@ -367,10 +370,25 @@ class Context:
         self.n_bins = 0
         self.n_trace = 0
 
-    def set_samples(self, n_bins: int, t_sample_max: float):
-        self._ck(self.lib.kmc_set_samples(self.h, int(n_bins), float(t_sample_max)))
+    def set_samples(self, n_bins: int, t_sample_max: float, window: int = 0):
+        """MSD sample grid of n_bins times up to t_sample_max; with window > 0 recorded `window`
+        bins at a time (records window x excitons instead of n_bins x excitons)."""
+        if window:
+            self._ck(self.lib.kmc_set_samples_window(self.h, int(n_bins), float(t_sample_max),
+                                                     int(window)))
+        else:
+            self._ck(self.lib.kmc_set_samples(self.h, int(n_bins), float(t_sample_max)))
         self.n_bins = int(n_bins)
         self.t_sample_max = float(t_sample_max)
+
+    def sample_window(self):
+        """(first bin, bins) of the open sample window."""
+        a, b = C.c_int32(0), C.c_int32(0)
+        self._ck(self.lib.kmc_get_sample_window(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def next_sample_window(self):
+        self._ck(self.lib.kmc_next_sample_window(self.h))
 
     def set_trace(self, n_trace: int, max_hops: int):
         self._ck(self.lib.kmc_set_trace(self.h, int(n_trace), int(max_hops)))
@@ -497,7 +515,7 @@ class Context:
         return c
 
     def get_samples(self):
-        T, E = self.n_bins, self.n_exc
+        T, E = self.sample_window()[1], self.n_exc
         disp = np.zeros((T, E, 3))
         dr = np.zeros((T, E), np.uint32)
         kr = np.zeros(E, np.uint32)
@@ -512,7 +530,7 @@ class Context:
         return dict(sites=s, times=t, n_hops=n)
 
     def reduce_samples(self, n_batches: int):
-        out = np.zeros((self.n_bins, n_batches, 5))
+        out = np.zeros((self.sample_window()[1], n_batches, 5))
         ms = C.c_double(0.0)
         self._ck(self.lib.kmc_reduce_samples(self.h, int(n_batches), _ptr(out, _D), C.byref(ms)))
         return out, ms.value
@@ -520,7 +538,7 @@ class Context:
     def reduce_samples_noalloc(self, n_batches: int):
         """reduce_samples through the allocation-free entry point (a static device scratch of
         32768 doubles: n_bins * n_batches * 5 must fit)."""
-        out = np.zeros((self.n_bins, n_batches, 5))
+        out = np.zeros((self.sample_window()[1], n_batches, 5))
         ms = C.c_double(0.0)
         self._ck(self.lib.kmc_reduce_samples_noalloc(self.h, int(n_batches), _ptr(out, _D),
                                                      C.byref(ms)))

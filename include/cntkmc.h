@@ -142,6 +142,23 @@ int kmc_inject(kmc_ctx* ctx, int64_t n_excitons);
  * of every exciton is recorded at each t_k it passes (kubo_save_avg_dispalcement_squared,
  * src/monte_carlo/monte_carlo.cpp:382-409, samples on a fixed clock).  Call before kmc_run. */
 int kmc_set_samples(kmc_ctx* ctx, int32_t n_bins, double t_sample_max);
+/* The same grid recorded window_bins bins at a time, so that the records take
+ * window_bins x n_excitons x 28 bytes instead of n_bins x n_excitons x 28 (the reference keeps
+ * one running average per output line, monte_carlo.cpp:382-409).  The first window holds bins
+ * [0, window_bins); kmc_reduce_samples, kmc_reduce_samples_noalloc and kmc_get_samples cover
+ * the open window only (kmc_get_sample_window tells which bins).  Run until every exciton's
+ * clock has passed the window's last sample time ((first + n) * t_sample_max / n_bins, e.g.
+ * kmc_run with that t_max), reduce, then kmc_next_sample_window.  Sample times, records and
+ * reductions are those of kmc_set_samples, bit for bit.  No reference counterpart. */
+int kmc_set_samples_window(kmc_ctx* ctx, int32_t n_bins, double t_sample_max,
+                           int32_t window_bins);
+/* The open window: its first bin on the grid and its number of bins (the last window of the
+ * grid may be shorter than window_bins). */
+int kmc_get_sample_window(kmc_ctx* ctx, int32_t* first_bin, int32_t* n_window);
+/* Opens the next window and clears the records.  KMC_ERR_STATE if some exciton has not yet
+ * recorded every bin of the open window (its records would be lost) or the open window is the
+ * grid's last. */
+int kmc_next_sample_window(kmc_ctx* ctx);
 
 /* Trace buffer: record (site, hop time) of every hop of excitons [0, n_trace) for up to
  * max_hops hops (track_particle, src/monte_carlo/monte_carlo.h:786-818). */
@@ -261,14 +278,17 @@ int kmc_get_excitons(kmc_ctx* ctx, int32_t* site, double* t_next, uint64_t* hops
 /* Per-site departure counts (hops that left each site) since kmc_inject, 64-bit: the device
  * keeps 32-bit per-XCD copies that are folded into 64-bit totals before any copy could wrap. */
 int kmc_get_hop_counts(kmc_ctx* ctx, uint64_t* counts);
-/* Raw MSD records: disp [n_bins][E][3] doubles, drains [n_bins][E], n_recorded [E]. */
+/* Raw MSD records of the open sample window (all n_bins without kmc_set_samples_window):
+ * disp [n][E][3] doubles, drains [n][E], and n_recorded [E], each exciton's count of samples
+ * recorded on the whole grid. */
 int kmc_get_samples(kmc_ctx* ctx, double* disp, uint32_t* drains, uint32_t* n_recorded);
 /* Trace read-back: sites/times [n_trace][max_hops], n_hops [n_trace]. */
 int kmc_get_trace(kmc_ctx* ctx, int32_t* sites, double* times, int64_t* n_hops);
 
 /* Job (d): wavefront-reduced MSD / current histograms.  Excitons are split into n_batches
- * contiguous id ranges; out is [n_bins][n_batches][5] = {count, sum dx^2, sum dy^2, sum dz^2,
- * sum drains} over the excitons whose record k exists.  Replaces the serial averages of
+ * contiguous id ranges; out is [n][n_batches][5] = {count, sum dx^2, sum dy^2, sum dz^2,
+ * sum drains} over the excitons whose record k exists, for the n bins of the open sample window
+ * (all n_bins without kmc_set_samples_window).  Replaces the serial averages of
  * kubo_save_avg_dispalcement_squared (monte_carlo.cpp:382-409) and save_currents
  * (src/monte_carlo/monte_carlo.h:584-643). */
 int kmc_reduce_samples(kmc_ctx* ctx, int32_t n_batches, double* out, double* kernel_ms);

@@ -1,7 +1,8 @@
 """GPU tests of how a run ends and what it counts: the early end of kmc_run once no exciton can
 hop (the reference's clock-bounded Kubo loop, src/main.cpp:64-76, called with a huge hop budget),
 64-bit departure counts past 2^32 per device copy, the nearest-node lookup on exact midpoint
-ties in the device CSR build (scattering_struct.h:42-49), and the hop-kernel timing of a run."""
+ties in the device CSR build (scattering_struct.h:42-49), the hop-kernel timing of a run, and
+MSD records taken window by window."""
 import math
 import sys
 
@@ -89,6 +90,58 @@ def test_hop_kernel_time(kmc, nw):
     ctx.run(100, timed=False)
     with pytest.raises(kmc.KmcError):
         ctx.hop_time()
+
+
+def test_sample_windows_equal_whole_grid(kmc, nw):
+    """kmc_set_samples_window: film_5k_sites recorded 16 bins at a time over a 64-bin grid (the
+    clock run to each window's last sample time, the window reduced, then the next one opened)
+    gives the records, reductions and final state of the whole grid recorded at once, bit for
+    bit, with a quarter of the record memory.  Opening the next window before every exciton has
+    recorded the open one, or after the last, is refused."""
+    say("sample windows: film_5k_sites 3000 excitons, 64 bins as 4 windows of 16")
+    w = nw.film_small()
+    net = nw.build_network(w)
+    th, z, a1, a2, rates = nw.forster_tables(1)
+
+    def fresh():
+        ctx = nw.make_context(w, net, seed=21)
+        ctx.set_rate_table(th, z, a1, a2, rates)
+        ctx.build_cells()
+        ctx.build_rates(kmc.RATE_TABLE)
+        ctx.inject(3000)
+        return ctx
+
+    a = fresh()
+    fin = np.isfinite(a.get_excitons()["t_next"])
+    T = float(np.median(a.get_excitons()["t_next"][fin])) * 200
+    a.set_samples(64, T)
+    a.run(BIG, t_max=T)
+    red_a, rec_a = a.reduce_samples(8)[0], a.get_samples()
+    b = fresh()
+    b.set_samples(64, T, window=16)
+    with pytest.raises(kmc.KmcError):
+        b.next_sample_window()          # nothing recorded yet
+    dt = T / 64
+    for i in range(4):
+        first, n = b.sample_window()
+        assert (first, n) == (16 * i, 16)
+        b.run(BIG, t_max=(first + n) * dt)
+        red_b, rec_b = b.reduce_samples(8)[0], b.get_samples()
+        assert np.array_equal(red_b, red_a[first:first + n])
+        assert np.array_equal(rec_b["disp"], rec_a["disp"][first:first + n])
+        assert np.array_equal(rec_b["drains"], rec_a["drains"][first:first + n])
+        assert np.all(rec_b["n_recorded"] == first + n)
+        if i < 3:
+            b.next_sample_window()
+    with pytest.raises(kmc.KmcError):
+        b.next_sample_window()          # the last window is open
+    ga, gb = a.get_excitons(), b.get_excitons()
+    for k in ("site", "hops", "hash", "drains", "disp", "t_next"):
+        assert np.array_equal(ga[k], gb[k]), k
+    assert np.array_equal(a.get_hop_counts(), b.get_hop_counts())
+    say(f"  {int(gb['hops'].sum())} hops, windows equal the whole grid")
+    a.close()
+    b.close()
 
 
 def test_ensemble_huge_budget_ends(kmc, nw):
