@@ -629,22 +629,24 @@ struct count_cache {
   uint32_t s0 = 0xFFFFFFFFu, s1 = 0xFFFFFFFFu, s2 = 0xFFFFFFFFu, s3 = 0xFFFFFFFFu;
   uint32_t n0 = 0, n1 = 0, n2 = 0, n3 = 0;
 
+  // Selects rather than a branch per entry: the lanes of a wave hit different entries, and a
+  // branch chain would run every arm under partial masks and rotate the eight registers at
+  // its merge.  The entries above the hit (all four on a miss) move down one place.
   __device__ __forceinline__ void add(uint32_t s, uint32_t* __restrict__ base) {
-    if (s == s0) {
-      ++n0;
-    } else if (s == s1) {
-      const uint32_t t = n1;
-      s1 = s0; n1 = n0; s0 = s; n0 = t + 1u;
-    } else if (s == s2) {
-      const uint32_t t = n2;
-      s2 = s1; n2 = n1; s1 = s0; n1 = n0; s0 = s; n0 = t + 1u;
-    } else if (s == s3) {
-      const uint32_t t = n3;
-      s3 = s2; n3 = n2; s2 = s1; n2 = n1; s1 = s0; n1 = n0; s0 = s; n0 = t + 1u;
-    } else {
-      if (n3) atomicAdd(base + s3, n3);
-      s3 = s2; n3 = n2; s2 = s1; n2 = n1; s1 = s0; n1 = n0; s0 = s; n0 = 1u;
-    }
+    const bool h0 = s == s0, h1 = s == s1, h2 = s == s2, h3 = s == s3;
+    if (!(h0 || h1 || h2 || h3) && n3 != 0u) atomicAdd(base + s3, n3);  // the evicted entry
+    // at most one entry holds s (entries are distinct lines; empty ones hold no line), so
+    // the hit's count is a sum of independent selects
+    const uint32_t c = (h0 ? n0 : 0u) + (h1 ? n1 : 0u) + (h2 ? n2 : 0u) + (h3 ? n3 : 0u) + 1u;
+    const bool m1 = !h0, m2 = !(h0 || h1), m3 = !(h0 || h1 || h2);
+    s3 = m3 ? s2 : s3;
+    n3 = m3 ? n2 : n3;
+    s2 = m2 ? s1 : s2;
+    n2 = m2 ? n1 : n2;
+    s1 = m1 ? s0 : s1;
+    n1 = m1 ? n0 : n1;
+    s0 = s;
+    n0 = c;
   }
   __device__ __forceinline__ void flush(uint32_t* __restrict__ base) {
     if (n0) atomicAdd(base + s0, n0);
