@@ -333,13 +333,23 @@ struct infer_args {
 constexpr int PK_VEC = (int)(NQ / 8);      // 6400 A fragments of 16 bytes
 constexpr int SB_N = NH * WD + WD + 1;     // hidden biases, w4, b4
 
-// bias + SiLU of one 32 x 32 accumulator into the B operands of k-blocks 2 ob, 2 ob + 1
-__device__ __forceinline__ void to_operands(const f32x16& acc, const float* bias, int ob, int h,
-                                            bf16x8& lo, bf16x8& hi) {
+// Output block ob of a layer with its biases as the accumulator's start: element r of lane
+// (h, c) of a 32 x 32 accumulator is output unit ob 32 + 8 (r / 4) + 4 h + r % 4 of entry c.
+// The biases are read from LDS where each accumulator starts: an opaque copy of h keeps the
+// compiler from hoisting the loads (256 values per lane) out of the tile loop.
+__device__ __forceinline__ f32x16 bias_acc(const float* bias, int ob, int h) {
+  asm volatile("" : "+v"(h));
+  f32x16 a;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) a[r] = bias[ob * 32 + 8 * (r >> 2) + 4 * h + (r & 3)];
+  return a;
+}
+
+// SiLU of one 32 x 32 accumulator (biases included) into the B operands of k-blocks 2 ob, 2 ob + 1
+__device__ __forceinline__ void to_operands(const f32x16& acc, bf16x8& lo, bf16x8& hi) {
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
-    const float v = acc[r] + bias[ob * 32 + 8 * (r >> 2) + 4 * h + (r & 3)];
-    const bf16 s = (bf16)silu_f(v);
+    const bf16 s = (bf16)silu_f(acc[r]);
     if (r < 8) lo[r] = s;
     else hi[r - 8] = s;
   }
@@ -354,16 +364,18 @@ __device__ __forceinline__ void hidden_layer(const bf16x8* s_w, const float* s_b
   bf16x8 y[8];
 #pragma unroll
   for (int ob = 0; ob < 4; ++ob) {
-    f32x16 acc = {};
+    f32x16 acc = bias_acc(s_b + L * WD, ob, h);
 #pragma unroll
     for (int kb = 0; kb < 8; ++kb) acc = mfma32(s_w[q_off(L) / 8 + (ob * 8 + kb) * 64 + lane], x[kb], acc);
-    to_operands(acc, s_b + L * WD, ob, h, y[2 * ob], y[2 * ob + 1]);
+    to_operands(acc, y[2 * ob], y[2 * ob + 1]);
   }
 #pragma unroll
   for (int kb = 0; kb < 8; ++kb) x[kb] = y[kb];
 }
 
-constexpr int INFER_WAVES = 8;  // waves per workgroup sharing one LDS copy of the weights
+// Waves per workgroup sharing one LDS copy of the weights.  The 103 KB copy allows one workgroup
+// per CU, and the kernel's 168 VGPRs allow three waves per SIMD: twelve waves fill both.
+constexpr int INFER_WAVES = 12;
 
 // One wave per tile of 32 entries (the 32 columns of the MFMA tiles); the workgroup's waves
 // share the LDS copy of the weights and stride over the tiles.
@@ -420,9 +432,8 @@ __global__ __launch_bounds__(64 * INFER_WAVES) void k_mlp_infer(infer_args a) {
       for (int k = 0; k < 8; ++k) x0[k] = (bf16)f[k];
 #pragma unroll
       for (int ob = 0; ob < 4; ++ob) {
-        const f32x16 zero = {};
-        const f32x16 acc = mfma32(s_w[ob * 64 + wl0], x0, zero);
-        to_operands(acc, s_b, ob, h, x[2 * ob], x[2 * ob + 1]);
+        const f32x16 acc = mfma32(s_w[ob * 64 + wl0], x0, bias_acc(s_b, ob, h));
+        to_operands(acc, x[2 * ob], x[2 * ob + 1]);
       }
     }
     hidden_layer<1>(s_w, s_b, lane, h, x);
@@ -433,13 +444,13 @@ __global__ __launch_bounds__(64 * INFER_WAVES) void k_mlp_infer(infer_args a) {
     asm volatile("" : "+v"(wl));
 #pragma unroll
     for (int ob = 0; ob < 4; ++ob) {
-      f32x16 acc = {};
+      f32x16 acc = bias_acc(s_b + 3 * WD, ob, h);
 #pragma unroll
       for (int kb = 0; kb < 8; ++kb) acc = mfma32(s_w[q_off(3) / 8 + (ob * 8 + kb) * 64 + wl], x[kb], acc);
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int u = ob * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
-        ys += (float)(bf16)silu_f(acc[r] + s_b[3 * WD + u]) * s_b[NH * WD + u];
+        ys += (float)(bf16)silu_f(acc[r]) * s_b[NH * WD + u];
       }
     }
     const float yy = ys + __shfl_xor(ys, 32, 64) + s_b[NH * WD + WD];
@@ -469,6 +480,7 @@ int mfail(int code, const std::string& m) {
 
 struct kmc_mlp {
   int device = 0;
+  int n_cu = 1;  // compute units of the device (grid of the inference kernel)
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   domain_consts dom{};
@@ -659,10 +671,12 @@ infer_args base_infer(const kmc_mlp* m, double T) {
   return a;
 }
 
-hipError_t launch_infer(const infer_args& a, hipStream_t st) {
+// At most one workgroup per CU, each striding over the tiles, so that every CU copies the
+// weights into LDS once per launch.
+hipError_t launch_infer(const infer_args& a, int n_cu, hipStream_t st) {
   if (a.n == 0) return hipSuccess;
   const int64_t tiles = (a.n + 31) / 32;
-  const unsigned grid = (unsigned)std::min<int64_t>((tiles + INFER_WAVES - 1) / INFER_WAVES, 1024);
+  const unsigned grid = (unsigned)std::min<int64_t>((tiles + INFER_WAVES - 1) / INFER_WAVES, n_cu);
   hipLaunchKernelGGL(k_mlp_infer, dim3(grid), dim3(64 * INFER_WAVES), 0, st, a);
   return hipGetLastError();
 }
@@ -730,7 +744,7 @@ hipError_t mlp_launch_pairs(const kmc_mlp* m, int64_t n, const double* theta, co
     a.cmap[k] = (int8_t)(chir_class[k] >= 0 && chir_class[k] < NCLS ? chir_class[k] : -1);
   a.rate = rate;
   a.flag = flag;
-  return launch_infer(a, st);
+  return launch_infer(a, m->n_cu, st);
 }
 
 }  // namespace cntkmc
@@ -763,6 +777,7 @@ int kmc_mlp_create(int32_t device, const kmc_mlp_domain* dom, kmc_mlp** out) {
   m->dom.y_shift = dom->y_shift;
   m->dom.y_scale = dom->y_scale;
   hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&m->n_cu, hipDeviceAttributeMultiprocessorCount, device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&m->ev0);
   if (e == hipSuccess) e = hipEventCreate(&m->ev1);
@@ -1019,7 +1034,7 @@ int kmc_mlp_rates_device(kmc_mlp* m, int64_t n, const double* d_theta, const dou
   a.cls_a = cls_a >= 0 && cls_a < NCLS ? cls_a : -1;
   a.rate = d_rate;
   a.flag = d_flag;
-  MLP_HIP(launch_infer(a, m->stream));
+  MLP_HIP(launch_infer(a, m->n_cu, m->stream));
   MLP_HIP(hipStreamSynchronize(m->stream));
   return KMC_OK;
 }
@@ -1044,7 +1059,7 @@ int kmc_mlp_rates(kmc_mlp* m, int64_t n, const double* th, const double* z, cons
   a.rate = m->d_rate;
   a.flag = m->d_flag;
   MLP_HIP(hipEventRecord(m->ev0, m->stream));
-  MLP_HIP(launch_infer(a, m->stream));
+  MLP_HIP(launch_infer(a, m->n_cu, m->stream));
   MLP_HIP(hipEventRecord(m->ev1, m->stream));
   MLP_HIP(hipMemcpyAsync(rate, m->d_rate, n * sizeof(double), hipMemcpyDeviceToHost, m->stream));
   if (flag) MLP_HIP(hipMemcpyAsync(flag, m->d_flag, n, hipMemcpyDeviceToHost, m->stream));
