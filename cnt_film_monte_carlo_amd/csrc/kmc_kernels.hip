@@ -537,15 +537,8 @@ __device__ __forceinline__ coop_row coop_finish(const uint4* lds_wave, uint32_t 
   return r;
 }
 
-// Both in one: the row of W, in registers.
-__device__ __forceinline__ coop_row coop_load_row(const uint4* __restrict__ blocks, uint32_t W,
-                                                  uint4* lds_wave, uint32_t lane) {
-  coop_issue(blocks, W, lds_wave, lane);
-  return coop_finish(lds_wave, lane);
-}
-
 // Target word idx < ROW_INLINE of the lane's row from the wave's LDS copy (units 5 and 6,
-// rotated by the lane as coop_load_row stores them).  Valid until the next coop_load_row.
+// rotated by the lane as coop_issue stores them).  Valid until the next coop_issue.
 __device__ __forceinline__ uint32_t target_lds(const uint4* lds_wave, uint32_t lane, uint32_t idx) {
   const uint32_t slot = (5u + (idx >> 2) + lane) & 7u;
   return reinterpret_cast<const uint32_t*>(lds_wave + 8u * lane + slot)[idx & 3u];
@@ -1530,9 +1523,11 @@ __global__ __launch_bounds__(256) void k_contact_keys(int64_t capacity, const in
 }
 
 // particle::step of every live particle for v = 0, built like the Kubo kernel's wave loop around
-// coop_load_row: every lane of a wave takes part in every row load, and the wave loops until no
-// lane has a hop due by the step's end (a lane whose particle has stopped keeps loading its
-// row).  Each particle's arithmetic and its order are those of the oracle's step.  The live
+// the cooperative row load: every lane of a wave takes part in every row load, and the wave
+// loops until no lane has a hop due by the step's end (a lane whose particle has stopped keeps
+// loading its row).  The next hop is selected as soon as a row arrives and its row load issued
+// before the dwell at the current site is drawn and the clock tested, so the load is in flight
+// meanwhile.  Each particle's arithmetic and its order are those of the oracle's step.  The live
 // slots, sorted by row, are dealt to the XCDs in eighths (from the device-side live count), so
 // each XCD's L2 serves one compact region of the film.  The grid is a multiple of 8 workgroups
 // covering the capacity.
@@ -1573,14 +1568,33 @@ __global__ __launch_bounds__(256, 4) void k_contact_wave(contact_args a) {
       W = row_word(L0, a.blocks[(size_t)ROW_UNITS * L0].x);
     }
     bool active = valid, pend = false;
+    uint32_t site_now = 0;  // when stopped: the site it stopped at (the row loaded after that
+                            // may be the one of the hop selected but not made)
     hop_words hw;
-    coop_row row;
+    coop_issue(a.blocks, W, lds_wave, lane);
     while (true) {
-      row = coop_load_row(a.blocks, W, lds_wave, lane);
+      const coop_row row = coop_finish(lds_wave, lane);
+      // (A) the hop from this site, selected before the clock test that decides whether it is
+      // made, and the load of the row it leads to, in flight while (B) and (C) run
+      const bool can = active && !trapped(row.n, row.gamma) && made < a.max_hops;
+      uint32_t W_next = W, r_next = 0;
+      if (can) {
+        uint32_t w_sel, w_dw;
+        hw.get(a.seed, id, h, w_sel, w_dw);
+        const double dice = div_rand_max(row.gamma * (double)r31(w_sel));
+        const uint32_t idx = select_entry(a.blocks, W, row.c, row.n, dice);
+        const uint32_t tg = idx < ROW_INLINE ? target_lds(lds_wave, lane, idx)
+                                             : target_wide(a.blocks, W, row.n, idx);
+        W_next = tg & ~EDGE_DRAIN_BIT;
+        r_next = nonzero_draw1(w_dw, a.seed, id, h, STREAM_HOP_REDRAW);
+      }
+      coop_issue(a.blocks, W_next, lds_wave, lane);
+      // (B) the dwell at this site of the hop that led here
       if (pend) {
-        t = t + dwell_time(row.n, row.gamma, r_dw);  // dwell at the new site
+        t = t + dwell_time(row.n, row.gamma, r_dw);
         pend = false;
       }
+      // (C) the hop of (A) is made if it falls due by the step's end
       if (active) {
         if (!(t <= a.t_end) || trapped(row.n, row.gamma)) {
           active = false;
@@ -1588,25 +1602,22 @@ __global__ __launch_bounds__(256, 4) void k_contact_wave(contact_args a) {
           atomicOr(a.err, CONTACT_ERR_HOPS);
           active = false;
         }
+        if (!active) site_now = row.site;
       }
-      if (__ballot(active) == 0) break;
       if (active) {
-        uint32_t w_sel, w_dw;
-        hw.get(a.seed, id, h, w_sel, w_dw);
-        const double dice = div_rand_max(row.gamma * (double)r31(w_sel));
-        const uint32_t idx = select_entry(a.blocks, W, row.c, row.n, dice);
-        const uint32_t tg = idx < ROW_INLINE ? target_lds(lds_wave, lane, idx)
-                                             : target_wide(a.blocks, W, row.n, idx);
-        W = tg & ~EDGE_DRAIN_BIT;
-        r_dw = nonzero_draw1(w_dw, a.seed, id, h, STREAM_HOP_REDRAW);
+        W = W_next;
+        r_dw = r_next;
         pend = true;
         ++h;
         ++made;
       }
+      if (__ballot(active) == 0) break;
     }
+    // the last issued load lands in this wave's LDS rows: wait for it before the wave moves on
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     if (valid) {
-      const double y_new = a.py[row.site];
-      a.s.site[e] = (int32_t)row.site;
+      const double y_new = a.py[site_now];
+      a.s.site[e] = (int32_t)site_now;
       a.s.t_next[e] = t;
       a.s.hops[e] = h;
       a.s.y[e] = y_new;
