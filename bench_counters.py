@@ -148,6 +148,21 @@ def trace_stats(d: Path) -> dict[str, dict]:
     return out
 
 
+def trace_largest_grid(d: Path, kernel: str) -> dict | None:
+    """Mean duration of the dispatches of `kernel` with the largest grid in the kernel trace
+    (the surrogate's 4M-pair roofline batch, not the table build's smaller calls)."""
+    rows = []
+    for f in glob.glob(str(d / "**" / "*kernel_trace.csv"), recursive=True):
+        with open(f) as fh:
+            rows += [r for r in csv.DictReader(fh) if kernel in r.get("Kernel_Name", "")]
+    if not rows:
+        return None
+    g = max(int(r["Grid_Size_X"]) for r in rows)
+    ms = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-6 for r in rows
+          if int(r["Grid_Size_X"]) == g]
+    return {"calls": len(ms), "avg_ms": sum(ms) / len(ms), "grid": g}
+
+
 def _stat(stats: dict, key: str) -> dict | None:
     hits = [v for k, v in stats.items() if key in k]
     return hits[0] if len(hits) == 1 else None
@@ -170,7 +185,7 @@ def summarize(out: Path) -> dict:
     sb = per_dispatch(out / "sq_b", HOP_KERNEL)
     stats = trace_stats(out / "trace")
     hop_t = _stat(stats, HOP_KERNEL)
-    mlp_t = _stat(stats, MLP_KERNEL)
+    mlp_t = trace_largest_grid(out / "trace", MLP_KERNEL)
     valu = sa["SQ_INSTS_VALU"]
     rec = {
         "source_hash": src_hash,
@@ -224,6 +239,7 @@ def summarize(out: Path) -> dict:
         }
         if mlp_t:
             mlp["trace_avg_launch_ms"] = mlp_t["avg_ms"]
+            mlp["trace_launches"] = mlp_t["calls"]
         if mops is not None and mlp_t:
             mlp["mfma_tflops"] = 512.0 * mops / (mlp_t["avg_ms"] * 1e-3) / 1e12
     res = {

@@ -51,7 +51,15 @@ def _fake_collection(out: Path, src_hash: str):
         w = csv.writer(f)
         w.writerow(["Name", "Calls", "TotalDurationNs", "AverageNs"])
         w.writerow([HOP, 10, 20_000_000, 2_000_000])
-        w.writerow([MLP, 2, 2_000_000, 1_000_000])
+        w.writerow([MLP, 3, 3_000_000, 1_000_000])
+    # the inference kernel's own time: the dispatches of the largest grid only (3 ms and 1 ms,
+    # not the 0.5 ms of a smaller call)
+    with open(out / "trace" / "run_kernel_trace.csv", "w", newline="") as f:
+        w = csv.writer(f)
+        w.writerow(["Kernel_Name", "Start_Timestamp", "End_Timestamp", "Grid_Size_X"])
+        w.writerow([MLP, 0, 500_000, 768])
+        w.writerow([MLP, 1_000_000, 4_000_000, 196_608])
+        w.writerow([MLP, 5_000_000, 6_000_000, 196_608])
 
 
 def test_summary_of_synthetic_counters(tmp_path):
@@ -73,6 +81,8 @@ def test_summary_of_synthetic_counters(tmp_path):
     m = res["surrogate"]
     assert m["mfma_mops_bf16"] == 1000.0                              # the largest grid only
     assert m["mfma_flop"] == 512_000.0
+    assert m["trace_avg_launch_ms"] == pytest.approx(2.0) and m["trace_launches"] == 2
+    assert m["mfma_tflops"] == pytest.approx(512_000.0 / 2e-3 / 1e12)
 
 
 def test_bench_refuses_stale_counter_records(tmp_path, monkeypatch):
