@@ -116,12 +116,14 @@ constexpr int MAX_CHIR = 16;
 constexpr int64_t COUNT_ALIGN = 1024;  // each copy starts on a 4 KB boundary
 // Departure counts are kept per block line (the first line of the departure site's block, in
 // the Morton placement of the blocks), so the counters an XCD's excitons touch -- one compact
-// region of the film -- form a compact, L2-resident range.  Line L has slot L of each copy, so
-// the counters of neighbouring blocks share cache lines: the atomics of departures that miss
-// the workgroup's LDS window and the register cache find their line in the L2 more often
-// (measured 1 % faster per hop launch than slots that put lines 32 apart in one cache line).
-__host__ __device__ __forceinline__ uint32_t count_slot(uint32_t L, int64_t /*count_stride*/) {
-  return L;
+// region of the film -- form a compact, L2-resident range.  Slot of line L in a copy of
+// count_stride (a multiple of 32) counters: the 32 counters of a 128-byte line hold lines 32
+// apart, L -> (L % 32) stride / 32 + L / 32, so that excitons hopping among neighbouring blocks
+// spread their atomics over many cache lines and L2 channels.  (Slot L for line L, which puts
+// neighbouring blocks in one cache line, made a hop launch 1 % faster on a film of 1.6 excitons
+// per site and 1.8 times slower on one of 10 per site, whose departures crowd a region's lines.)
+__host__ __device__ __forceinline__ uint32_t count_slot(uint32_t L, int64_t count_stride) {
+  return (L & 31u) * (uint32_t)(count_stride >> 5) + (L >> 5);
 }
 
 struct __align__(16) site_hdr {
