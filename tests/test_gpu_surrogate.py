@@ -187,3 +187,27 @@ def test_reversing_both_tubes_gives_the_same_rate(S, net):
     r0, _, _ = s.rates(th, z, a1, a2, 0, 0, 300.0)
     r1, _, _ = s.rates(th, z, -a1, -a2, 0, 0, 300.0)
     np.testing.assert_array_equal(r0, r1)
+
+
+def test_inference_many_tiles_per_wave(S, MO, net):
+    """300,000 pairs are 9,375 tiles of 32: more than one per wave of the one-workgroup-per-CU
+    grid, so each wave strides over several tiles.  Every rate equals the one the same kernel
+    gives on chunks of 2,000 pairs (one tile per wave), bit for bit, and a sample follows the
+    oracle forward to the bar of the fused kernel against the training forward (5e-3, median
+    2e-4): the fused kernel sums in its own order."""
+    s, p = net
+    rng = np.random.default_rng(10)
+    n = 300_000
+    th, z, a1, a2 = geometries(rng, n, S)
+    big, fl, ms = s.rates(th, z, a1, a2, 1, 2, 250.0)
+    say(f"many tiles: {n} pairs in {ms:.3f} ms")
+    parts = [s.rates(th[i:i + 2000], z[i:i + 2000], a1[i:i + 2000], a2[i:i + 2000], 1, 2, 250.0)
+             for i in range(0, n, 2000)]
+    np.testing.assert_array_equal(big, np.concatenate([q[0] for q in parts]))
+    np.testing.assert_array_equal(fl, np.concatenate([q[1] for q in parts]))
+    k = rng.choice(n, 500, replace=False)
+    f = s.features(th[k], z[k], a1[k], a2[k], np.ones(500, int), np.full(500, 2), np.full(500, 250.0))
+    yo, _ = MO.forward(p.astype(np.float64), f.astype(np.float64), bf=True)
+    dy = np.abs(np.log10(big[k]) - s.log10_of(yo)) / s.domain.y_scale
+    say(f"  vs oracle: max |dy| {dy.max():.2e}, median {np.median(dy):.2e}")
+    assert dy.max() <= 5e-3 and np.median(dy) <= 2e-4
