@@ -581,8 +581,11 @@ def main():
             "insts_per_wave_hop": ctr["valu_insts_per_wave_hop"],
             "issue_frac": ctr["valu_insts"] * 4 / (1024 * 2.4e9 * avg_ms * 1e-3),
             "l2_requests_per_hop": ctr["l2_requests_per_hop"],
+            "wave_wait_frac": (ctr["wait_any"] / ctr["wave_cycles"]
+                               if ctr.get("wait_any") and ctr.get("wave_cycles") else None),
             "note": "SQ_INSTS_VALU / (hops / 64); x 4 cycles / (1,024 SIMDs x 2.4 GHz x this "
-                    "run's launch time); TCC_REQ_sum / hops",
+                    "run's launch time); TCC_REQ_sum / hops; SQ_WAIT_ANY / SQ_WAVE_CYCLES (the "
+                    "share of a wave's resident time spent waiting, mostly on its row load)",
         }
     sur_ctr = counter_record("surrogate", None)
     if sur_ctr and sur_roof:
