@@ -1163,10 +1163,17 @@ int kmc_set_trace(kmc_ctx* c, int64_t n_trace, int64_t max_hops) {
   return KMC_OK;
 }
 
-// Departure counts: each sub-launch adds at most E x (its hops) to any slot of a 32-bit copy,
-// so launches are capped at UINT32_MAX / E hops while counting, and the copies are folded into
-// the 64-bit totals before a launch that could take `pending` past UINT32_MAX.
+// Departure counts: each sub-launch adds at most L x (its hops) to any slot of a 32-bit copy,
+// L the excitons a launch could still move (E, or fewer once the live count has been read), so
+// launches are capped at UINT32_MAX / L hops while counting, and the copies are folded into the
+// 64-bit totals before a launch that could take `pending` past UINT32_MAX.
 constexpr uint64_t COUNT_COPY_MAX = 0xFFFFFFFFull;
+
+// Once at most E / TAIL_FRACTION excitons can still move (a clock-bounded run whose remaining
+// excitons have long hop chains), sub-launches grow TAIL_STRETCH-fold: the few chains left
+// gain nothing from the sort, and each launch's sort and start-up would otherwise cost about as
+// much as its hops.
+constexpr int64_t TAIL_FRACTION = 64, TAIL_STRETCH = 16;
 
 // Reads of the device's live-exciton count (each a host synchronisation) after launches 1, 2,
 // 4 and every 8th: a run stops early once no exciton can hop (all past t_max or trapped).
@@ -1203,13 +1210,18 @@ static int run_hops(kmc_ctx* c, hop_args a, int64_t hops, int64_t chunk, bool co
     c->sort_tmp_bytes = need;
   }
   a.perm = c->d_ids.p + E;
-  int64_t step = chunk > 0 ? chunk : std::max<int64_t>(hops, 1);
-  if (count) step = std::min<int64_t>(step, (int64_t)(COUNT_COPY_MAX / (uint64_t)E));
+  const int64_t step = chunk > 0 ? chunk : std::max<int64_t>(hops, 1);
   int64_t done = 0, launches = 0;
+  int64_t movable = E;  // an upper bound on the excitons a launch can move (the last live count)
   do {
-    const int64_t sub = std::min(step, hops - done);
+    int64_t len = step;
+    if (chunk > 0 && movable * TAIL_FRACTION <= E && step <= INT64_MAX / TAIL_STRETCH)
+      len = step * TAIL_STRETCH;
+    if (count)
+      len = std::min<int64_t>(len, (int64_t)(COUNT_COPY_MAX / (uint64_t)std::max<int64_t>(movable, 1)));
+    const int64_t sub = std::min(len, hops - done);
     if (count && fold) {
-      const uint64_t add = (uint64_t)E * (uint64_t)sub;
+      const uint64_t add = (uint64_t)movable * (uint64_t)sub;
       if (c->count_pending + add > COUNT_COPY_MAX) KMC_TRY(fold_counts(c, c->stream));
       c->count_pending += add;
     }
@@ -1236,6 +1248,7 @@ static int run_hops(kmc_ctx* c, hop_args a, int64_t hops, int64_t chunk, bool co
       uint32_t live = 0;
       KMC_TRY(copy_out(c, &live, c->d_live.p, sizeof(uint32_t)));
       if (live == 0u) break;
+      movable = std::min<int64_t>(movable, (int64_t)live);
     }
   } while (done < hops);
   c->run_launches = launches;

@@ -1065,7 +1065,9 @@ __device__ __forceinline__ void hop_wave(const net_view& v, uint32_t e, bool val
       W_next = (tg & EDGE_DRAIN_BIT) ? v.src_roff[r31(draw4(v.seed, e, h, STREAM_REINJECT).v[0]) % v.n_src]
                                      : (tg & ~EDGE_DRAIN_BIT);
     }
-    coop_issue(v.blocks, W_next, lds_wave, lane);
+    // a lane whose exciton has stopped never reads its row again: it loads block line 0, which
+    // every such lane of the wave shares, instead of its own row
+    coop_issue(v.blocks, active ? W_next : 0u, lds_wave, lane);
     // (B) the hop into this site finishes: dwell at the new site (particle.cpp:183 -> ff_time)
     if (pend) {
       dx = dx + min_image((double)row.x - (double)px, v.half_per[0]);
@@ -1525,7 +1527,7 @@ __global__ __launch_bounds__(256) void k_contact_keys(int64_t capacity, const in
 // particle::step of every live particle for v = 0, built like the Kubo kernel's wave loop around
 // the cooperative row load: every lane of a wave takes part in every row load, and the wave
 // loops until no lane has a hop due by the step's end (a lane whose particle has stopped keeps
-// loading its row).  The next hop is selected as soon as a row arrives and its row load issued
+// taking part in the loads).  The next hop is selected as soon as a row arrives and its row load issued
 // before the dwell at the current site is drawn and the clock tested, so the load is in flight
 // meanwhile.  Each particle's arithmetic and its order are those of the oracle's step.  The live
 // slots, sorted by row, are dealt to the XCDs in eighths (from the device-side live count), so
@@ -1588,7 +1590,9 @@ __global__ __launch_bounds__(256, 4) void k_contact_wave(contact_args a) {
         W_next = tg & ~EDGE_DRAIN_BIT;
         r_next = nonzero_draw1(w_dw, a.seed, id, h, STREAM_HOP_REDRAW);
       }
-      coop_issue(a.blocks, W_next, lds_wave, lane);
+      // a lane whose particle has stopped never reads its row again: it loads block line 0,
+      // which every such lane of the wave shares, instead of its own row
+      coop_issue(a.blocks, active ? W_next : 0u, lds_wave, lane);
       // (B) the dwell at this site of the hop that led here
       if (pend) {
         t = t + dwell_time(row.n, row.gamma, r_dw);

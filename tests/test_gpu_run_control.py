@@ -34,7 +34,9 @@ def _compare_state(g, o, label):
 
 def test_huge_budget_ends_at_t_max(kmc, orc, nw):
     """kmc_run(ctx, INT64_MAX / 2, t_cap) with the default schedule returns once every exciton
-    is past t_cap, bit-equal to the oracle's run to the same clock, after a handful of launches."""
+    is past t_cap, bit-equal to the oracle's run to the same clock.  Launches are 625 hops until
+    a read-back finds at most 1/64 of the excitons still able to move, 10,000 hops after that,
+    so the few long chains of this film's tail take fewer launches than 625-hop ones would."""
     say("huge budget: film_5k_sites 1000 excitons, t_max cap, budget 2^62")
     w = nw.film_small()
     net = nw.build_network(w)
@@ -51,12 +53,16 @@ def test_huge_budget_ends_at_t_max(kmc, orc, nw):
     ctx.run(BIG, t_max=t_cap)
     launches, live = ctx.run_stats()
     g = ctx.get_excitons()
-    # launches of 625 hops until every exciton is past t_cap, read back after launches 1, 2,
-    # 4 and every 8th: at most 8 launches beyond the longest exciton's hops
-    need = -(-int(g["hops"].max()) // 625)
-    say(f"  {launches} launches ({need} needed, longest {int(g['hops'].max())} hops), "
+    # live counts read back after launches 1, 2, 4 and every 8th: at most 8 launches beyond the
+    # longest exciton's hops at 625 per launch, at least its hops at 10,000 per launch, and
+    # fewer than 625-hop launches would need once the tail is long
+    longest = int(g["hops"].max())
+    need = -(-longest // 625)
+    say(f"  {launches} launches ({need} of 625 hops for the longest, {longest} hops), "
         f"{live} excitons live")
-    assert live == 0 and need <= launches <= need + 8
+    assert live == 0 and -(-longest // 10_000) <= launches <= need + 8
+    if need > 64:
+        assert launches < need
     ox.run(BIG, t_max=t_cap)
     _compare_state(g, ox, "huge budget")
     assert np.all(g["t_next"] > t_cap)
