@@ -187,7 +187,7 @@ struct kmc_ctx {
   dbuf<uint64_t> c_next;
   dbuf<int32_t> c_err, c_pop_out, c_cur_out;
   dbuf<uint64_t> c_made;                  // hops made since kmc_contact_init
-  dbuf<uint32_t> c_skeys, c_sids;         // [2 capacity] slot sort by row: keys, slot ids
+  dbuf<uint32_t> c_skeys, c_sids;         // [2 capacity] slot sort (contact_key_layout_of): keys, ids
   dbuf<uint8_t> c_sort_tmp;
   double c_step_ms = 0.0;                 // device time of the step kernels since then
   std::vector<hipEvent_t> c_ev;           // CONTACT_EV_PAIRS (start, stop) pairs, reused
@@ -1871,6 +1871,38 @@ contact_slots contact_buf(kmc_ctx* c, int b) {
                        c->c_y[b].p,    c->c_head[b].p, c->c_soff[b].p, c->c_tlast[b].p};
 }
 
+// The order in which a step's threads take the live slots.  Slots sort by the eighth of the film
+// their site lies in (the eighths of the sorted slots go to the eight XCDs, so each XCD's L2
+// serves one compact region), then by the site's chirality, then by the site's row.  The
+// chirality sets the rates a particle meets along its tube, so the particles of one wave make
+// similar numbers of hops in a step and fewer of its lanes idle while its last particle hops.
+// A film whose keys would not fit 32 bits sorts by the row alone.
+contact_key_layout contact_key_layout_of(const kmc_ctx* c, uint32_t lines) {
+  const auto bits = [](uint64_t v) {  // bits that hold the values [0, v]
+    int b = 1;
+    while (b < 64 && (v >> b) != 0) ++b;
+    return b;
+  };
+  contact_key_layout kl{};
+  kl.lines = lines;
+  uint8_t top = 0;
+  for (const uint8_t ch : c->h_chir) top = std::max(top, ch);
+  kl.chir_bits = top ? bits(top) : 0;
+  kl.row_bits = bits(lines - 1u);
+  if (4 + kl.chir_bits + kl.row_bits <= 32) {
+    kl.regions = 1;
+    kl.dead = (uint32_t)KMC_COUNT_COPIES << (kl.chir_bits + kl.row_bits);
+    kl.key_bits = 4 + kl.chir_bits + kl.row_bits;
+  } else {
+    kl.regions = 0;
+    kl.chir_bits = 0;
+    kl.dead = lines;
+    kl.key_bits = std::min(bits(lines), 32);
+    kl.row_bits = kl.key_bits;
+  }
+  return kl;
+}
+
 contact_args contact_args_of(kmc_ctx* c) {
   contact_args a{};
   a.blocks = c->d_blocks.p;
@@ -2075,13 +2107,12 @@ int kmc_contact_run(kmc_ctx* c, double dt, int64_t n_steps, uint64_t max_hops_pe
   // errors are this call's: a hop cap reached leaves every particle consistent (the one that
   // stopped continues in the next step), so a retry with a larger cap may go on
   KMC_HIP(c, hipMemsetAsync(c->c_err.p, 0, sizeof(int32_t), c->stream));
-  // each step's threads take the live slots in the order of their sites' rows (radix sort of
-  // row lines; dead slots key `lines` and sort last): draws are keyed by particle id and hop
-  // count, so the order changes no result
+  // each step's threads take the live slots in the order of contact_key_layout_of (a radix sort;
+  // dead slots sort last): draws are keyed by particle id and hop count, so the order changes no
+  // result
   const int64_t cap = c->c_cap;
-  const uint32_t lines = (uint32_t)(c->n_units / ROW_UNITS);
-  int key_bits = 1;
-  while (key_bits < 32 && (lines >> key_bits) != 0) ++key_bits;
+  const contact_key_layout kl = contact_key_layout_of(c, (uint32_t)(c->n_units / ROW_UNITS));
+  const int key_bits = kl.key_bits;
   KMC_TRY(dalloc(c, c->c_skeys, 2 * (size_t)cap));
   KMC_TRY(dalloc(c, c->c_sids, 2 * (size_t)cap));
   size_t sort_need = 0;
@@ -2096,8 +2127,8 @@ int kmc_contact_run(kmc_ctx* c, double dt, int64_t n_steps, uint64_t max_hops_pe
     a.max_hops = max_hops_per_step;
     a.pop = c->c_pop_out.p + k * S;
     a.cur = c->c_cur_out.p + k * (S - 1);
-    KMC_HIP(c, launch_contact_keys(cap, c->c_live.p, c->c_site[c->c_buf].p, c->d_roff.p, lines,
-                                   c->c_skeys.p, c->c_sids.p, c->stream));
+    KMC_HIP(c, launch_contact_keys(cap, c->c_live.p, c->c_site[c->c_buf].p, c->d_roff.p,
+                                   c->d_chir.p, kl, c->c_skeys.p, c->c_sids.p, c->stream));
     size_t have = c->c_sort_tmp.n;
     KMC_HIP(c, hipcub::DeviceRadixSort::SortPairs(c->c_sort_tmp.p, have, c->c_skeys.p,
                                                   c->c_skeys.p + cap, c->c_sids.p,

@@ -99,7 +99,7 @@ struct contact_args {
   int32_t* err;             // CONTACT_ERR_* bits
   net_view fv;              // v > 0: links, periods, velocity and blocks for the flights
   unsigned long long* hops_made;  // hops made in contact mode so far (all particles)
-  const uint32_t* perm;     // thread j steps slot perm[j] (slots sorted by row); null: slot j
+  const uint32_t* perm;     // thread j steps slot perm[j] (the sorted slots); null: slot j
 };
 
 struct trace_view {         // trace [n_trace][tr_max] (single network only)
@@ -168,10 +168,20 @@ hipError_t launch_exciton_keys(int64_t n, const int32_t* site, const int64_t* ro
 hipError_t launch_fill_blocks(int64_t n, const site_hdr* hdr, const double* cum,
                               const edge_rec* edge, const int64_t* roff, const double* px,
                               const double* py, const double* pz, uint4* blocks, hipStream_t st);
-// contact slots by the row of their site: keys[i] = row line of slot i's site for i < *n_live,
-// else `dead` (sorts last); ids[i] = i
+// Sort key of a live contact slot: with `regions`, (eighth of the film by row line, chirality of
+// the site, row line) in bit fields [row_bits + chir_bits, +3], [row_bits, +chir_bits], [0,
+// row_bits); without, the row line alone.  Dead slots key `dead`, which sorts last.
+struct contact_key_layout {
+  uint32_t lines;      // row lines of the film (the eighths split [0, lines))
+  int32_t row_bits, chir_bits;
+  int32_t regions;     // 0: the key is the row line alone
+  uint32_t dead;
+  int32_t key_bits;    // bits the radix sort looks at
+};
+// keys[i] = sort key of slot i (contact_key_layout), ids[i] = i
 hipError_t launch_contact_keys(int64_t capacity, const int64_t* n_live, const int32_t* site,
-                               const int64_t* roff, uint32_t dead, uint32_t* keys, uint32_t* ids,
+                               const int64_t* roff, const uint8_t* chir,
+                               const contact_key_layout& kl, uint32_t* keys, uint32_t* ids,
                                hipStream_t st);
 // keys[i] = Morton code of site i's cell, ids[i] = i (block placement order, after a sort)
 hipError_t launch_morton_keys(int64_t n, const int32_t* cell_of, int32_t nx, int32_t ny,

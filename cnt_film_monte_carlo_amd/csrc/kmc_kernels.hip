@@ -1516,11 +1516,23 @@ __global__ __launch_bounds__(256) void k_contact_fly(contact_args a) {
 __global__ __launch_bounds__(256) void k_contact_keys(int64_t capacity, const int64_t* n_live,
                                                       const int32_t* __restrict__ site,
                                                       const int64_t* __restrict__ roff,
-                                                      uint32_t dead, uint32_t* __restrict__ keys,
+                                                      const uint8_t* __restrict__ chir,
+                                                      contact_key_layout kl,
+                                                      uint32_t* __restrict__ keys,
                                                       uint32_t* __restrict__ ids) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= capacity) return;
-  keys[e] = e < *n_live ? (uint32_t)roff[site[e]] : dead;
+  uint32_t k = kl.dead;
+  if (e < *n_live) {
+    const int32_t s = site[e];
+    const uint32_t row = (uint32_t)roff[s];
+    k = row;
+    if (kl.regions) {
+      const uint32_t region = (uint32_t)((uint64_t)row * KMC_COUNT_COPIES / kl.lines);
+      k |= (region << (kl.chir_bits + kl.row_bits)) | ((uint32_t)chir[s] << kl.row_bits);
+    }
+  }
+  keys[e] = k;
   ids[e] = (uint32_t)e;
 }
 
@@ -1530,9 +1542,9 @@ __global__ __launch_bounds__(256) void k_contact_keys(int64_t capacity, const in
 // taking part in the loads).  The next hop is selected as soon as a row arrives and its row load issued
 // before the dwell at the current site is drawn and the clock tested, so the load is in flight
 // meanwhile.  Each particle's arithmetic and its order are those of the oracle's step.  The live
-// slots, sorted by row, are dealt to the XCDs in eighths (from the device-side live count), so
-// each XCD's L2 serves one compact region of the film.  The grid is a multiple of 8 workgroups
-// covering the capacity.
+// slots, sorted by region, chirality and row, are dealt to the XCDs in eighths (from the
+// device-side live count), so each XCD's L2 serves one compact region of the film.  The grid is
+// a multiple of 8 workgroups covering the capacity.
 __global__ __launch_bounds__(256, 4) void k_contact_wave(contact_args a) {
 #pragma clang fp contract(off)
   __shared__ uint4 rows[4 * 64 * ROW_UNITS];  // 8 KB of staged rows per wave
@@ -1941,11 +1953,12 @@ hipError_t launch_contact_step(const contact_args& a, bool fly, hipStream_t st) 
 }
 
 hipError_t launch_contact_keys(int64_t capacity, const int64_t* n_live, const int32_t* site,
-                               const int64_t* roff, uint32_t dead, uint32_t* keys, uint32_t* ids,
+                               const int64_t* roff, const uint8_t* chir,
+                               const contact_key_layout& kl, uint32_t* keys, uint32_t* ids,
                                hipStream_t st) {
   if (capacity == 0) return hipSuccess;
   hipLaunchKernelGGL(k_contact_keys, grid_for(capacity), dim3(256), 0, st, capacity, n_live, site,
-                     roff, dead, keys, ids);
+                     roff, chir, kl, keys, ids);
   return hipGetLastError();
 }
 
