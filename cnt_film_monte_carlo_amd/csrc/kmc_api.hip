@@ -192,11 +192,18 @@ struct kmc_ctx {
   double c_step_ms = 0.0;                 // device time of the step kernels since then
   std::vector<hipEvent_t> c_ev;           // CONTACT_EV_PAIRS (start, stop) pairs, reused
   std::vector<char> c_ev_busy;            // pair recorded and not yet read
+  // live counts after recent steps, copied back to pinned memory (CONTACT_LIVE_LAG slots, each
+  // with the event recorded after its copy)
+  int64_t* c_live_host = nullptr;
+  std::vector<hipEvent_t> c_live_ev;
 };
 
 // step-kernel timing: event pairs reused round robin; a pair is read before it is recorded
 // again, so the host waits only on a step CONTACT_EV_PAIRS steps back
 constexpr int CONTACT_EV_PAIRS = 64;
+// the live count after step k - CONTACT_LIVE_LAG bounds the slots step k's sort, step and
+// compaction cover; the host waits for it only that many steps back
+constexpr int CONTACT_LIVE_LAG = 2;
 
 namespace {
 
@@ -521,6 +528,9 @@ int kmc_destroy(kmc_ctx* c) {
   dfree(c->c_made); dfree(c->c_skeys); dfree(c->c_sids); dfree(c->c_sort_tmp);
   for (hipEvent_t e : c->c_ev)
     if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : c->c_live_ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->c_live_host) (void)hipHostFree(c->c_live_host);
   for (hipEvent_t e : c->hop_ev)
     if (e) (void)hipEventDestroy(e);
   for (int b = 0; b < 2; ++b) {
@@ -2070,6 +2080,12 @@ int kmc_contact_init(kmc_ctx* c, int32_t n_seg, int64_t c1_pop, int64_t c2_pop, 
     for (hipEvent_t& e : c->c_ev) KMC_HIP(c, hipEventCreate(&e));
   }
   c->c_ev_busy.assign(CONTACT_EV_PAIRS, 0);
+  if (c->c_live_ev.empty()) {
+    c->c_live_ev.assign(CONTACT_LIVE_LAG, nullptr);
+    for (hipEvent_t& e : c->c_live_ev) KMC_HIP(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  if (!c->c_live_host)
+    KMC_HIP(c, hipHostMalloc((void**)&c->c_live_host, CONTACT_LIVE_LAG * sizeof(int64_t)));
   KMC_HIP(c, hipMemsetAsync(c->c_live.p, 0, sizeof(int64_t), c->stream));
   KMC_HIP(c, hipMemsetAsync(c->c_next.p, 0, sizeof(uint64_t), c->stream));
   KMC_HIP(c, hipMemsetAsync(c->c_err.p, 0, sizeof(int32_t), c->stream));
@@ -2120,46 +2136,69 @@ int kmc_contact_run(kmc_ctx* c, double dt, int64_t n_steps, uint64_t max_hops_pe
                                                 c->c_skeys.p + cap, c->c_sids.p, c->c_sids.p + cap,
                                                 (int)cap, 0, key_bits, c->stream));
   KMC_TRY(dalloc(c, c->c_sort_tmp, std::max<size_t>(sort_need, 1)));
+  // Step k covers the slots [0, bound): at most the live count, which a step raises by at most
+  // the two contacts' injections.  The count is read exactly here, then after every step into
+  // pinned memory, and step k uses the one after step k - CONTACT_LIVE_LAG.
+  const int64_t inject = c->c_pop[0] + c->c_pop[1];
+  int64_t live0 = 0;
+  KMC_HIP(c, hipMemcpyAsync(c->c_live_host, c->c_live.p, sizeof(int64_t), hipMemcpyDeviceToHost,
+                            c->stream));
+  KMC_HIP(c, hipStreamSynchronize(c->stream));
+  live0 = c->c_live_host[0];
   for (int64_t k = 0; k < n_steps; ++k) {
     // monte_carlo::step, then save_metrics (main.cpp:95-97): every particle to the step's end
+    int64_t bound = live0 + k * inject;
+    if (k >= CONTACT_LIVE_LAG) {
+      const int slot = (int)((k - CONTACT_LIVE_LAG) % CONTACT_LIVE_LAG);
+      KMC_HIP(c, hipEventSynchronize(c->c_live_ev[slot]));
+      bound = c->c_live_host[slot] + (CONTACT_LIVE_LAG - 1) * inject;
+    }
+    bound = std::min(bound, cap);
     contact_args a = contact_args_of(c);
     a.t_end = c->c_time + dt;
     a.max_hops = max_hops_per_step;
     a.pop = c->c_pop_out.p + k * S;
     a.cur = c->c_cur_out.p + k * (S - 1);
-    KMC_HIP(c, launch_contact_keys(cap, c->c_live.p, c->c_site[c->c_buf].p, c->d_roff.p,
+    KMC_HIP(c, launch_contact_keys(bound, c->c_live.p, c->c_site[c->c_buf].p, c->d_roff.p,
                                    c->d_chir.p, kl, c->c_skeys.p, c->c_sids.p, c->stream));
     size_t have = c->c_sort_tmp.n;
-    KMC_HIP(c, hipcub::DeviceRadixSort::SortPairs(c->c_sort_tmp.p, have, c->c_skeys.p,
-                                                  c->c_skeys.p + cap, c->c_sids.p,
-                                                  c->c_sids.p + cap, (int)cap, 0, key_bits,
-                                                  c->stream));
+    if (bound > 0)
+      KMC_HIP(c, hipcub::DeviceRadixSort::SortPairs(c->c_sort_tmp.p, have, c->c_skeys.p,
+                                                    c->c_skeys.p + cap, c->c_sids.p,
+                                                    c->c_sids.p + cap, (int)bound, 0, key_bits,
+                                                    c->stream));
     a.perm = c->c_sids.p + cap;
     const int q = (int)(k % CONTACT_EV_PAIRS);
     KMC_TRY(contact_read_pair(c, q));
     KMC_HIP(c, hipEventRecord(c->c_ev[2 * q], c->stream));
-    KMC_HIP(c, launch_contact_step(a, c->c_vel > 0.0, c->stream));
+    KMC_HIP(c, launch_contact_step(a, c->c_vel > 0.0, bound, c->stream));
     KMC_HIP(c, hipEventRecord(c->c_ev[2 * q + 1], c->stream));
     c->c_ev_busy[q] = 1;
     c->c_time = a.t_end;  // _time += dt (monte_carlo.h:354)
     // repopulate_contacts (monte_carlo.h:443-491): contact 1, then contact n_seg; the kept
     // particles keep their order, the new ones follow them
+    int64_t n_side = bound;  // slots the live ones lie in: contact 1's injection may add some
     for (int side = 0; side < 2; ++side) {
       a.s = contact_buf(c, c->c_buf);
-      KMC_HIP(c, launch_contact_keep(a, c->c_rm_lo[side], c->c_rm_hi[side], c->c_keep.p,
+      KMC_HIP(c, launch_contact_keep(a, n_side, c->c_rm_lo[side], c->c_rm_hi[side], c->c_keep.p,
                                      c->stream));
-      KMC_HIP(c, exclusive_scan(c->c_keep.p, c->c_off.p, c->c_cap, c->c_scan_tmp.p, c->stream));
-      KMC_HIP(c, launch_contact_compact(c->c_cap, contact_buf(c, c->c_buf),
+      KMC_HIP(c, exclusive_scan(c->c_keep.p, c->c_off.p, n_side, c->c_scan_tmp.p, c->stream));
+      KMC_HIP(c, launch_contact_compact(n_side, contact_buf(c, c->c_buf),
                                         contact_buf(c, 1 - c->c_buf), c->c_keep.p, c->c_off.p,
                                         c->stream));
       c->c_buf = 1 - c->c_buf;
       a.s = contact_buf(c, c->c_buf);
-      KMC_HIP(c, launch_contact_inject(a, c->c_off.p + c->c_cap, c->c_pop[side],
+      KMC_HIP(c, launch_contact_inject(a, c->c_off.p + n_side, c->c_pop[side],
                                        c->c_sites.p + c->c_list_off[side], c->c_list_n[side],
                                        c->c_next.p, c->c_time, c->c_live.p, c->stream));
+      n_side = std::min(cap, n_side + c->c_pop[side]);
     }
     KMC_HIP(c, hipMemcpyAsync(c->c_live_out.p + k, c->c_live.p, sizeof(int64_t),
                               hipMemcpyDeviceToDevice, c->stream));
+    const int slot = (int)(k % CONTACT_LIVE_LAG);
+    KMC_HIP(c, hipMemcpyAsync(c->c_live_host + slot, c->c_live.p, sizeof(int64_t),
+                              hipMemcpyDeviceToHost, c->stream));
+    KMC_HIP(c, hipEventRecord(c->c_live_ev[slot], c->stream));
   }
   KMC_HIP(c, hipStreamSynchronize(c->stream));
   for (int q = 0; q < CONTACT_EV_PAIRS; ++q) KMC_TRY(contact_read_pair(c, q));

@@ -211,9 +211,11 @@ hipError_t launch_ens_keys(int64_t n, const net_view* nets, const uint16_t* exc_
                            const int64_t* ebase, uint64_t* keys, uint32_t* ids, uint32_t* live,
                            hipStream_t st);
 hipError_t launch_reduce(const reduce_args& a, hipStream_t st);
-hipError_t launch_contact_step(const contact_args& a, bool fly, hipStream_t st);
-hipError_t launch_contact_keep(const contact_args& a, double y_lo, double y_hi, uint32_t* keep,
-                               hipStream_t st);
+// the step of the live slots; n_max >= *a.n_live bounds the grid
+hipError_t launch_contact_step(const contact_args& a, bool fly, int64_t n_max, hipStream_t st);
+// keep[i] for the slots i < n (n >= *a.n_live; slots past the live count keep 0)
+hipError_t launch_contact_keep(const contact_args& a, int64_t n, double y_lo, double y_hi,
+                               uint32_t* keep, hipStream_t st);
 hipError_t launch_contact_compact(int64_t capacity, const contact_slots& src,
                                   const contact_slots& dst, const uint32_t* keep,
                                   const int64_t* off, hipStream_t st);

@@ -1650,10 +1650,10 @@ __global__ __launch_bounds__(256, 4) void k_contact_wave(contact_args a) {
 }
 
 // repopulate (monte_carlo.h:461-474): a particle whose y lies in [y_lo, y_hi] leaves
-__global__ __launch_bounds__(256) void k_contact_keep(contact_args a, double y_lo, double y_hi,
-                                                      uint32_t* keep) {
+__global__ __launch_bounds__(256) void k_contact_keep(contact_args a, int64_t n, double y_lo,
+                                                      double y_hi, uint32_t* keep) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= a.capacity) return;
+  if (e >= n) return;
   uint32_t k = 0;
   if (e < *a.n_live) {
     const double y = a.s.y[e];
@@ -1939,14 +1939,15 @@ hipError_t launch_fold_counts(int64_t n_lines, int64_t stride, const uint4* bloc
   return hipGetLastError();
 }
 
-hipError_t launch_contact_step(const contact_args& a, bool fly, hipStream_t st) {
-  if (a.capacity == 0) return hipSuccess;
-  if (a.n_seg < 1 || a.n_seg > CONTACT_MAX_SEG) return hipErrorInvalidValue;
+hipError_t launch_contact_step(const contact_args& a, bool fly, int64_t n_max, hipStream_t st) {
+  if (n_max == 0) return hipSuccess;
+  if (a.n_seg < 1 || a.n_seg > CONTACT_MAX_SEG || n_max < 0 || n_max > a.capacity)
+    return hipErrorInvalidValue;
   if (fly)
-    hipLaunchKernelGGL(k_contact_fly, grid_for(a.capacity), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_contact_fly, grid_for(n_max), dim3(256), 0, st, a);
   else
     hipLaunchKernelGGL(k_contact_wave,
-                       dim3((unsigned)((a.capacity + 256 * KMC_COUNT_COPIES - 1) /
+                       dim3((unsigned)((n_max + 256 * KMC_COUNT_COPIES - 1) /
                                        (256 * KMC_COUNT_COPIES) * KMC_COUNT_COPIES)),
                        dim3(256), (size_t)2 * a.n_seg * sizeof(int32_t), st, a);
   return hipGetLastError();
@@ -1962,10 +1963,11 @@ hipError_t launch_contact_keys(int64_t capacity, const int64_t* n_live, const in
   return hipGetLastError();
 }
 
-hipError_t launch_contact_keep(const contact_args& a, double y_lo, double y_hi, uint32_t* keep,
-                               hipStream_t st) {
-  if (a.capacity == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_contact_keep, grid_for(a.capacity), dim3(256), 0, st, a, y_lo, y_hi, keep);
+hipError_t launch_contact_keep(const contact_args& a, int64_t n, double y_lo, double y_hi,
+                               uint32_t* keep, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  if (n < 0 || n > a.capacity) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_contact_keep, grid_for(n), dim3(256), 0, st, a, n, y_lo, y_hi, keep);
   return hipGetLastError();
 }
 
