@@ -145,7 +145,8 @@ struct kmc_ctx {
   dbuf<uint64_t> d_counts64;    // running 64-bit totals the copies are folded into
   int64_t count_stride = 0;
   uint64_t count_pending = 0;   // bound on any copy's count since the last fold
-  dbuf<uint32_t> d_live;        // excitons able to hop after the last launch
+  dbuf<uint32_t> d_live;        // [0] excitons able to hop after the last launch; [1], [2] the
+                                // excitons not idle at a launch's start, used alternately
   int64_t run_launches = 0;     // hop launches of the last kmc_run
   // hop-kernel timing of a timed kmc_run: one (start, stop) event pair per launch
   std::vector<hipEvent_t> hop_ev;
@@ -1061,8 +1062,8 @@ int kmc_inject(kmc_ctx* c, int64_t n_excitons) {
   KMC_TRY(dalloc(c, c->d_counts64, c->n_sites));
   KMC_HIP(c, hipMemsetAsync(c->d_counts64.p, 0, c->n_sites * sizeof(uint64_t), c->stream));
   c->count_pending = 0;
-  KMC_TRY(dalloc(c, c->d_live, 1));
-  KMC_HIP(c, hipMemsetAsync(c->d_live.p, 0, sizeof(uint32_t), c->stream));
+  KMC_TRY(dalloc(c, c->d_live, 3));
+  KMC_HIP(c, hipMemsetAsync(c->d_live.p, 0, 3 * sizeof(uint32_t), c->stream));
   c->run_launches = 0;
   c->n_exc = E;
   c->n_bins = c->bin0 = c->win_bins = 0;
@@ -1220,6 +1221,19 @@ static int run_hops(kmc_ctx* c, hop_args a, int64_t hops, int64_t chunk, bool co
     c->sort_tmp_bytes = need;
   }
   a.perm = c->d_ids.p + E;
+  // A clock-bounded run leaves out of each launch the excitons whose next hop falls after t_max
+  // (with no MSD record due before it): they sort after the others, which the hop kernel alone
+  // deals to the XCDs, so the waves hold excitons that still hop.
+  const bool leave_idle = !fly && std::isfinite(a.t_max);
+  idle_test it{};
+  if (leave_idle) {
+    it.t_next = c->x_t.p;
+    it.t_max = a.t_max;
+    it.k_rec = sample ? c->x_krec.p : nullptr;
+    it.n_bins = a.v.n_bins;
+    it.dt_sample = a.v.dt_sample;
+    KMC_HIP(c, hipMemsetAsync(c->d_live.p + 1, 0, 2 * sizeof(uint32_t), c->stream));
+  }
   const int64_t step = chunk > 0 ? chunk : std::max<int64_t>(hops, 1);
   int64_t done = 0, launches = 0;
   int64_t movable = E;  // an upper bound on the excitons a launch can move (the last live count)
@@ -1235,7 +1249,12 @@ static int run_hops(kmc_ctx* c, hop_args a, int64_t hops, int64_t chunk, bool co
       if (c->count_pending + add > COUNT_COPY_MAX) KMC_TRY(fold_counts(c, c->stream));
       c->count_pending += add;
     }
-    KMC_HIP(c, launch_exciton_keys(E, c->x_site.p, c->d_roff.p, c->d_keys.p, c->d_ids.p,
+    if (leave_idle) {
+      it.alive = c->d_live.p + 1 + (launches & 1);
+      it.alive_next = c->d_live.p + 1 + ((launches + 1) & 1);
+      a.n_alive = it.alive;
+    }
+    KMC_HIP(c, launch_exciton_keys(E, c->x_site.p, c->d_roff.p, it, c->d_keys.p, c->d_ids.p,
                                    c->d_live.p, c->stream));
     size_t have = c->sort_tmp_bytes;
     KMC_HIP(c, hipcub::DeviceRadixSort::SortPairs(c->d_sort_tmp.p, have, c->d_keys.p,

@@ -115,6 +115,9 @@ struct hop_args {
   double t_max;
   const uint32_t* perm;     // exciton processed at each position (NULL = identity)
   trace_view tv;
+  // k_hop: positions [0, *n_alive) of perm (the idle excitons sorted after them are left out);
+  // null: all v.x.n
+  const uint32_t* n_alive;
 };
 
 struct ens_args {
@@ -162,9 +165,21 @@ hipError_t launch_block_sizes(int64_t n, const site_hdr* hdr, const int32_t* ord
                               hipStream_t st);
 hipError_t launch_place_blocks(int64_t n, const int32_t* order, const int64_t* first, int64_t* roff,
                                hipStream_t st);
-// keys[e] = first line of exciton e's block, ids[e] = e; *live = 0 (the next launch's count)
-hipError_t launch_exciton_keys(int64_t n, const int32_t* site, const int64_t* roff, uint32_t* keys,
-                               uint32_t* ids, uint32_t* live, hipStream_t st);
+// Which excitons a clock-bounded launch can leave out (k_exciton_keys); alive null: none.
+struct idle_test {
+  const double* t_next;
+  double t_max;
+  const uint32_t* k_rec;  // null: the launch records no MSD samples
+  int32_t n_bins;         // the sample window's end on the grid
+  double dt_sample;
+  uint32_t* alive;        // += the excitons not idle (zero on entry)
+  uint32_t* alive_next;   // set to zero for the next launch
+};
+// keys[e] = first line of exciton e's block (idle excitons: 0xFFFFFFFF, after every line),
+// ids[e] = e; *live = 0 (the next launch's count)
+hipError_t launch_exciton_keys(int64_t n, const int32_t* site, const int64_t* roff,
+                               const idle_test& it, uint32_t* keys, uint32_t* ids, uint32_t* live,
+                               hipStream_t st);
 hipError_t launch_fill_blocks(int64_t n, const site_hdr* hdr, const double* cum,
                               const edge_rec* edge, const int64_t* roff, const double* px,
                               const double* py, const double* pz, uint4* blocks, hipStream_t st);

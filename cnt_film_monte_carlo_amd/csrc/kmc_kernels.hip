@@ -371,16 +371,42 @@ __global__ __launch_bounds__(256) void k_place_blocks(int64_t n, const int32_t* 
 }
 
 // Sort keys of the excitons: the first line of their current site's block.
+// An exciton is idle in a launch if its next hop falls after t_max and no MSD record falls due
+// before that hop (the hop kernel would only reload and store its state).  Idle excitons key
+// IDLE_KEY and sort after every block line; `alive` counts the others (the hop kernel deals only
+// those positions to the XCDs) and `alive_next`, the next launch's count, starts at zero.
+constexpr uint32_t IDLE_KEY = 0xFFFFFFFFu;
 __global__ __launch_bounds__(256) void k_exciton_keys(int64_t n, const int32_t* __restrict__ site,
                                                       const int64_t* __restrict__ roff,
-                                                      uint32_t* __restrict__ keys,
+                                                      idle_test it, uint32_t* __restrict__ keys,
                                                       uint32_t* __restrict__ ids,
                                                       uint32_t* __restrict__ live) {
+#pragma clang fp contract(off)
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e == 0 && live) *live = 0u;
-  if (e >= n) return;
-  keys[e] = (uint32_t)roff[site[e]];
-  ids[e] = (uint32_t)e;
+  if (e == 0) {
+    if (live) *live = 0u;
+    if (it.alive_next) *it.alive_next = 0u;
+  }
+  bool alive = false;
+  if (e < n) {
+    uint32_t key = (uint32_t)roff[site[e]];
+    if (it.alive) {
+      const double t = it.t_next[e];
+      bool idle = !(t <= it.t_max);
+      if (idle && it.k_rec) {  // the hop kernel's record test: t_rec = (k + 1) dt_sample < t
+        const uint32_t k = it.k_rec[e];
+        idle = !(k < (uint32_t)it.n_bins && (double)(k + 1u) * it.dt_sample < t);
+      }
+      if (idle) key = IDLE_KEY;
+      alive = !idle;
+    }
+    keys[e] = key;
+    ids[e] = (uint32_t)e;
+  }
+  if (it.alive) {
+    const uint64_t b = __ballot(alive);
+    if ((threadIdx.x & 63u) == 0u && b != 0ull) atomicAdd(it.alive, (uint32_t)__popcll(b));
+  }
 }
 
 __global__ __launch_bounds__(256) void k_pack_rows(int64_t n, const site_hdr* __restrict__ hdr,
@@ -1170,10 +1196,23 @@ template <bool COUNT, bool SAMPLE, bool TRACE>
 __global__ __launch_bounds__(256, 4) void k_hop(hop_args a) {
   __shared__ uint4 rows[4 * 64 * ROW_UNITS];  // 8 KB of staged rows per wave
   __shared__ uint32_t win[COUNT_WINDOW_WORDS]; // 8 KB: the workgroup's departure-count window
-  const uint32_t pos = xcd_position();
+  // the positions taken: all excitons, or the ones not idle in a clock-bounded launch (sorted
+  // first), dealt in eighths to the XCDs like all of them
+  uint32_t pos, n;
+  if (a.n_alive) {
+    n = *a.n_alive;
+    const uint32_t per_xcd = ((n + 255u) / 256u + KMC_COUNT_COPIES - 1) / KMC_COUNT_COPIES;
+    const uint32_t rank = blockIdx.x / KMC_COUNT_COPIES;
+    pos = rank < per_xcd
+              ? ((blockIdx.x % KMC_COUNT_COPIES) * per_xcd + rank) * blockDim.x + threadIdx.x
+              : 0xFFFFFFFFu;
+  } else {
+    n = (uint32_t)a.v.x.n;
+    pos = xcd_position();
+  }
   const uint32_t lane = threadIdx.x & 63u;
-  const bool wave_live = __builtin_amdgcn_readfirstlane(pos - lane) < a.v.x.n;
-  const bool valid = pos < a.v.x.n;
+  const bool wave_live = __builtin_amdgcn_readfirstlane(pos - lane) < n;
+  const bool valid = pos < n;
   const uint32_t e = valid ? (a.perm ? a.perm[pos] : pos) : 0u;
   uint4* lds_wave = rows + 64u * ROW_UNITS * __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if constexpr (!COUNT) {
@@ -1187,7 +1226,7 @@ __global__ __launch_bounds__(256, 4) void k_hop(hop_args a) {
     for (uint32_t i = threadIdx.x; i < COUNT_WINDOW_WORDS; i += blockDim.x) win[i] = 0u;
     const uint32_t p0 = pos - threadIdx.x;
     uint32_t base = 0u;
-    if (p0 < a.v.x.n) {
+    if (p0 < n) {
       const uint32_t e0 = a.perm ? a.perm[p0] : p0;
       const uint32_t L0 = (uint32_t)a.v.roff[a.v.x.site[e0]];
       base = L0 > COUNT_WINDOW_LINES / 2 ? L0 - COUNT_WINDOW_LINES / 2 : 0u;
@@ -1799,10 +1838,12 @@ hipError_t launch_place_blocks(int64_t n, const int32_t* order, const int64_t* f
   return hipGetLastError();
 }
 
-hipError_t launch_exciton_keys(int64_t n, const int32_t* site, const int64_t* roff, uint32_t* keys,
-                               uint32_t* ids, uint32_t* live, hipStream_t st) {
+hipError_t launch_exciton_keys(int64_t n, const int32_t* site, const int64_t* roff,
+                               const idle_test& it, uint32_t* keys, uint32_t* ids, uint32_t* live,
+                               hipStream_t st) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_exciton_keys, grid_for(n), dim3(256), 0, st, n, site, roff, keys, ids, live);
+  hipLaunchKernelGGL(k_exciton_keys, grid_for(n), dim3(256), 0, st, n, site, roff, it, keys, ids,
+                     live);
   return hipGetLastError();
 }
 
