@@ -1075,6 +1075,10 @@ __device__ __forceinline__ void hop_wave(const net_view& v, uint32_t e, bool val
 
   coop_issue(v.blocks, W, lds_wave, lane);
   while (true) {
+    // the wave issues at raised priority from its row's arrival until the next row load is out
+    // (the stretch its exciton's hops wait on); the dwell and records of (B) and (C) run at the
+    // normal one, so the SIMD's other waves reach their own loads first
+    __builtin_amdgcn_s_setprio(2);
     const coop_row row = coop_finish(lds_wave, lane);
     if (COUNT && count_line != COUNT_NONE) count_departure();
     // (A) the hop from this site, if the exciton may still hop (scatterer::update_state,
@@ -1094,6 +1098,7 @@ __device__ __forceinline__ void hop_wave(const net_view& v, uint32_t e, bool val
     // a lane whose exciton has stopped never reads its row again: it loads block line 0, which
     // every such lane of the wave shares, instead of its own row
     coop_issue(v.blocks, active ? W_next : 0u, lds_wave, lane);
+    __builtin_amdgcn_s_setprio(0);
     // (B) the hop into this site finishes: dwell at the new site (particle.cpp:183 -> ff_time)
     if (pend) {
       dx = dx + min_image((double)row.x - (double)px, v.half_per[0]);
@@ -1626,6 +1631,7 @@ __global__ __launch_bounds__(256, 4) void k_contact_wave(contact_args a) {
     hop_words hw;
     coop_issue(a.blocks, W, lds_wave, lane);
     while (true) {
+      __builtin_amdgcn_s_setprio(2);  // raised until the next row load is out, as in hop_wave
       const coop_row row = coop_finish(lds_wave, lane);
       // (A) the hop from this site, selected before the clock test that decides whether it is
       // made, and the load of the row it leads to, in flight while (B) and (C) run
@@ -1644,6 +1650,7 @@ __global__ __launch_bounds__(256, 4) void k_contact_wave(contact_args a) {
       // a lane whose particle has stopped never reads its row again: it loads block line 0,
       // which every such lane of the wave shares, instead of its own row
       coop_issue(a.blocks, active ? W_next : 0u, lds_wave, lane);
+      __builtin_amdgcn_s_setprio(0);
       // (B) the dwell at this site of the hop that led here
       if (pend) {
         t = t + dwell_time(row.n, row.gamma, r_dw);
