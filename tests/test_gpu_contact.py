@@ -149,6 +149,34 @@ def test_contact_errors(kmc, nw):
     ctx.close()
 
 
+def test_contact_near_capacity(kmc, orc, nw):
+    """Both contacts held at 100 particles in 4 sections: with a capacity of 410 the live count
+    reaches 408 and the run equals the oracle bit for bit (the steps' slot range is bounded by
+    the live count and clamped at the capacity); with 400, the injections overflow it, which the
+    oracle reports too, and the call fails with ERR_CAPACITY."""
+    say("contact mode near the capacity: 4 sections, 100/100, capacities 410 and 400")
+    w, net, ctx, tab = _film(kmc, nw, seed=29)
+    ctx.contact_init(4, 100, 100, 410)
+    pop, cur, live = ctx.contact_run(2e-8, 6)
+    onet = orc.Net(net, w.cutoff, 0.4 * NM)
+    onet.set_table(*tab)
+    ref = orc.contact(net, onet.csr(0), 29, 4, 100, 100, 410, 2e-8, 6)
+    assert np.array_equal(live, ref["n_live"]) and live.max() == 408
+    assert np.array_equal(pop, ref["population"]) and np.array_equal(cur, ref["crossings"])
+    p = ctx.contact_particles()
+    assert np.array_equal(p["id"], ref["id"]) and np.array_equal(p["site"], ref["site"])
+    assert np.array_equal(p["hops"], ref["hops"])
+    ctx.close()
+    with pytest.raises(RuntimeError):
+        orc.contact(net, onet.csr(0), 29, 4, 100, 100, 400, 2e-8, 6)
+    w, net, ctx, tab = _film(kmc, nw, seed=29)
+    ctx.contact_init(4, 100, 100, 400)
+    with pytest.raises(kmc.KmcError) as e:
+        ctx.contact_run(2e-8, 6)
+    assert e.value.code == kmc.ERR_CAPACITY and "capacity" in str(e.value)
+    ctx.close()
+
+
 def test_fick_known_diffusion(kmc):
     # simple cubic lattice, spacing a, cutoff 1.2 a (nearest neighbours only, the face
     # diagonals at 1.41 a excluded), rate k to each: along y the walk has D = k a^2
