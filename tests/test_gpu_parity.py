@@ -168,6 +168,40 @@ def test_time_cap(kmc, orc, nw, small):
     ctx.close()
 
 
+def test_time_cap_in_calls_with_trace_and_samples(kmc, orc, nw, small):
+    """A clock-bounded run in three calls of rising t_max with the trace and MSD samples on.
+    Each launch leaves out the excitons whose next hop falls after t_max and that owe no record
+    before it (they sort last and the hop kernel deals only the others); state, per-site counts,
+    trace and records equal the oracle's."""
+    say("t_max in three calls with trace and samples: film_5k_sites 512 excitons")
+    ctx = _ctx(kmc, nw, small, seed=17)
+    E = 512
+    ctx.inject(E)
+    csr = ctx.get_csr()
+    ox = orc.Excitons(csr, small["net"]["flags"], seed=17, n_excitons=E)
+    T = float(np.median(ox.t_next)) * 300
+    for o in (ctx, ox):
+        o.set_samples(24, T)
+        o.set_trace(8, 50_000)
+    for f in (0.25, 0.6, 1.0):
+        ctx.run(1_000_000, t_max=f * T)
+        ox.run(1_000_000, t_max=f * T)
+        _compare_state(ctx.get_excitons(), ox, f"t_max {f} T")
+    assert np.array_equal(ctx.get_hop_counts(), ox.site_count)
+    tr = ctx.get_trace()
+    assert np.array_equal(tr["n_hops"], ox.tr_n)
+    assert np.array_equal(tr["sites"], ox.tr_site)
+    np.testing.assert_allclose(tr["times"], ox.tr_time, rtol=T_REL, atol=0)
+    s = ctx.get_samples()
+    assert np.array_equal(s["n_recorded"], ox.k_rec)
+    valid = np.arange(24)[:, None] < ox.k_rec[None, :]
+    assert np.array_equal(s["disp"][valid], ox.rec_disp[valid])
+    assert np.array_equal(s["drains"][valid], ox.rec_drain[valid])
+    g = ctx.get_excitons()
+    say(f"  {int(g['hops'].sum())} hops, {int(valid.sum())} records")
+    ctx.close()
+
+
 def test_trapped_exciton(kmc, orc):
     say("trapped: isolated source site")
     # site 0 is isolated (no neighbour within the cutoff) and the only source
