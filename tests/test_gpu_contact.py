@@ -209,6 +209,44 @@ def test_fick_known_diffusion(kmc):
     ctx.close()
 
 
+def test_contact_full_size_matches_oracle(kmc, orc, nw):
+    """The 625k-site film with three chiralities at the bench's populations (20 sections,
+    100,000 particles held in contact 1, ~1 M live) for three short steps (5 mean dwell times,
+    ~30 hops per particle and step): populations, crossings, live counts, every particle and the
+    hop total equal the oracle's.  This runs the full-size slot sort by region, chirality and row,
+    the live-count bound and the wave kernel's XCD dealing against the literal restatement."""
+    say("contact mode, 625k-site film vs oracle: 3 steps of 5 mean dwell times")
+    w = nw.film_large()
+    net = nw.build_network(w)
+    net["flags"] = np.zeros(net["n"], np.uint8)
+    tab = nw.forster_tables(3)
+    ctx = kmc.Context(kmc.Config(cutoff=w.cutoff, min_distance=0.4 * NM, seed=77))
+    ctx.set_sites(net["x"], net["y"], net["z"], net["ox"], net["oy"], net["oz"], net["left"],
+                  net["right"], net["chirality"], net["flags"])
+    ctx.set_rate_table(*tab)
+    ctx.build_cells()
+    ctx.build_rates(kmc.RATE_TABLE)
+    g = ctx.get_csr()
+    gam = g["cum"][g["row_ptr"][1:][np.diff(g["row_ptr"]) > 0] - 1]
+    dt = 5.0 * float(np.mean(1.0 / gam))
+    ctx.contact_init(20, 100_000, 0, 4_000_000)
+    pop, cur, live = ctx.contact_run(dt, 3)
+    onet = orc.Net(net, w.cutoff, 0.4 * NM)
+    onet.set_table(*tab)
+    ref = orc.contact(net, onet.csr(0), 77, 20, 100_000, 0, 4_000_000, dt, 3)
+    assert np.array_equal(pop, ref["population"])
+    assert np.array_equal(cur, ref["crossings"])
+    assert np.array_equal(live, ref["n_live"])
+    p = ctx.contact_particles()
+    assert np.array_equal(p["id"], ref["id"]) and np.array_equal(p["site"], ref["site"])
+    assert np.array_equal(p["hops"], ref["hops"])
+    np.testing.assert_allclose(p["t_next"], ref["t_next"], rtol=1e-12)
+    made, _ = ctx.contact_stats()
+    assert made == ref["hops_made"]
+    say(f"  live {live[-1]}, hops {made}")
+    ctx.close()
+
+
 def test_contact_full_size_split_equals_whole(kmc, nw):
     """The 625k-site film at the bench's contact workload (20 sections, 100,000 particles held in
     contact 1, ~1 M live): a run split in two calls equals one call bit for bit (populations,
