@@ -202,6 +202,37 @@ def test_time_cap_in_calls_with_trace_and_samples(kmc, orc, nw, small):
     ctx.close()
 
 
+def test_large_film_time_cap(kmc, orc, nw):
+    """The 625k-site film, 10^4 excitons run to t_max = 300 median dwell times in two calls with
+    MSD samples on (up to ~4e4 hops for one exciton, so most launches leave most excitons out as
+    idle and deal the rest over the XCDs): state, per-site counts and records equal the oracle's."""
+    say("large film, clock-bounded: 10^4 excitons to 300 median dwell times")
+    w = nw.film_large()
+    net = nw.build_network(w)
+    th, z, a1, a2, rates = nw.forster_tables(3)
+    ctx = nw.make_context(w, net, seed=23)
+    ctx.set_rate_table(th, z, a1, a2, rates)
+    ctx.build_cells()
+    ctx.build_rates(kmc.RATE_TABLE)
+    E = 10_000
+    ctx.inject(E)
+    ox = orc.Excitons(ctx.get_csr(), net["flags"], seed=23, n_excitons=E)
+    T = float(np.median(ox.t_next)) * 300
+    ctx.set_samples(16, T)
+    ox.set_samples(16, T)
+    for f in (0.5, 1.0):
+        ctx.run(10_000_000, t_max=f * T)
+        ox.run(10_000_000, t_max=f * T)
+        _compare_state(ctx.get_excitons(), ox, f"large film, t_max {f} T")
+    assert np.array_equal(ctx.get_hop_counts(), ox.site_count)
+    s = ctx.get_samples()
+    assert np.array_equal(s["n_recorded"], ox.k_rec)
+    valid = np.arange(16)[:, None] < ox.k_rec[None, :]
+    assert np.array_equal(s["disp"][valid], ox.rec_disp[valid])
+    say(f"  {int(ox.hops.sum())} hops, launches {ctx.run_stats()[0]}")
+    ctx.close()
+
+
 def test_trapped_exciton(kmc, orc):
     say("trapped: isolated source site")
     # site 0 is isolated (no neighbour within the cutoff) and the only source
