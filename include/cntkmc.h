@@ -164,7 +164,7 @@ int kmc_next_sample_window(kmc_ctx* ctx);
  * max_hops hops (track_particle, src/monte_carlo/monte_carlo.h:786-818). */
 int kmc_set_trace(kmc_ctx* ctx, int64_t n_trace, int64_t max_hops);
 
-/* Job (c): persistent hop kernel.  Replaces the kubo_step loop
+/* Job (c): the hop kernel, in sub-launches (kmc_set_hop_schedule).  Replaces the kubo_step loop
  * (src/monte_carlo/monte_carlo.cpp:319-342) with particle::step / update_state / ff_time
  * (src/monte_carlo/particle.cpp:57-80, scatterer.cpp:9-31, scatterer.h:74-80), event driven:
  * every exciton performs up to hops_per_exciton further hops, stopping early at the first hop
@@ -174,7 +174,9 @@ int kmc_set_trace(kmc_ctx* ctx, int64_t n_trace, int64_t max_hops);
  * time of the whole pass (all sub-launches and the exciton sorts between them), measured with
  * HIP events on the context stream.  The run ends early once no exciton can hop any more (every
  * one past t_max or trapped), so a budget far beyond the hops that t_max allows (the reference's
- * while (time < max_time) loop, src/main.cpp:64-76) returns after the hops that happen. */
+ * while (time < max_time) loop, src/main.cpp:64-76) returns after the hops that happen; the
+ * excitons already past t_max are left out of the later sub-launches.  Neither changes any
+ * trajectory. */
 int kmc_run(kmc_ctx* ctx, int64_t hops_per_exciton, double t_max, int32_t count_sites,
             double* kernel_ms);
 /* After kmc_run: the hop launches it made and the excitons still able to hop (t <= t_max and
@@ -232,17 +234,19 @@ int kmc_contact_init(kmc_ctx* ctx, int32_t n_seg, int64_t c1_pop, int64_t c2_pop
  * the particles' positions at the start and the end of the step (save_currents: +1 upward,
  * -1 downward); then the contacts are repopulated and n_live[step] receives the particle
  * count.  Densities and currents are these over (area dy) and (interface area dt), as the
- * reference writes them.  Any output may be NULL. */
+ * reference writes them.  Any output may be NULL.  The call returns once every step is done;
+ * while queueing them the host stays at most two steps ahead of the device (it reads each
+ * step's live count back to bound the slots the following steps cover). */
 int kmc_contact_run(kmc_ctx* ctx, double dt, int64_t n_steps, uint64_t max_hops_per_step,
                     int32_t* population, int32_t* crossings, int64_t* n_live);
-/* area[n_seg] (get_area: x extent times z extent of the section's sites), ymin, dy and the
- * simulated time; any output may be NULL. */
 /* Since kmc_contact_init: the hops made by all particles (also those since removed at the
  * contacts) and the device time of the step kernels (particle::step of every particle, HIP
  * events around each step's launch), without the repopulation.  No counterpart in the
  * reference, which times nothing; a throughput measurement of the contact mode.  Either
  * output may be NULL. */
 int kmc_contact_stats(kmc_ctx* ctx, uint64_t* hops, double* step_kernel_ms);
+/* area[n_seg] (get_area: x extent times z extent of the section's sites), ymin, dy and the
+ * simulated time; any output may be NULL. */
 int kmc_contact_sections(const kmc_ctx* ctx, double* area, double* ymin, double* dy,
                          double* time);
 /* The live particles in slot order: id, site, time of the next hop, hops made and y (FP64).
